@@ -1,0 +1,143 @@
+"""Benchmark: VTCS-2 step throughput on one MI355X.
+
+Workload: 2-D Cartesian open-ocean propagation, 4096 x 4096 FP32 grid,
+dx = dy = 500 m, depth 4000 m minus seeded fractal seamounts (never dry),
+Gaussian hump (1 m, sigma 20 km), zero velocity, open boundaries,
+non-dispersive; default 10,000 timed steps.  One step = x-sweep over rows,
+y-sweep over columns, max-wave update (cliffs_main.f:317-355).  State is
+resident in HBM before the timed region; no host copies inside it.
+
+Prints ONE JSON line (see DESIGN.md for the roofline bookkeeping).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cliffs-src_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+METRIC = ("Gcell-updates/s & %HBM roofline; ensemble scenarios/hr at ≤2% "
+          "max-runup error")
+HBM_PEAK_GBS = 8000.0
+X_BYTES = 28   # x-sweep: read s, cp, u, v; write cp, u, v (FP32)
+Y_BYTES = 44   # y-sweep: the same 28 B + fused max-wave 16 B (read+write hmax, velmax)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n, seed, steps):
+    """FP64 C restatement (oracle/) on the host cores, on a bounded sample of
+    the same workload: `steps` steps of the full n x n grid."""
+    import numpy as np  # noqa: F401
+    import oracle
+    import synth
+    if not os.path.exists(oracle.ORACLE_SO):
+        oracle.build(with_ref=False)
+    case = synth.open_ocean(n=n, seed=seed)
+    s1, s2, zeta = case.metrics()
+    o = oracle.Oracle(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground)
+    o.set_state(case.cel, case.xvel, case.yvel)
+    o.steps(1)
+    t0 = time.perf_counter()
+    o.steps(steps)
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": n * n * steps / dt / 1e9, "unit": "Gcell-updates/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{steps} steps of the {n}x{n} open-ocean grid after 1 warm-up step, "
+                      f"FP64 C restatement of cliffs.f with OpenMP over lines, "
+                      f"{dt:.1f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--timing-steps", type=int, default=200,
+                    help="steps of the per-kernel HIP-event pass")
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU; the path shards by scenario or by "
+              "y-slab across GPUs (DESIGN.md), but no multi-GPU run is made here",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import cliffs_gpu
+    import synth
+
+    n = args.n
+    log(f"[bench] generating {n}x{n} open-ocean case, seed {args.seed}")
+    case = synth.open_ocean(n=n, seed=args.seed, steps=args.steps)
+    s1, s2, zeta = case.metrics()
+    g = cliffs_gpu.Grid(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground,
+                        crough=case.crough, device=0)
+    g.upload(case.cel, case.xvel, case.yvel)
+    log(f"[bench] warmup {args.warmup} steps")
+    g.run(args.warmup)
+    g.sync()
+
+    log(f"[bench] timed {args.steps} steps")
+    t0 = time.perf_counter()
+    g.run(args.steps)
+    g.sync()
+    wall = time.perf_counter() - t0
+    cells = n * n
+    value = cells * args.steps / wall / 1e9
+    log(f"[bench] {wall:.3f} s, {value:.2f} Gcell-updates/s")
+
+    log(f"[bench] per-kernel HIP-event pass, {args.timing_steps} steps")
+    ms_tot, ms_x, ms_y = g.run_timed(args.timing_steps)
+    c, u, v = g.download()
+    finite = bool(np.isfinite(c).all() and np.isfinite(u).all())
+    g.close()
+
+    ach_x = X_BYTES * cells / (ms_x * 1e-3) / 1e9
+    ach_y = Y_BYTES * cells / (ms_y * 1e-3) / 1e9
+    dom = "sweep_y" if ms_y >= ms_x else "sweep_x"
+    ach, byt, ms = (ach_y, Y_BYTES, ms_y) if dom == "sweep_y" else (ach_x, X_BYTES, ms_x)
+    step_bw56 = 56 * value
+    roof = {
+        "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+        "kernel": dom, "bytes_per_cell": byt, "ms_per_launch": round(ms, 4),
+        "other_kernel": {"sweep_x" if dom == "sweep_y" else "sweep_y": {
+            "ms_per_launch": round(ms_x if dom == "sweep_y" else ms_y, 4),
+            "achieved": round(ach_x if dom == "sweep_y" else ach_y, 1),
+            "bytes_per_cell": X_BYTES if dom == "sweep_y" else Y_BYTES}},
+        "step_56B_frac": round(step_bw56 / HBM_PEAK_GBS, 4),
+        "step_72B_frac": round(72 * value / HBM_PEAK_GBS, 4),
+    }
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] CPU baseline: {args.cpu_steps} steps of the FP64 oracle")
+        cpu = cpu_baseline(n, args.seed, args.cpu_steps)
+        log(f"[bench] CPU {cpu['value']:.4f} Gcell-updates/s on {cpu['cores']} cores")
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"2D Cartesian open ocean {n}x{n}, dx=dy=500 m, depth 4000 m "
+                               f"minus fractal seamounts <=2500 m, Gaussian hump 1 m sigma 20 km, "
+                               f"open boundaries, non-dispersive, seed {args.seed}",
+                   "nx": n, "ny": n, "dt_s": round(case.dt, 6), "state_finite": finite},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

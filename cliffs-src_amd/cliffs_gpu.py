@@ -1,0 +1,223 @@
+"""Python binding of the cliffs_gpu C ABI (include/cliffs_gpu.h) via ctypes.
+
+Mirrors the reference's call surface for the hot path: a grid handle owning
+the MASTER/PARAMETERS state (global_modules.f:16-55), ``sweep`` for one
+OpenMP loop of ``cliffs(iu,irw,n)`` calls (cliffs.f:17, cliffs_main.f:325-329,
+340-344), ``maxwave`` (cliffs_main.f:347-355) and ``step``/``run`` for whole
+time steps.  Arrays are numpy float64 of shape (ny, nx) -- the reference's
+Fortran (nx, ny) bytes.
+
+There is no CPU fallback: constructing a ``Grid`` without the built HIP
+library, or on a machine without a HIP device, raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcliffs_gpu.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "cliffs_gpu.h")
+
+OK = 0
+ERR = {-1: "CLIFFS_ERR_ARG", -2: "CLIFFS_ERR_HIP", -3: "CLIFFS_ERR_NOMEM",
+       -4: "CLIFFS_ERR_METRIC", -5: "CLIFFS_ERR_NODEVICE", -6: "CLIFFS_ERR_STATE"}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class CliffsError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERR.get(code, code)}: {msg}")
+        self.code = code
+
+
+class _Desc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("dep", _dp), ("s1", _dp), ("s2", _dp),
+                ("zeta", _dp), ("dt", C.c_double), ("ground", C.c_double),
+                ("crough", C.c_double), ("device", C.c_int)]
+
+
+_lib = None
+
+
+def load_library():
+    """Load libcliffs_gpu.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with cliffs-src_amd/build.sh "
+            "(or __graft_entry__.build()); there is no CPU fallback")
+    lib = C.CDLL(LIB_PATH)
+    H = C.c_void_p
+    sig = {
+        "cliffs_create": ([C.POINTER(_Desc), C.POINTER(H)], C.c_int),
+        "cliffs_destroy": ([H], C.c_int),
+        "cliffs_upload_state": ([H, _dp, _dp, _dp], C.c_int),
+        "cliffs_download_state": ([H, _dp, _dp, _dp], C.c_int),
+        "cliffs_download_max": ([H, _dp, _dp], C.c_int),
+        "cliffs_reset_max": ([H], C.c_int),
+        "cliffs_sweep": ([H, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int], C.c_int),
+        "cliffs_maxwave": ([H], C.c_int),
+        "cliffs_step": ([H, _dp, _dp, _dp, _dp, C.c_int], C.c_int),
+        "cliffs_run": ([H, C.c_int], C.c_int),
+        "cliffs_run_timed": ([H, C.c_int, _dp, _dp, _dp], C.c_int),
+        "cliffs_gauges": ([H, C.c_int, _ip, _ip, _dp, _dp, _dp], C.c_int),
+        "cliffs_sync": ([H], C.c_int),
+        "cliffs_stream": ([H, C.POINTER(C.c_void_p)], C.c_int),
+        "cliffs_info": ([H, _dp, _dp, _ip, _ip, _ip, _ip], C.c_int),
+        "cliffs_last_error": ([], C.c_char_p),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = lib
+    return lib
+
+
+def header_symbols(path: str = HEADER):
+    """Names of the functions declared in include/cliffs_gpu.h."""
+    import re
+    txt = open(path).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(cliffs_\w+)\s*\(", txt, re.M)))
+
+
+def _chk(rc: int):
+    if rc != OK:
+        raise CliffsError(rc, load_library().cliffs_last_error().decode())
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None and a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+class Grid:
+    """One grid resident on a GPU (the reference's MASTER state)."""
+
+    def __init__(self, dep, s1=None, s2=None, zeta=None, *, dt, ground, crough=0.0, device=0):
+        lib = load_library()
+        dep = _f64(dep)
+        ny, nx = dep.shape
+        self.nx, self.ny = nx, ny
+        keep = [dep]
+        s1a = _f64(s1) if s1 is not None and nx > 2 else None
+        s2a = _f64(s2) if s2 is not None and ny > 2 else None
+        za = _f64(zeta) if zeta is not None and ny > 2 else None
+        keep += [s1a, s2a, za]
+        d = _Desc(nx, ny, _p(dep), _p(s1a), _p(s2a), _p(za), float(dt), float(ground),
+                  float(crough), int(device))
+        h = C.c_void_p()
+        _chk(lib.cliffs_create(C.byref(d), C.byref(h)))
+        self._h = h
+        self._keep = keep
+        g, cm = C.c_double(), C.c_double()
+        r = [C.c_int() for _ in range(4)]
+        _chk(lib.cliffs_info(h, C.byref(g), C.byref(cm), *[C.byref(x) for x in r]))
+        self.grav, self.celmin = g.value, cm.value
+        self.ii1, self.ii2, self.jj1, self.jj2 = [x.value for x in r]
+        self.dep = dep
+        self.mx = max(ny if nx > 2 else 1, nx if ny > 2 else 1)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().cliffs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- state --------------------------------------------------------------
+    def upload(self, cel, xvel, yvel):
+        shp = (self.ny, self.nx)
+        c, u, v = _f64(cel, shp), _f64(xvel, shp), _f64(yvel, shp)
+        _chk(load_library().cliffs_upload_state(self._h, _p(c), _p(u), _p(v)))
+
+    def download(self):
+        out = [np.empty((self.ny, self.nx)) for _ in range(3)]
+        _chk(load_library().cliffs_download_state(self._h, *[_p(a) for a in out]))
+        return tuple(out)
+
+    def download_max(self):
+        out = [np.empty((self.ny, self.nx)) for _ in range(2)]
+        _chk(load_library().cliffs_download_max(self._h, *[_p(a) for a in out]))
+        return tuple(out)
+
+    def reset_max(self):
+        _chk(load_library().cliffs_reset_max(self._h))
+
+    # -- compute ------------------------------------------------------------
+    @staticmethod
+    def _edge(e, mx):
+        if e is None:
+            return None
+        e = np.asarray(e, dtype=np.float64)
+        if e.shape[1] != 3 or e.shape[0] < mx:
+            raise ValueError("edge arrays are (mx, 3)")
+        return np.ascontiguousarray(e.T)   # Fortran (mx,3) column-major bytes
+
+    def sweep(self, iu, first, last, edge1=None, edge2=None):
+        e1, e2 = self._edge(edge1, self.mx), self._edge(edge2, self.mx)
+        mx = self.mx if (e1 is not None or e2 is not None) else 0
+        if e1 is not None:
+            mx = e1.shape[1]
+        elif e2 is not None:
+            mx = e2.shape[1]
+        _chk(load_library().cliffs_sweep(self._h, iu, first, last, _p(e1), _p(e2), mx))
+
+    def maxwave(self):
+        _chk(load_library().cliffs_maxwave(self._h))
+
+    def step(self, xedge1=None, xedge2=None, yedge1=None, yedge2=None):
+        es = [self._edge(e, self.mx) for e in (xedge1, xedge2, yedge1, yedge2)]
+        mx = next((e.shape[1] for e in es if e is not None), self.mx)
+        _chk(load_library().cliffs_step(self._h, *[_p(e) for e in es], mx))
+
+    def run(self, nsteps: int):
+        _chk(load_library().cliffs_run(self._h, int(nsteps)))
+
+    def run_timed(self, nsteps: int):
+        """-> (ms_total, ms_xsweep_mean, ms_ysweep_mean), HIP events on the
+        handle's stream."""
+        t, x, y = C.c_double(), C.c_double(), C.c_double()
+        _chk(load_library().cliffs_run_timed(self._h, int(nsteps), C.byref(t), C.byref(x),
+                                             C.byref(y)))
+        return t.value, x.value, y.value
+
+    def gauges(self, I, J):
+        I = np.ascontiguousarray(I, dtype=np.int32)
+        J = np.ascontiguousarray(J, dtype=np.int32)
+        n = len(I)
+        eta, u, v = np.empty(n), np.empty(n), np.empty(n)
+        _chk(load_library().cliffs_gauges(self._h, n, I.ctypes.data_as(_ip), J.ctypes.data_as(_ip),
+                                          _p(eta), _p(u), _p(v)))
+        return eta, u, v
+
+    def sync(self):
+        _chk(load_library().cliffs_sync(self._h))
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        _chk(load_library().cliffs_stream(self._h, C.byref(s)))
+        return s.value or 0

@@ -1,0 +1,678 @@
+// C-ABI of the cliffs_gpu library (include/cliffs_gpu.h): device state,
+// host <-> device conversion and the launch sequence of a time step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/cliffs_gpu.h"
+#include "cliffs_dev.h"
+#include "convert.h"
+
+namespace cliffs {
+hipError_t launch_sweep_x(const XArgs& a, hipStream_t st);
+hipError_t launch_sweep_y(const YArgs& a, hipStream_t st);
+hipError_t launch_line_flags(const DevGrid& g, const Params& p, unsigned char* fout, int dir,
+                             unsigned* fclr, int nclr, hipStream_t st);
+hipError_t launch_maxwave(const DevGrid& g, hipStream_t st);
+}  // namespace cliffs
+
+using namespace cliffs;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+int hipfail(hipError_t e, const char* where) {
+    return fail(CLIFFS_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+#define HIPCHK(call)                                  \
+    do {                                              \
+        hipError_t e_ = (call);                       \
+        if (e_ != hipSuccess) return hipfail(e_, #call); \
+    } while (0)
+
+constexpr int kEdgeSlots = 16;
+
+}  // namespace
+
+struct cliffs_handle {
+    int dev = 0;
+    hipStream_t st = nullptr;
+    int nx = 0, ny = 0, pitch = 0;
+    Consts64 k{};
+    Params p{};
+    std::vector<double> dep;   // FP64 depth (host copy, for conversions)
+    std::vector<float> s;      // device background celerity (host copy)
+    // line ranges (cliffs_main.f:140-164), 0-based inclusive
+    bool xrun = false, yrun = false;
+    int ii1 = 0, ii2 = 0, jj1 = 0, jj2 = 0, mx = 1;
+    // device memory
+    float* fields = nullptr;   // S, CP, U, V, HM, VM
+    size_t fstride = 0;        // floats per field
+    float4* xm = nullptr;
+    float2* xr = nullptr;
+    float4* ym = nullptr;
+    float* zeta = nullptr;
+    float* edges = nullptr;    // kEdgeSlots x (6*nx + 6*ny) floats
+    float* hedges = nullptr;   // pinned staging, same size
+    hipEvent_t edge_ev[kEdgeSlots] = {};
+    int edge_slot = 0;
+    size_t edge_slot_floats = 0;
+    unsigned char* flags = nullptr;  // 3 buffers of fbytes
+    size_t fbytes = 0;
+    int m = 0;                 // kernel sequence number
+    bool flags_valid = false;
+    bool state = false;
+    std::vector<float> hbuf;   // pitched host staging
+
+    DevGrid grid() const {
+        DevGrid g;
+        g.nx = nx;
+        g.ny = ny;
+        g.pitch = pitch;
+        g.S = fields;
+        g.CP = fields + fstride;
+        g.U = fields + 2 * fstride;
+        g.V = fields + 3 * fstride;
+        g.HM = fields + 4 * fstride;
+        g.VM = fields + 5 * fstride;
+        return g;
+    }
+    unsigned char* fbuf(int i) { return flags + (size_t)(((i % 3) + 3) % 3) * fbytes; }
+};
+
+namespace {
+
+void free_handle(cliffs_handle* h) {
+    if (!h) return;
+    if (h->dev >= 0) (void)hipSetDevice(h->dev);
+    if (h->st) (void)hipStreamSynchronize(h->st);
+    for (auto& e : h->edge_ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipFree(h->fields);
+    (void)hipFree(h->xm);
+    (void)hipFree(h->xr);
+    (void)hipFree(h->ym);
+    (void)hipFree(h->zeta);
+    (void)hipFree(h->edges);
+    (void)hipFree(h->flags);
+    if (h->hedges) (void)hipHostFree(h->hedges);
+    if (h->st) (void)hipStreamDestroy(h->st);
+    delete h;
+}
+
+// pitched FP32 host buffer <-> device field
+int put_field(cliffs_handle* h, float* dst, const std::vector<float>& src) {
+    HIPCHK(hipMemcpy2DAsync(dst, (size_t)h->pitch * sizeof(float), src.data(),
+                            (size_t)h->pitch * sizeof(float), (size_t)h->pitch * sizeof(float),
+                            (size_t)h->ny, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return CLIFFS_OK;
+}
+int get_field(cliffs_handle* h, const float* src, std::vector<float>& dst) {
+    dst.resize((size_t)h->pitch * h->ny);
+    HIPCHK(hipMemcpy2DAsync(dst.data(), (size_t)h->pitch * sizeof(float), src,
+                            (size_t)h->pitch * sizeof(float), (size_t)h->pitch * sizeof(float),
+                            (size_t)h->ny, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return CLIFFS_OK;
+}
+inline size_t hidx(const cliffs_handle* h, int i, int j) {
+    return (size_t)j * h->pitch + kOff + i;
+}
+
+// Edge arrays (mx,3) FP64 -> device [6][n] (ub, vb, g*eta for edge1, then
+// edge2) for a sweep of direction iu over lines of count n.
+int stage_edges(cliffs_handle* h, int iu, const double* e1, const double* e2, int mx,
+                const float** dev_out) {
+    *dev_out = nullptr;
+    if (!e1 && !e2) return CLIFFS_OK;
+    const int n = iu == 1 ? h->ny : h->nx;
+    if (mx < n) return fail(CLIFFS_ERR_ARG, "edge leading dimension mx smaller than line count");
+    const int slot = h->edge_slot;
+    h->edge_slot = (h->edge_slot + 1) % kEdgeSlots;
+    HIPCHK(hipEventSynchronize(h->edge_ev[slot]));
+    float* hs = h->hedges + (size_t)slot * h->edge_slot_floats;
+    const int cn = iu == 1 ? 0 : 1;  // column of the normal velocity in edge(:,1:2)
+    const int ct = 1 - cn;
+    for (int side = 0; side < 2; side++) {
+        const double* e = side == 0 ? e1 : e2;
+        float* o = hs + (size_t)side * 3 * n;
+        for (int l = 0; l < n; l++) {
+            o[l] = e ? (float)e[l + (size_t)mx * cn] : 0.0f;
+            o[n + l] = e ? (float)e[l + (size_t)mx * ct] : 0.0f;
+            o[2 * n + l] = e ? (float)(h->k.grav * e[l + (size_t)mx * 2]) : 0.0f;
+        }
+    }
+    float* dv = h->edges + (size_t)slot * h->edge_slot_floats;
+    HIPCHK(hipMemcpyAsync(dv, hs, sizeof(float) * 6 * n, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipEventRecord(h->edge_ev[slot], h->st));
+    *dev_out = dv;
+    return CLIFFS_OK;
+}
+
+int clear_flag_words(cliffs_handle* h) { return (int)(h->fbytes / 4); }
+
+// Line flags of direction dir (0 rows, 1 columns) from the current state.
+int launch_flags(cliffs_handle* h, int dir) {
+    const hipError_t e = launch_line_flags(h->grid(), h->p, h->fbuf(h->m + 1), dir,
+                                           reinterpret_cast<unsigned*>(h->fbuf(h->m + 2)),
+                                           clear_flag_words(h), h->st);
+    if (e != hipSuccess) return hipfail(e, "line_flags");
+    h->m++;
+    return CLIFFS_OK;
+}
+
+int ensure_flags(cliffs_handle* h, int dir) {
+    if (h->flags_valid) return CLIFFS_OK;
+    HIPCHK(hipMemsetAsync(h->flags, 0, 3 * h->fbytes, h->st));
+    h->m = 0;
+    const int rc = launch_flags(h, dir);
+    if (rc) return rc;
+    h->flags_valid = true;
+    return CLIFFS_OK;
+}
+
+XArgs xargs(cliffs_handle* h) {
+    XArgs a{};
+    a.g = h->grid();
+    a.xm = h->xm;
+    a.xr = h->xr;
+    a.p = h->p;
+    a.nstrips = (h->nx + kXStrip - 1) / kXStrip;
+    return a;
+}
+YArgs yargs(cliffs_handle* h) {
+    YArgs a{};
+    a.g = h->grid();
+    a.ym = h->ym;
+    a.zeta = h->zeta;
+    a.p = h->p;
+    return a;
+}
+
+// one chained x-sweep kernel: reads F[m], writes F[m+1], clears F[m+2]
+int chained_x(cliffs_handle* h, const float* edges, bool one_d) {
+    XArgs a = xargs(h);
+    a.edges = edges;
+    a.fin = h->fbuf(h->m);
+    a.fout = h->fbuf(h->m + 1);
+    a.fclr = reinterpret_cast<unsigned*>(h->fbuf(h->m + 2));
+    a.nclr = clear_flag_words(h);
+    if (one_d) {
+        a.row_lo = 0;
+        a.row_hi = h->ny - 1;
+        a.act_lo = a.act_hi = 0;
+        a.flag_rows = 1;
+        a.fuse_max = 1;
+    } else {
+        a.row_lo = 0;
+        a.row_hi = h->ny - 1;
+        a.act_lo = h->jj1;
+        a.act_hi = h->jj2;
+        a.flag_rows = 0;
+        a.fuse_max = 0;
+    }
+    const hipError_t e = launch_sweep_x(a, h->st);
+    if (e != hipSuccess) return hipfail(e, "sweep_x");
+    h->m++;
+    return CLIFFS_OK;
+}
+
+int chained_y(cliffs_handle* h, const float* edges, bool one_d) {
+    YArgs a = yargs(h);
+    a.edges = edges;
+    a.fin = h->fbuf(h->m);
+    a.fout = h->fbuf(h->m + 1);
+    a.fclr = reinterpret_cast<unsigned*>(h->fbuf(h->m + 2));
+    a.nclr = clear_flag_words(h);
+    a.col_base = 0;
+    if (one_d) {
+        a.col_lo = a.col_hi = 0;
+        a.flag_cols = 1;
+    } else {
+        a.col_lo = h->ii1;
+        a.col_hi = h->ii2;
+        a.flag_cols = 0;
+    }
+    a.fuse_max = 1;
+    a.col_span = h->nx;  // every column: fused max-wave and flags
+    const hipError_t e = launch_sweep_y(a, h->st);
+    if (e != hipSuccess) return hipfail(e, "sweep_y");
+    h->m++;
+    return CLIFFS_OK;
+}
+
+int do_step(cliffs_handle* h, const float* xe, const float* ye) {
+    int rc;
+    if (h->xrun && h->yrun) {
+        if ((rc = ensure_flags(h, 0))) return rc;
+        if ((rc = chained_x(h, xe, false))) return rc;
+        if ((rc = chained_y(h, ye, false))) return rc;
+    } else if (h->xrun) {
+        if ((rc = ensure_flags(h, 0))) return rc;
+        if ((rc = chained_x(h, xe, true))) return rc;
+    } else if (h->yrun) {
+        if ((rc = ensure_flags(h, 1))) return rc;
+        if ((rc = chained_y(h, ye, true))) return rc;
+    } else {
+        const hipError_t e = launch_maxwave(h->grid(), h->st);
+        if (e != hipSuccess) return hipfail(e, "maxwave");
+    }
+    return CLIFFS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cliffs_last_error(void) { return g_err.c_str(); }
+
+int cliffs_create(const cliffs_grid_desc* d, cliffs_handle** out) {
+    if (!d || !out) return fail(CLIFFS_ERR_ARG, "NULL descriptor or output pointer");
+    *out = nullptr;
+    if (d->nx < 1 || d->ny < 1 || !d->dep)
+        return fail(CLIFFS_ERR_ARG, "grid must be at least 1x1 with a depth array");
+    if (!(d->ground > 0.0)) return fail(CLIFFS_ERR_ARG, "ground must be > 0 (c_min = sqrt(g ground))");
+    if (!(d->dt > 0.0)) return fail(CLIFFS_ERR_ARG, "dt must be > 0");
+    if (d->nx > 2 && !d->s1) return fail(CLIFFS_ERR_ARG, "s1 required when nx > 2");
+    if (d->ny > 2 && (!d->s2 || !d->zeta)) return fail(CLIFFS_ERR_ARG, "s2 and zeta required when ny > 2");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(CLIFFS_ERR_NODEVICE, "no HIP device");
+    if (d->device < 0 || d->device >= ndev) return fail(CLIFFS_ERR_NODEVICE, "device index out of range");
+
+    cliffs_handle* h = new (std::nothrow) cliffs_handle;
+    if (!h) return fail(CLIFFS_ERR_NOMEM, "host allocation");
+    const int nx = d->nx, ny = d->ny;
+    h->dev = d->device;
+    h->nx = nx;
+    h->ny = ny;
+    h->pitch = ((nx + 2 * kOff) + 63) / 64 * 64;
+    h->k = make_consts(d->ground);
+    h->p.dt = (float)d->dt;
+    h->p.grav = 9.8f;
+    h->p.celmin = h->k.celmin32;
+    h->p.gground = (float)(h->k.grav * d->ground);
+    h->p.gn2 = (float)(h->k.grav * d->crough);
+    h->p.sqrt2 = 1.41421356f;
+    h->p.friction = d->crough > 0.0 ? 1 : 0;
+    // line ranges (cliffs_main.f:140-164), 0-based
+    h->xrun = nx > 2;
+    h->yrun = ny > 2;
+    h->jj1 = h->yrun ? 1 : 0;
+    h->jj2 = h->yrun ? ny - 2 : 0;
+    h->ii1 = h->xrun ? 1 : 0;
+    h->ii2 = h->xrun ? nx - 2 : 0;
+    h->mx = std::max(h->xrun ? ny : 1, h->yrun ? nx : 1);
+
+    auto bail = [&](int rc) {
+        free_handle(h);
+        return rc;
+    };
+    if (hipSetDevice(h->dev) != hipSuccess) return bail(fail(CLIFFS_ERR_HIP, "hipSetDevice"));
+    if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(CLIFFS_ERR_HIP, "hipStreamCreate"));
+
+    const size_t m = (size_t)nx * ny;
+    h->dep.assign(d->dep, d->dep + m);
+    h->s.resize(m);
+    for (size_t c = 0; c < m; c++) h->s[c] = to_s(h->dep[c], h->k);
+
+    // ---- metrics.  x: s1(i,j) = a(j) b(i)  (MasterGrid.f:86-99)
+    std::vector<float4> xm(nx, make_float4(0.f, 0.f, 0.f, 0.f));
+    std::vector<float2> xr(ny, make_float2(1.f, 1.f));
+    if (nx > 2) {
+        const double* s1 = d->s1;
+        const int l = nx - 1;
+        std::vector<double> b(l), aj(ny);
+        for (int i = 0; i < l; i++) b[i] = s1[i];
+        for (int j = 0; j < ny; j++) aj[j] = s1[(size_t)l * j] / b[0];
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < l; i++) {
+                const double v = s1[i + (size_t)l * j];
+                if (!(v > 0.0) || std::fabs(aj[j] * b[i] - v) > 1e-9 * v)
+                    return bail(fail(CLIFFS_ERR_METRIC, "s1(i,j) must be positive and separable a(j)*b(i)"));
+            }
+        for (int i = 0; i < nx; i++) {
+            const double bm = i >= 1 ? b[i - 1] : b[0];
+            const double bp = i < l ? b[i] : b[l - 1];
+            xm[i] = make_float4((float)(1.0 / bm), (float)(1.0 / bp), (float)(1.0 / (bm + bp)),
+                                (float)(bp - bm));
+        }
+        for (int j = 0; j < ny; j++) xr[j] = make_float2((float)(1.0 / aj[j]), (float)aj[j]);
+    }
+    // ---- metrics.  y: s2(j), zeta(j)  (MasterGrid.f:103-116)
+    std::vector<float4> ym(ny, make_float4(0.f, 0.f, 0.f, 0.f));
+    std::vector<float> zt(ny, 0.0f);
+    if (ny > 2) {
+        const double* s2 = d->s2;
+        for (int j = 0; j < ny - 1; j++)
+            if (!(s2[j] > 0.0)) return bail(fail(CLIFFS_ERR_METRIC, "s2 must be positive"));
+        for (int j = 0; j < ny; j++) {
+            const double bm = j >= 1 ? s2[j - 1] : s2[0];
+            const double bp = j < ny - 1 ? s2[j] : s2[ny - 2];
+            ym[j] = make_float4((float)(1.0 / bm), (float)(1.0 / bp), (float)(1.0 / (bm + bp)),
+                                (float)(bp - bm));
+            zt[j] = (float)d->zeta[j];
+        }
+    }
+
+    // ---- device allocations
+    h->fstride = (size_t)h->pitch * ny + 64;
+    if (hipMalloc(&h->fields, 6 * h->fstride * sizeof(float)) != hipSuccess)
+        return bail(fail(CLIFFS_ERR_NOMEM, "device state allocation"));
+    if (hipMalloc(&h->xm, sizeof(float4) * nx) != hipSuccess ||
+        hipMalloc(&h->xr, sizeof(float2) * ny) != hipSuccess ||
+        hipMalloc(&h->ym, sizeof(float4) * ny) != hipSuccess ||
+        hipMalloc(&h->zeta, sizeof(float) * ny) != hipSuccess)
+        return bail(fail(CLIFFS_ERR_NOMEM, "device metric allocation"));
+    h->edge_slot_floats = ((size_t)6 * nx + 6 * ny + 63) / 64 * 64;
+    if (hipMalloc(&h->edges, sizeof(float) * h->edge_slot_floats * kEdgeSlots) != hipSuccess ||
+        hipHostMalloc(&h->hedges, sizeof(float) * h->edge_slot_floats * kEdgeSlots) != hipSuccess)
+        return bail(fail(CLIFFS_ERR_NOMEM, "edge staging allocation"));
+    for (auto& e : h->edge_ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            return bail(fail(CLIFFS_ERR_HIP, "hipEventCreate"));
+    h->fbytes = ((size_t)std::max(nx, ny) + 8 + 255) / 256 * 256;
+    if (hipMalloc(&h->flags, 3 * h->fbytes) != hipSuccess)
+        return bail(fail(CLIFFS_ERR_NOMEM, "flag allocation"));
+
+    if (hipMemcpy(h->xm, xm.data(), sizeof(float4) * nx, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->xr, xr.data(), sizeof(float2) * ny, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->ym, ym.data(), sizeof(float4) * ny, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->zeta, zt.data(), sizeof(float) * ny, hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(CLIFFS_ERR_HIP, "metric upload"));
+    if (hipMemset(h->fields, 0, 6 * h->fstride * sizeof(float)) != hipSuccess)
+        return bail(fail(CLIFFS_ERR_HIP, "state clear"));
+
+    // background celerity
+    h->hbuf.assign((size_t)h->pitch * ny, 0.0f);
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) h->hbuf[hidx(h, i, j)] = h->s[(size_t)i + (size_t)nx * j];
+    int rc = put_field(h, h->grid().S, h->hbuf);
+    if (rc) return bail(rc);
+    rc = cliffs_reset_max(h);
+    if (rc) return bail(rc);
+    *out = h;
+    return CLIFFS_OK;
+}
+
+int cliffs_destroy(cliffs_handle* h) {
+    free_handle(h);
+    return CLIFFS_OK;
+}
+
+int cliffs_upload_state(cliffs_handle* h, const double* cel, const double* xvel,
+                        const double* yvel) {
+    if (!h || !cel || !xvel || !yvel) return fail(CLIFFS_ERR_ARG, "NULL argument");
+    HIPCHK(hipSetDevice(h->dev));
+    const int nx = h->nx, ny = h->ny;
+    std::vector<float>& b = h->hbuf;
+    b.assign((size_t)h->pitch * ny, 0.0f);
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            const size_t c = (size_t)i + (size_t)nx * j;
+            b[hidx(h, i, j)] = to_cp(cel[c], h->dep[c], h->s[c], h->k);
+        }
+    int rc = put_field(h, h->grid().CP, b);
+    if (rc) return rc;
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) b[hidx(h, i, j)] = (float)xvel[(size_t)i + (size_t)nx * j];
+    if ((rc = put_field(h, h->grid().U, b))) return rc;
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) b[hidx(h, i, j)] = (float)yvel[(size_t)i + (size_t)nx * j];
+    if ((rc = put_field(h, h->grid().V, b))) return rc;
+    h->state = true;
+    h->flags_valid = false;
+    return CLIFFS_OK;
+}
+
+int cliffs_download_state(cliffs_handle* h, double* cel, double* xvel, double* yvel) {
+    if (!h || !cel || !xvel || !yvel) return fail(CLIFFS_ERR_ARG, "NULL argument");
+    HIPCHK(hipSetDevice(h->dev));
+    const int nx = h->nx, ny = h->ny;
+    std::vector<float> b;
+    int rc = get_field(h, h->grid().CP, b);
+    if (rc) return rc;
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            const size_t c = (size_t)i + (size_t)nx * j;
+            cel[c] = to_cel(h->s[c], b[hidx(h, i, j)], h->dep[c], h->k);
+        }
+    if ((rc = get_field(h, h->grid().U, b))) return rc;
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) xvel[(size_t)i + (size_t)nx * j] = b[hidx(h, i, j)];
+    if ((rc = get_field(h, h->grid().V, b))) return rc;
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) yvel[(size_t)i + (size_t)nx * j] = b[hidx(h, i, j)];
+    return CLIFFS_OK;
+}
+
+int cliffs_download_max(cliffs_handle* h, double* hmax, double* velmax) {
+    if (!h || !hmax || !velmax) return fail(CLIFFS_ERR_ARG, "NULL argument");
+    HIPCHK(hipSetDevice(h->dev));
+    const int nx = h->nx, ny = h->ny;
+    std::vector<float> b;
+    int rc = get_field(h, h->grid().HM, b);
+    if (rc) return rc;
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            const size_t c = (size_t)i + (size_t)nx * j;
+            hmax[c] = to_cel(h->s[c], b[hidx(h, i, j)], h->dep[c], h->k);
+        }
+    if ((rc = get_field(h, h->grid().VM, b))) return rc;
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) velmax[(size_t)i + (size_t)nx * j] = b[hidx(h, i, j)];
+    return CLIFFS_OK;
+}
+
+int cliffs_reset_max(cliffs_handle* h) {
+    if (!h) return fail(CLIFFS_ERR_ARG, "NULL handle");
+    HIPCHK(hipSetDevice(h->dev));
+    // hmax = 0 means c = 0, i.e. cp = -max(s,0)
+    std::vector<float>& b = h->hbuf;
+    b.assign((size_t)h->pitch * h->ny, 0.0f);
+    for (int j = 0; j < h->ny; j++)
+        for (int i = 0; i < h->nx; i++) b[hidx(h, i, j)] = -c0f(h->s[(size_t)i + (size_t)h->nx * j]);
+    int rc = put_field(h, h->grid().HM, b);
+    if (rc) return rc;
+    HIPCHK(hipMemsetAsync(h->grid().VM, 0, h->fstride * sizeof(float), h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return CLIFFS_OK;
+}
+
+int cliffs_sweep(cliffs_handle* h, int iu, int first, int last, const double* edge1,
+                 const double* edge2, int mx) {
+    if (!h) return fail(CLIFFS_ERR_ARG, "NULL handle");
+    if (!h->state) return fail(CLIFFS_ERR_STATE, "no state uploaded");
+    if (iu != 1 && iu != 2) return fail(CLIFFS_ERR_ARG, "iu must be 1 or 2");
+    const int nl = iu == 1 ? h->ny : h->nx;
+    const int n = iu == 1 ? h->nx : h->ny;
+    if (first < 1 || last > nl || first > last) return fail(CLIFFS_ERR_ARG, "line range out of grid");
+    if (n < 3) return fail(CLIFFS_ERR_ARG, "line length must be >= 3");
+    HIPCHK(hipSetDevice(h->dev));
+    const float* ed = nullptr;
+    int rc = stage_edges(h, iu, edge1, edge2, mx, &ed);
+    if (rc) return rc;
+    // flags of this direction from the current state, then the sweep
+    HIPCHK(hipMemsetAsync(h->flags, 0, 3 * h->fbytes, h->st));
+    h->m = 0;
+    if ((rc = launch_flags(h, iu == 1 ? 0 : 1))) return rc;
+    if (iu == 1) {
+        XArgs a = xargs(h);
+        a.edges = ed;
+        a.fin = h->fbuf(h->m);
+        a.fout = h->fbuf(h->m + 1);
+        a.fclr = reinterpret_cast<unsigned*>(h->fbuf(h->m + 2));
+        a.nclr = clear_flag_words(h);
+        a.row_lo = a.act_lo = first - 1;
+        a.row_hi = a.act_hi = last - 1;
+        a.flag_rows = 1;
+        a.fuse_max = 0;
+        const hipError_t e = launch_sweep_x(a, h->st);
+        if (e != hipSuccess) return hipfail(e, "sweep_x");
+    } else {
+        YArgs a = yargs(h);
+        a.edges = ed;
+        a.fin = h->fbuf(h->m);
+        a.fout = h->fbuf(h->m + 1);
+        a.fclr = reinterpret_cast<unsigned*>(h->fbuf(h->m + 2));
+        a.nclr = clear_flag_words(h);
+        a.col_base = (first - 1) & ~63;
+        a.col_span = last - a.col_base;
+        a.col_lo = first - 1;
+        a.col_hi = last - 1;
+        a.flag_cols = 1;
+        a.fuse_max = 0;
+        const hipError_t e = launch_sweep_y(a, h->st);
+        if (e != hipSuccess) return hipfail(e, "sweep_y");
+    }
+    h->m++;
+    h->flags_valid = false;
+    return CLIFFS_OK;
+}
+
+int cliffs_maxwave(cliffs_handle* h) {
+    if (!h) return fail(CLIFFS_ERR_ARG, "NULL handle");
+    HIPCHK(hipSetDevice(h->dev));
+    const hipError_t e = launch_maxwave(h->grid(), h->st);
+    if (e != hipSuccess) return hipfail(e, "maxwave");
+    return CLIFFS_OK;
+}
+
+int cliffs_step(cliffs_handle* h, const double* xedge1, const double* xedge2,
+                const double* yedge1, const double* yedge2, int mx) {
+    if (!h) return fail(CLIFFS_ERR_ARG, "NULL handle");
+    if (!h->state) return fail(CLIFFS_ERR_STATE, "no state uploaded");
+    HIPCHK(hipSetDevice(h->dev));
+    const float *xe = nullptr, *ye = nullptr;
+    int rc;
+    if (h->xrun && (rc = stage_edges(h, 1, xedge1, xedge2, mx, &xe))) return rc;
+    if (h->yrun && (rc = stage_edges(h, 2, yedge1, yedge2, mx, &ye))) return rc;
+    return do_step(h, xe, ye);
+}
+
+int cliffs_run(cliffs_handle* h, int nsteps) {
+    if (!h) return fail(CLIFFS_ERR_ARG, "NULL handle");
+    if (!h->state) return fail(CLIFFS_ERR_STATE, "no state uploaded");
+    if (nsteps < 0) return fail(CLIFFS_ERR_ARG, "nsteps < 0");
+    HIPCHK(hipSetDevice(h->dev));
+    for (int s = 0; s < nsteps; s++) {
+        const int rc = do_step(h, nullptr, nullptr);
+        if (rc) return rc;
+    }
+    return CLIFFS_OK;
+}
+
+int cliffs_run_timed(cliffs_handle* h, int nsteps, double* ms_total, double* ms_x, double* ms_y) {
+    if (!h || !ms_total || !ms_x || !ms_y) return fail(CLIFFS_ERR_ARG, "NULL argument");
+    if (!h->state) return fail(CLIFFS_ERR_STATE, "no state uploaded");
+    if (nsteps < 1) return fail(CLIFFS_ERR_ARG, "nsteps < 1");
+    HIPCHK(hipSetDevice(h->dev));
+    int rc;
+    if ((rc = ensure_flags(h, (h->xrun || !h->yrun) ? 0 : 1))) return rc;
+    std::vector<hipEvent_t> ev(4 * (size_t)nsteps + 2);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    size_t q = 0;
+    HIPCHK(hipEventRecord(ev[q++], h->st));
+    for (int s = 0; s < nsteps; s++) {
+        if (h->xrun) {
+            HIPCHK(hipEventRecord(ev[2 + 4 * (size_t)s], h->st));
+            if ((rc = chained_x(h, nullptr, !h->yrun))) return rc;
+            HIPCHK(hipEventRecord(ev[3 + 4 * (size_t)s], h->st));
+        }
+        if (h->yrun) {
+            HIPCHK(hipEventRecord(ev[4 + 4 * (size_t)s], h->st));
+            if ((rc = chained_y(h, nullptr, !h->xrun))) return rc;
+            HIPCHK(hipEventRecord(ev[5 + 4 * (size_t)s], h->st));
+        }
+        if (!h->xrun && !h->yrun) {
+            const hipError_t e = launch_maxwave(h->grid(), h->st);
+            if (e != hipSuccess) return hipfail(e, "maxwave");
+        }
+    }
+    HIPCHK(hipEventRecord(ev[1], h->st));
+    HIPCHK(hipEventSynchronize(ev[1]));
+    float t;
+    HIPCHK(hipEventElapsedTime(&t, ev[0], ev[1]));
+    *ms_total = t;
+    double sx = 0, sy = 0;
+    for (int s = 0; s < nsteps; s++) {
+        if (h->xrun) {
+            HIPCHK(hipEventElapsedTime(&t, ev[2 + 4 * (size_t)s], ev[3 + 4 * (size_t)s]));
+            sx += t;
+        }
+        if (h->yrun) {
+            HIPCHK(hipEventElapsedTime(&t, ev[4 + 4 * (size_t)s], ev[5 + 4 * (size_t)s]));
+            sy += t;
+        }
+    }
+    *ms_x = h->xrun ? sx / nsteps : 0.0;
+    *ms_y = h->yrun ? sy / nsteps : 0.0;
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return CLIFFS_OK;
+}
+
+int cliffs_gauges(cliffs_handle* h, int n, const int* I, const int* J, double* eta, double* u,
+                  double* v) {
+    if (!h || n < 0 || (n > 0 && (!I || !J || !eta || !u || !v)))
+        return fail(CLIFFS_ERR_ARG, "bad gauge arguments");
+    HIPCHK(hipSetDevice(h->dev));
+    HIPCHK(hipStreamSynchronize(h->st));
+    const DevGrid g = h->grid();
+    for (int q = 0; q < n; q++) {
+        if (I[q] < 1 || I[q] > h->nx || J[q] < 1 || J[q] > h->ny)
+            return fail(CLIFFS_ERR_ARG, "gauge index out of grid");
+        const int i = I[q] - 1, j = J[q] - 1;
+        const size_t o = hidx(h, i, j);
+        float cp, uu, vv;
+        HIPCHK(hipMemcpy(&cp, g.CP + o, sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&uu, g.U + o, sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&vv, g.V + o, sizeof(float), hipMemcpyDeviceToHost));
+        const size_t c = (size_t)i + (size_t)h->nx * j;
+        const double cel = to_cel(h->s[c], cp, h->dep[c], h->k);
+        eta[q] = cel > h->k.celmin ? cel * cel / h->k.grav - h->dep[c] : (double)(float)-1.e32;
+        u[q] = uu;
+        v[q] = vv;
+    }
+    return CLIFFS_OK;
+}
+
+int cliffs_sync(cliffs_handle* h) {
+    if (!h) return fail(CLIFFS_ERR_ARG, "NULL handle");
+    HIPCHK(hipSetDevice(h->dev));
+    HIPCHK(hipStreamSynchronize(h->st));
+    HIPCHK(hipGetLastError());
+    return CLIFFS_OK;
+}
+
+int cliffs_stream(cliffs_handle* h, void** stream) {
+    if (!h || !stream) return fail(CLIFFS_ERR_ARG, "NULL argument");
+    *stream = (void*)h->st;
+    return CLIFFS_OK;
+}
+
+int cliffs_info(cliffs_handle* h, double* grav, double* celmin, int* ii1, int* ii2, int* jj1,
+                int* jj2) {
+    if (!h) return fail(CLIFFS_ERR_ARG, "NULL handle");
+    if (grav) *grav = h->k.grav;
+    if (celmin) *celmin = h->k.celmin;
+    if (ii1) *ii1 = h->ii1 + 1;
+    if (ii2) *ii2 = h->ii2 + 1;
+    if (jj1) *jj1 = h->jj1 + 1;
+    if (jj2) *jj2 = h->jj2 + 1;
+    return CLIFFS_OK;
+}
+
+}  // extern "C"

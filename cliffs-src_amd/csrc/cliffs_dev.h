@@ -1,0 +1,71 @@
+// Device-side data layout and kernel argument blocks.
+//
+// HBM layout (one allocation per field, FP32):
+//   row j of field F starts at F + j*pitch; node i of the row sits at
+//   column off + i, with off = 4 (16-byte aligned, room for the x-sweep's
+//   left halo) and pitch = roundup(nx + 8, 64) floats (256-byte rows).
+//   x is contiguous, exactly the reference's Fortran (nx, ny) order.
+// Fields: S (background celerity, read-only), CP (celerity perturbation),
+// U (x-velocity), V (y-velocity), HM (max CP), VM (max u^2+v^2).
+//
+// Line flags ("line has a wet node", cliffs.f:113) are byte arrays, one
+// byte per row or column, produced by the previous kernel in the chain:
+// the x-sweep produces column flags for the y-sweep, the y-sweep produces
+// row flags for the next x-sweep.  Three buffers rotate: kernel m reads
+// F[m%3], writes F[(m+1)%3] (only ever storing 1s) and clears F[(m+2)%3].
+#pragma once
+
+#include "vtcs_node.h"
+
+namespace cliffs {
+
+constexpr int kOff = 4;         // column offset of node 0 within a row
+constexpr int kXStrip = 248;    // output nodes per x-sweep wave (window 256)
+constexpr int kXRows = 8;       // rows per x-sweep wave
+constexpr int kYRows = 128;     // output rows per y-sweep block
+
+struct DevGrid {
+    int nx, ny, pitch;
+    float* S;
+    float* CP;
+    float* U;
+    float* V;
+    float* HM;
+    float* VM;
+};
+
+struct XArgs {
+    DevGrid g;
+    const float4* xm;     // per node i: (1/b(i-1), 1/b(i), 1/(b(i-1)+b(i)), b(i)-b(i-1))
+    const float2* xr;     // per row j: (1/a(j), a(j));  dx(i,j) = a(j) b(i)
+    const float* edges;   // [6][ny]: west ub, vb, g*eta; east ub, vb, g*eta (or null)
+    const unsigned char* fin;   // row flags
+    unsigned char* fout;        // produced flags (columns, or rows in 1-D)
+    unsigned* fclr;             // words to clear
+    int nclr;
+    Params p;
+    int nstrips;
+    int row_lo, row_hi;   // rows visited (0-based, inclusive)
+    int act_lo, act_hi;   // rows swept
+    int flag_rows;        // produce row flags (1-D) instead of column flags
+    int fuse_max;         // fuse the max-wave update (1-D step)
+};
+
+struct YArgs {
+    DevGrid g;
+    const float4* ym;     // per row j: (1/s2(j-1), 1/s2(j), 1/(s2(j-1)+s2(j)), s2(j)-s2(j-1))
+    const float* zeta;    // per row j
+    const float* edges;   // [6][nx]: south ub, vb, g*eta; north ub, vb, g*eta (or null)
+    const unsigned char* fin;   // column flags
+    unsigned char* fout;        // produced flags (rows, or columns in 1-D)
+    unsigned* fclr;
+    int nclr;
+    Params p;
+    int col_base;         // first column of the launch grid
+    int col_span;         // columns covered by the launch grid
+    int col_lo, col_hi;   // columns swept (0-based, inclusive)
+    int flag_cols;        // produce column flags (1-D) instead of row flags
+    int fuse_max;         // fuse the max-wave update (2-D step)
+};
+
+}  // namespace cliffs

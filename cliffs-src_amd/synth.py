@@ -1,0 +1,189 @@
+"""Seeded synthetic inputs for the VTCS-2 step (pure numpy, FP64).
+
+Each generator returns a ``Case``: FP64 depth and initial state laid out as
+the reference holds them (Fortran (nx, ny) column-major == numpy (ny, nx)),
+the grid metrics of MasterGrid.f:85-116, and the run parameters.  The
+initial state follows the reference's initialisation: surface deformation
+added to depth, then H -> c = sqrt(g H) with a `ground` cut
+(cliffs_main.f:191-227).
+
+Workloads (sizes as named in the project's benchmark configurations):
+  * ``beach_1d``   -- 1-D sloping-beach run-up, 1 x 2000 cells, dx 0.25 m,
+                      1:20 beach to +0.5 m land, sech^2 solitary wave.
+  * ``open_ocean`` -- 2-D Cartesian open ocean, dx = dy = 500 m, depth
+                      4000 m minus fractal seamounts up to 2500 m, Gaussian
+                      hump (1 m, sigma 20 km), open boundaries.
+  * ``coastal``    -- 2-D shelf + seeded coastline with land and run-up,
+                      Cartesian or spherical, elliptical dipole source.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+GRAV = float(np.float32(9.8))        # global_modules.f:17 (FP32 literal widened)
+PI = float(np.float32(3.1415926536))
+EARTH_R = 6371009.0
+
+
+@dataclasses.dataclass
+class Case:
+    name: str
+    dep: np.ndarray          # (ny, nx) depth, positive = water
+    cel: np.ndarray          # (ny, nx) celerity
+    xvel: np.ndarray
+    yvel: np.ndarray
+    xcrd: np.ndarray
+    ycrd: np.ndarray
+    cartesian: bool
+    dt: float
+    ground: float
+    crough: float
+    steps: int
+    meta: dict
+
+    @property
+    def shape(self):
+        return self.dep.shape
+
+    def metrics(self):
+        return metrics(self.xcrd, self.ycrd, self.cartesian)
+
+
+def metrics(xcrd, ycrd, cartesian: bool):
+    """MasterGrid.f:85-116 in numpy: s1 (ny, nx-1), s2 (ny-1,), zeta (ny,)."""
+    x = np.asarray(xcrd, dtype=np.float64)
+    y = np.asarray(ycrd, dtype=np.float64)
+    nx, ny = len(x), len(y)
+    beta = PI * EARTH_R / 180.0
+    s1 = np.zeros((ny, max(nx - 1, 1)))
+    s2 = np.zeros(max(ny - 1, 1))
+    zeta = np.zeros(ny)
+    if nx > 2:
+        if cartesian:
+            s1[:, :] = (x[1:] - x[:-1])[None, :]
+        else:
+            r = beta * (x[1:] - x[:-1])
+            alpha = np.cos(PI / 180.0 * y)
+            s1[:, :] = alpha[:, None] * r[None, :]
+    if ny > 2:
+        if cartesian:
+            s2[:] = y[1:] - y[:-1]
+        else:
+            s2[:] = beta * (y[1:] - y[:-1])
+            zeta[:] = (0.125 / EARTH_R) * np.tan(PI / 180.0 * y)
+    return s1, s2, zeta
+
+
+def depth_to_celerity(H, ground):
+    """cliffs_main.f:218-226."""
+    H = np.asarray(H, dtype=np.float64)
+    return np.where(H <= ground, 0.0, np.sqrt(GRAV * np.maximum(H, 0.0)))
+
+
+def fractal_field(ny, nx, rng, beta=3.0, kmin=2.0):
+    """Zero-mean spectral fractal field with power ~ k^-beta, unit max-abs."""
+    ky = np.fft.fftfreq(ny)[:, None] * ny
+    kx = np.fft.rfftfreq(nx)[None, :] * nx
+    k = np.sqrt(kx * kx + ky * ky)
+    amp = np.where(k > 0, np.maximum(k, kmin) ** (-beta / 2.0), 0.0)
+    phase = rng.uniform(0, 2 * np.pi, size=amp.shape)
+    spec = amp * np.exp(1j * phase)
+    f = np.fft.irfft2(spec, s=(ny, nx))
+    f -= f.mean()
+    return f / np.abs(f).max()
+
+
+def open_ocean(n=4096, seed=0, steps=10000, dx=500.0, ny=None):
+    """Open ocean, no land: depth 4000 m minus fractal seamounts in [0, 2500] m."""
+    ny = n if ny is None else ny
+    nx = n
+    rng = np.random.default_rng(seed)
+    f = fractal_field(ny, nx, rng)
+    seam = (f - f.min()) / (f.max() - f.min()) * 2500.0
+    dep = 4000.0 - seam
+    x = np.arange(nx, dtype=np.float64) * dx
+    y = np.arange(ny, dtype=np.float64) * dx
+    sigma = 20000.0
+    cx = rng.uniform(0.25, 0.75) * x[-1]
+    cy = rng.uniform(0.25, 0.75) * y[-1]
+    eta = np.exp(-((x[None, :] - cx) ** 2 + (y[:, None] - cy) ** 2) / (2 * sigma * sigma))
+    ground = 0.01
+    cel = depth_to_celerity(dep + eta, ground)
+    dt = 0.8 * dx / math.sqrt(GRAV * 4001.0)
+    z = np.zeros_like(dep)
+    return Case("open_ocean", dep, cel, z.copy(), z.copy(), x, y, True, dt, ground, 0.0, steps,
+                dict(seed=seed, amplitude=1.0, sigma=sigma, centre=(cx, cy), dx=dx))
+
+
+def beach_1d(seed=0, n=2000, dx=0.25, steps=20000, amplitude=None):
+    """1-D sloping beach run-up (land at low x).  Amplitude from seed in [0.02, 0.3]."""
+    rng = np.random.default_rng(seed)
+    A = float(rng.uniform(0.02, 0.3)) if amplitude is None else float(amplitude)
+    x = np.arange(n, dtype=np.float64) * dx
+    d = np.where(x < 30.0, x / 20.0 - 0.5, 1.0)
+    x0 = 30.0 + 30.0 * 1.0
+    eta = A / np.cosh(np.sqrt(3.0 * A / 4.0) * (x - x0)) ** 2
+    u = -2.0 * (np.sqrt(GRAV * (1.0 + eta)) - np.sqrt(GRAV))
+    ground = 0.003
+    H = d + eta
+    cel = depth_to_celerity(H, ground)
+    u = np.where(cel > 0, u, 0.0)
+    dt = 0.8 * dx / float(np.max(np.abs(u) + cel))
+    dep = d[None, :]
+    return Case("beach_1d", dep, cel[None, :], u[None, :], np.zeros((1, n)), x, np.zeros(1), True,
+                dt, ground, 0.0, steps, dict(seed=seed, amplitude=A, x0=x0, dx=dx))
+
+
+def coastal(nx=256, ny=192, seed=0, cartesian=True, dx=500.0, steps=200, source_amp=3.0,
+            crough=0.0, dlon=None):
+    """Shelf + seeded coastline with land up to +300 m; elliptical uplift/
+    subsidence dipole (+A, -A/2) offshore.  Land is on the high-x side."""
+    rng = np.random.default_rng(seed)
+    if cartesian:
+        x = np.arange(nx, dtype=np.float64) * dx
+        y = np.arange(ny, dtype=np.float64) * dx
+        X, Y = np.meshgrid(x, y)
+        Lx, Ly = x[-1], y[-1]
+    else:
+        dl = (30.0 / 3600.0) if dlon is None else dlon
+        lon0, lat0 = 140.0, 38.0
+        x = lon0 + np.arange(nx) * dl
+        y = lat0 + np.arange(ny) * dl
+        Xd, Yd = np.meshgrid(x - x[0], y - y[0])
+        X = Xd * PI / 180.0 * EARTH_R * np.cos(PI / 180.0 * lat0)
+        Y = Yd * PI / 180.0 * EARTH_R
+        Lx, Ly = X.max(), Y.max()
+    # coastline position wanders with y
+    fr = fractal_field(ny, 1, rng, beta=2.0)[:, 0] if ny > 1 else np.zeros(1)
+    xc = Lx * (0.72 + 0.06 * fr)[:, None]
+    shelf_w = 0.15 * Lx
+    abyss = 4000.0 + 2000.0 * (fractal_field(ny, nx, rng) * 0.5 + 0.5)
+    dist = xc - X   # positive offshore
+    dep = np.where(dist > shelf_w, abyss,
+                   np.where(dist > 0, 200.0 * np.minimum(dist / (0.3 * shelf_w), 1.0) + 1.0, 0.0))
+    land = dist <= 0
+    dep = np.where(land, dist / (0.28 * Lx) * 300.0 - 0.5, dep)
+    # small-scale roughness on shelf/land
+    dep = dep + np.where(dist > shelf_w, 0.0, 5.0 * fractal_field(ny, nx, rng))
+    dep = np.where((dist > 0) & (dep < 0.5), 0.5, dep)
+    # dipole source
+    sx = rng.uniform(0.35, 0.55) * Lx
+    sy = rng.uniform(0.3, 0.7) * Ly
+    th = rng.uniform(0, np.pi)
+    a, b = 0.12 * Lx, 0.06 * Lx
+    xr = (X - sx) * np.cos(th) + (Y - sy) * np.sin(th)
+    yr = -(X - sx) * np.sin(th) + (Y - sy) * np.cos(th)
+    q = source_amp * np.exp(-((xr - 0.5 * a) / a) ** 2 - (yr / b) ** 2) \
+        - 0.5 * source_amp * np.exp(-((xr + 0.5 * a) / a) ** 2 - (yr / b) ** 2)
+    ground = 0.01
+    H = dep + q
+    cel = depth_to_celerity(H, ground)
+    s1, s2, _ = metrics(x, y, cartesian)
+    dmin = min(float(s1.min()), float(s2.min()) if ny > 2 else float(s1.min()))
+    dt = 0.8 * dmin / math.sqrt(GRAV * (float(dep.max()) + 2 * source_amp))
+    z = np.zeros_like(dep)
+    return Case("coastal", dep, cel, z.copy(), z.copy(), x, y, cartesian, dt, ground, crough,
+                steps, dict(seed=seed, amplitude=source_amp))

@@ -1,0 +1,125 @@
+/*
+ * cliffs_gpu -- MI355X (gfx950) implementation of the Cliffs VTCS-2
+ * dimensionally-split shallow-water step behind a plain C ABI.
+ *
+ * The reference solver keeps all state in Fortran modules and advances one
+ * grid line per call:
+ *
+ *     subroutine cliffs(iu,irw,n)                   cliffs.f:17
+ *
+ * called from two OpenMP loops per time step (cliffs_main.f:325-329 rows,
+ * 340-344 columns), followed by the max-wave update (cliffs_main.f:347-355).
+ * This library owns the same state on the GPU (uploaded once, resident in
+ * HBM) and replaces each whole OpenMP loop with one kernel launch.  The
+ * Fortran side binds these entry points with ISO_C_BINDING (see the
+ * package's fortran/ directory).
+ *
+ * Conventions
+ *   - Every 2-D array is the reference's Fortran (nx, ny) column-major
+ *     layout: element (i,j) (1-based) at [(i-1) + nx*(j-1)].
+ *   - Line indices (irw, first, last) and gauge indices are 1-based, as in
+ *     the reference.
+ *   - Edge arrays are the reference's edge1/edge2(mx,3) (global_modules.f:34),
+ *     column-major: (:,1) x-velocity, (:,2) y-velocity, (:,3) surface
+ *     elevation.  NULL means all-zero edges (open boundary, still-water
+ *     inflow; cliffs_main.f:288-290).
+ *   - Every function returns CLIFFS_OK (0) or a negative error code and
+ *     never aborts the process; cliffs_last_error() describes the failure.
+ *   - A handle belongs to one host thread.  Compute calls are asynchronous
+ *     on the handle's HIP stream; downloads synchronise.
+ */
+#ifndef CLIFFS_GPU_H
+#define CLIFFS_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLIFFS_OK 0
+#define CLIFFS_ERR_ARG (-1)      /* bad argument (NULL, size, index range) */
+#define CLIFFS_ERR_HIP (-2)      /* HIP runtime or launch failure */
+#define CLIFFS_ERR_NOMEM (-3)    /* device or host allocation failed */
+#define CLIFFS_ERR_METRIC (-4)   /* s1(i,j) is not a(j)*b(i) (MasterGrid.f:86-99 always is) */
+#define CLIFFS_ERR_NODEVICE (-5) /* no HIP device / device index out of range */
+#define CLIFFS_ERR_STATE (-6)    /* call out of order (e.g. compute before upload) */
+
+typedef struct cliffs_handle cliffs_handle;
+
+/* Grid and run parameters: the module data `cliffs` reads
+ * (cliffs.f:19-20; global_modules.f:16-55). */
+typedef struct cliffs_grid_desc {
+    int nx, ny;          /* nXn, nYn */
+    const double *dep;   /* dep(nXn,nYn): depth, positive = water */
+    const double *s1;    /* s1(nXn-1,nYn): x spacing; may be NULL if nx <= 2 */
+    const double *s2;    /* s2(nYn-1): y spacing; may be NULL if ny <= 2 */
+    const double *zeta;  /* zeta(nYn): spherical / channel-width term; may be NULL if ny <= 2 */
+    double dt;           /* time step (PARAMETERS dt) */
+    double ground;       /* minimal flow depth, > 0 (PARAMETERS ground) */
+    double crough;       /* Manning n^2 (PARAMETERS crough) */
+    int device;          /* HIP device ordinal */
+} cliffs_grid_desc;
+
+/* Allocate device state for one grid.  Replaces the allocation of the
+ * MASTER arrays (cliffs_main.f:191-197) and precomputes celmin
+ * (cliffs_main.f:227) and the device form of dep/s1/s2/zeta. */
+int cliffs_create(const cliffs_grid_desc *desc, cliffs_handle **out);
+int cliffs_destroy(cliffs_handle *h);
+
+/* cel (celerity sqrt(g H), cliffs_main.f:218-226), xvel, yvel: FP64
+ * (nx,ny).  Upload converts to the device's FP32 perturbation form so that
+ * the wet/dry masks of the next sweep equal the FP64 ones bit for bit. */
+int cliffs_upload_state(cliffs_handle *h, const double *cel, const double *xvel,
+                        const double *yvel);
+int cliffs_download_state(cliffs_handle *h, double *cel, double *xvel, double *yvel);
+
+/* Running maxima hmax = max c, velmax = max(u^2+v^2) (cliffs_main.f:347-355). */
+int cliffs_download_max(cliffs_handle *h, double *hmax, double *velmax);
+int cliffs_reset_max(cliffs_handle *h); /* hmax = velmax = 0 (cliffs_main.f:196-197) */
+
+/* One directional sweep over lines first..last (1-based, inclusive):
+ * iu=1 rows along x, iu=2 columns along y.  Equivalent to
+ *     do irw = first, last; call cliffs(iu, irw, n); end do
+ * (cliffs_main.f:325-329 / 340-344).  edge1/edge2 as described above, mx
+ * their leading dimension. */
+int cliffs_sweep(cliffs_handle *h, int iu, int first, int last, const double *edge1,
+                 const double *edge2, int mx);
+
+/* Max-wave update over all cells (cliffs_main.f:347-355). */
+int cliffs_maxwave(cliffs_handle *h);
+
+/* One full time step (cliffs_main.f:317-355): x-sweep over rows jj1..jj2,
+ * y-sweep over columns ii1..ii2, max-wave -- with the 1-D/2-D line ranges
+ * of cliffs_main.f:140-164.  xedge1/xedge2 are the W/E edges of the x-sweep,
+ * yedge1/yedge2 the S/N edges of the y-sweep, already interpolated in time
+ * by the caller (cliffs_main.f:318-324, 333-339). */
+int cliffs_step(cliffs_handle *h, const double *xedge1, const double *xedge2,
+                const double *yedge1, const double *yedge2, int mx);
+
+/* nsteps steps with zero edges (open boundaries), enqueued without host
+ * synchronisation. */
+int cliffs_run(cliffs_handle *h, int nsteps);
+
+/* Like cliffs_run, bracketing every sweep launch with HIP events on the
+ * handle's stream.  Returns wall time of the whole enqueue (ms_total) and the
+ * mean device duration of the x- and y-sweep kernels. */
+int cliffs_run_timed(cliffs_handle *h, int nsteps, double *ms_total, double *ms_xsweep,
+                     double *ms_ysweep);
+
+/* Gauge sampling (output_gages.f:18-29): eta = c^2/g - d, or -1e32 (the
+ * reference's real*4 NAN) where c <= celmin; u = xvel, v = yvel.  I, J are
+ * 1-based. */
+int cliffs_gauges(cliffs_handle *h, int n, const int *I, const int *J, double *eta,
+                  double *u, double *v);
+
+int cliffs_sync(cliffs_handle *h);
+/* The handle's hipStream_t, for callers that enqueue their own work. */
+int cliffs_stream(cliffs_handle *h, void **stream);
+/* Gravity, celmin and line ranges as the device uses them. */
+int cliffs_info(cliffs_handle *h, double *grav, double *celmin, int *ii1, int *ii2, int *jj1,
+                int *jj2);
+const char *cliffs_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CLIFFS_GPU_H */
