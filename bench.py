@@ -65,7 +65,22 @@ def main():
                     help="steps of the per-kernel HIP-event pass")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (x5) to calibrate "
+                         "rocprofv3 FETCH_SIZE/WRITE_SIZE counters, then exit")
     args = ap.parse_args()
+
+    if args.copy_calib:
+        import torch
+        n = args.copy_calib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calib_mib": args.copy_calib, "launches": 5,
+                          "bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n}))
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; the path shards by scenario or by "

@@ -61,26 +61,34 @@ inline float to_cp(double c, double d, float s, const Consts64& k) {
     } else {
         cp = (float)c;
     }
-    for (int it = 0; it < 64; ++it) {
-        const float c32 = c0f(s) + cp;
-        const bool land32 = c32 <= k.celmin32;
-        if (land32 == want_land) break;
-        cp = std::nextafterf(cp, want_land ? -INFINITY : INFINITY);
+    auto mismatch = [&](float q) { return ((c0f(s) + q) <= k.celmin32) != want_land; };
+    auto ulp = [](float x) {
+        x = std::fabs(x);
+        return std::nextafterf(x, INFINITY) - x;
+    };
+    if (mismatch(cp)) {
+        // within rounding of the threshold: start from it and step by one
+        // ulp of the FP32 sum c0 + cp
+        cp = k.celmin32 - c0f(s);
+        const float step = std::fmax(ulp(c0f(s)), ulp(k.celmin32));
+        for (int it = 0; it < 64 && mismatch(cp); ++it) cp += want_land ? -step : step;
     }
     return cp;
 }
 
 // Device (s, cp) back to an FP64 celerity whose surface elevation
-// c^2/g - d equals the device's g*eta/g.
+// c^2/g - d equals the device's g*eta/g.  The FP64 predicate c <= celmin
+// of the result equals the device's FP32 predicate.
 inline double to_cel(float s, float cp, double d, const Consts64& k) {
     const float c32 = c0f(s) + cp;
-    if (c32 <= k.celmin32) return (double)c0f(s) + (double)cp;
+    if (c32 <= k.celmin32) return std::fmin((double)c0f(s) + (double)cp, k.celmin);
+    double c = (double)cp;
     if (s > 0.0f) {
         const double geta = (double)cp * (2.0 * (double)s + (double)cp);
         const double h = k.grav * d + geta;
-        return h > 0 ? std::sqrt(h) : (double)c0f(s) + (double)cp;
+        c = h > 0 ? std::sqrt(h) : (double)c0f(s) + (double)cp;
     }
-    return (double)cp;
+    return std::fmax(c, std::nextafter(k.celmin, INFINITY));
 }
 
 }  // namespace cliffs

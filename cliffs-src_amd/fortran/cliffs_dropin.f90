@@ -1,0 +1,107 @@
+! Drop-in replacements for the reference hot path, compiled together with
+! the driver's own MASTER / PARAMETERS modules (global_modules.f:16-55).
+!
+!   cliffs_gpu_attach(device)     after the H->c conversion (cliffs_main.f:227):
+!                                 device copy of dep/s1/s2/zeta/dt/ground/crough
+!                                 and of the state cel/xvel/yvel; hmax/velmax = 0
+!   cliffs_gpu_sweep(iu,first,last)
+!                                 one whole OpenMP loop of cliffs calls
+!                                 (cliffs_main.f:325-329 or 340-344), with the
+!                                 driver's edge1/edge2 (cliffs_main.f:318-339)
+!   cliffs_gpu_maxwave()          the max-wave loop (cliffs_main.f:347-355)
+!   cliffs_gpu_to_host()          state and maxima back into cel/xvel/yvel/
+!                                 hmax/velmax, before output_* or
+!                                 feed_children_grid read them
+!   cliffs_gpu_detach()
+!   subroutine cliffs(iu,irw,n)   the reference call surface (cliffs.f:17);
+!                                 one line on the device state.  One handle
+!                                 belongs to one host thread: call it outside
+!                                 OpenMP regions.
+module cliffs_dropin
+  use iso_c_binding
+  use cliffs_gpu_mod
+  implicit none
+  private
+  type(c_ptr), save, public :: cliffs_gpu_handle = c_null_ptr
+  public :: cliffs_gpu_attach, cliffs_gpu_sweep, cliffs_gpu_maxwave
+  public :: cliffs_gpu_to_host, cliffs_gpu_detach, cliffs_gpu_check
+
+contains
+
+  subroutine cliffs_gpu_check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: where
+    if (rc /= CLIFFS_OK) then
+      write(*, *) 'cliffs_gpu: ', where, ' failed: ', trim(cliffs_error_message())
+      stop 1
+    end if
+  end subroutine cliffs_gpu_check
+
+  subroutine cliffs_gpu_attach(device)
+    use MASTER, only: nXn, nYn, dep, cel, xvel, yvel, s1, s2, zeta
+    use PARAMETERS, only: dt, ground, crough
+    integer, intent(in) :: device
+    integer(c_int) :: rc
+    ! the module arrays are not TARGETs: their addresses are taken on the
+    ! TARGET dummies of cliffs_gpu_open, inside the call
+    if (allocated(s1) .and. allocated(s2)) then
+      rc = cliffs_gpu_open(cliffs_gpu_handle, nXn, nYn, dep, dt, ground, crough, device, &
+                           s1=s1, s2=s2, zeta=zeta)
+    else if (allocated(s1)) then
+      rc = cliffs_gpu_open(cliffs_gpu_handle, nXn, nYn, dep, dt, ground, crough, device, s1=s1)
+    else
+      rc = cliffs_gpu_open(cliffs_gpu_handle, nXn, nYn, dep, dt, ground, crough, device, &
+                           s2=s2, zeta=zeta)
+    end if
+    call cliffs_gpu_check(rc, 'cliffs_create')
+    call cliffs_gpu_check(cliffs_upload_state(cliffs_gpu_handle, cel, xvel, yvel), &
+                          'cliffs_upload_state')
+  end subroutine cliffs_gpu_attach
+
+  subroutine cliffs_gpu_sweep(iu, first, last)
+    use MASTER, only: edge1, edge2
+    integer, intent(in) :: iu, first, last
+    if (allocated(edge1)) then
+      call sweep_with_edges(iu, first, last, edge1, edge2)
+    else
+      call cliffs_gpu_check(cliffs_sweep(cliffs_gpu_handle, iu, first, last, c_null_ptr, &
+                                         c_null_ptr, 0), 'cliffs_sweep')
+    end if
+  end subroutine cliffs_gpu_sweep
+
+  subroutine sweep_with_edges(iu, first, last, e1, e2)
+    integer, intent(in) :: iu, first, last
+    real(c_double), intent(in), target, contiguous :: e1(:, :), e2(:, :)
+    call cliffs_gpu_check(cliffs_sweep(cliffs_gpu_handle, iu, first, last, c_loc(e1), &
+                                       c_loc(e2), int(size(e1, 1), c_int)), 'cliffs_sweep')
+  end subroutine sweep_with_edges
+
+  subroutine cliffs_gpu_maxwave()
+    call cliffs_gpu_check(cliffs_maxwave(cliffs_gpu_handle), 'cliffs_maxwave')
+  end subroutine cliffs_gpu_maxwave
+
+  subroutine cliffs_gpu_to_host()
+    use MASTER, only: cel, xvel, yvel, hmax, velmax
+    call cliffs_gpu_check(cliffs_download_state(cliffs_gpu_handle, cel, xvel, yvel), &
+                          'cliffs_download_state')
+    call cliffs_gpu_check(cliffs_download_max(cliffs_gpu_handle, hmax, velmax), &
+                          'cliffs_download_max')
+  end subroutine cliffs_gpu_to_host
+
+  subroutine cliffs_gpu_detach()
+    if (c_associated(cliffs_gpu_handle)) then
+      call cliffs_gpu_check(cliffs_destroy(cliffs_gpu_handle), 'cliffs_destroy')
+      cliffs_gpu_handle = c_null_ptr
+    end if
+  end subroutine cliffs_gpu_detach
+
+end module cliffs_dropin
+
+! The reference entry point (cliffs.f:17), advancing line irw of direction
+! iu on the device-resident state.
+subroutine cliffs(iu, irw, n)
+  use cliffs_dropin, only: cliffs_gpu_sweep
+  implicit none
+  integer iu, irw, n
+  call cliffs_gpu_sweep(iu, irw, irw)
+end subroutine cliffs

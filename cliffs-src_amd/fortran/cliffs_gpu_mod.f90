@@ -1,0 +1,158 @@
+! ISO_C_BINDING interface of libcliffs_gpu (include/cliffs_gpu.h) for the
+! Fortran driver.  Self-contained: it depends on no driver module, every
+! array is passed explicitly in the reference's (nX,nY) column-major layout.
+!
+! The drop-in subroutines with the reference call surface
+! (cliffs(iu,irw,n), cliffs.f:17) are in cliffs_dropin.f90, which is
+! compiled together with the driver's own MASTER/PARAMETERS modules.
+module cliffs_gpu_mod
+  use iso_c_binding
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: CLIFFS_OK = 0
+  integer(c_int), parameter, public :: CLIFFS_ERR_ARG = -1
+  integer(c_int), parameter, public :: CLIFFS_ERR_HIP = -2
+  integer(c_int), parameter, public :: CLIFFS_ERR_NOMEM = -3
+  integer(c_int), parameter, public :: CLIFFS_ERR_METRIC = -4
+  integer(c_int), parameter, public :: CLIFFS_ERR_NODEVICE = -5
+  integer(c_int), parameter, public :: CLIFFS_ERR_STATE = -6
+
+  ! struct cliffs_grid_desc
+  type, bind(C), public :: cliffs_grid_desc
+    integer(c_int) :: nx, ny
+    type(c_ptr) :: dep, s1, s2, zeta
+    real(c_double) :: dt, ground, crough
+    integer(c_int) :: device
+  end type cliffs_grid_desc
+
+  public :: cliffs_create, cliffs_destroy, cliffs_upload_state, cliffs_download_state
+  public :: cliffs_download_max, cliffs_reset_max, cliffs_sweep, cliffs_maxwave
+  public :: cliffs_step, cliffs_run, cliffs_run_timed, cliffs_gauges, cliffs_sync
+  public :: cliffs_stream, cliffs_info, cliffs_last_error
+  public :: cliffs_gpu_open, cliffs_error_message
+
+  interface
+    integer(c_int) function cliffs_create(desc, h) bind(C, name="cliffs_create")
+      import :: c_int, c_ptr, cliffs_grid_desc
+      type(cliffs_grid_desc), intent(in) :: desc
+      type(c_ptr), intent(out) :: h
+    end function
+    integer(c_int) function cliffs_destroy(h) bind(C, name="cliffs_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function cliffs_upload_state(h, cel, xvel, yvel) bind(C, name="cliffs_upload_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: cel(*), xvel(*), yvel(*)
+    end function
+    integer(c_int) function cliffs_download_state(h, cel, xvel, yvel) bind(C, name="cliffs_download_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: cel(*), xvel(*), yvel(*)
+    end function
+    integer(c_int) function cliffs_download_max(h, hmax, velmax) bind(C, name="cliffs_download_max")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: hmax(*), velmax(*)
+    end function
+    integer(c_int) function cliffs_reset_max(h) bind(C, name="cliffs_reset_max")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    ! edge1/edge2 are passed as c_ptr so that absent edges can be c_null_ptr
+    integer(c_int) function cliffs_sweep(h, iu, first, last, edge1, edge2, mx) bind(C, name="cliffs_sweep")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, edge1, edge2
+      integer(c_int), value :: iu, first, last, mx
+    end function
+    integer(c_int) function cliffs_maxwave(h) bind(C, name="cliffs_maxwave")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function cliffs_step(h, xe1, xe2, ye1, ye2, mx) bind(C, name="cliffs_step")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, xe1, xe2, ye1, ye2
+      integer(c_int), value :: mx
+    end function
+    integer(c_int) function cliffs_run(h, nsteps) bind(C, name="cliffs_run")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: nsteps
+    end function
+    integer(c_int) function cliffs_run_timed(h, nsteps, ms_total, ms_x, ms_y) bind(C, name="cliffs_run_timed")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: nsteps
+      real(c_double), intent(out) :: ms_total, ms_x, ms_y
+    end function
+    integer(c_int) function cliffs_gauges(h, n, ig, jg, eta, u, v) bind(C, name="cliffs_gauges")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: n
+      integer(c_int), intent(in) :: ig(*), jg(*)
+      real(c_double), intent(out) :: eta(*), u(*), v(*)
+    end function
+    integer(c_int) function cliffs_sync(h) bind(C, name="cliffs_sync")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function cliffs_stream(h, stream) bind(C, name="cliffs_stream")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      type(c_ptr), intent(out) :: stream
+    end function
+    integer(c_int) function cliffs_info(h, grav, celmin, ii1, ii2, jj1, jj2) bind(C, name="cliffs_info")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: grav, celmin
+      integer(c_int), intent(out) :: ii1, ii2, jj1, jj2
+    end function
+    type(c_ptr) function cliffs_last_error() bind(C, name="cliffs_last_error")
+      import :: c_ptr
+    end function
+  end interface
+
+contains
+
+  ! Create a device grid from the arrays MasterGrid fills (MasterGrid.f:85-121)
+  ! and the PARAMETERS dt, ground, crough.  s1/s2/zeta may be absent for 1-D
+  ! grids (nx <= 2 or ny <= 2).
+  integer(c_int) function cliffs_gpu_open(h, nx, ny, dep, dt, ground, crough, device, s1, s2, zeta)
+    type(c_ptr), intent(out) :: h
+    integer, intent(in) :: nx, ny, device
+    real(c_double), intent(in), target :: dep(nx, ny)
+    real(c_double), intent(in) :: dt, ground, crough
+    real(c_double), intent(in), target, contiguous, optional :: s1(:, :), s2(:), zeta(:)
+    type(cliffs_grid_desc) :: d
+    d%nx = nx
+    d%ny = ny
+    d%dep = c_loc(dep)
+    d%s1 = c_null_ptr
+    d%s2 = c_null_ptr
+    d%zeta = c_null_ptr
+    if (present(s1)) d%s1 = c_loc(s1)
+    if (present(s2)) d%s2 = c_loc(s2)
+    if (present(zeta)) d%zeta = c_loc(zeta)
+    d%dt = dt
+    d%ground = ground
+    d%crough = crough
+    d%device = device
+    cliffs_gpu_open = cliffs_create(d, h)
+  end function cliffs_gpu_open
+
+  ! cliffs_last_error() as a Fortran string
+  function cliffs_error_message() result(msg)
+    character(len=256) :: msg
+    character(kind=c_char), pointer :: p(:)
+    integer :: i
+    msg = ' '
+    call c_f_pointer(cliffs_last_error(), p, [256])
+    do i = 1, 256
+      if (p(i) == c_null_char) exit
+      msg(i:i) = p(i)
+    end do
+  end function cliffs_error_message
+
+end module cliffs_gpu_mod

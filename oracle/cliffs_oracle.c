@@ -544,3 +544,50 @@ int orc_feed_child(const orc_grid *g, double cuke, double hmin,
     free(clm1);
     return 1;
 }
+
+void orc_apply_initial_conditions(int nx, int ny, const double *xq, const double *yq,
+                                  const double *qq, int kx, int ky, const double *xp,
+                                  const double *yp, double *pp) {
+    const double zero = (double)0.1e-20f; /* FP32 literal, apply_initial_conditions.f:12 */
+    const double x1 = xq[0] - zero, x2 = xq[nx - 1] + zero;
+    const double y1 = yq[0] - zero, y2 = yq[ny - 1] + zero;
+#define QQ(i, j) qq[(size_t)((i) - 1) + (size_t)nx * ((j) - 1)]
+#define PP(i, j) pp[(size_t)((i) - 1) + (size_t)kx * ((j) - 1)]
+    int ii, jj;
+    double xcc, ycc, q;
+    if (nx > 2 && ny > 2) {
+        for (int i = 1; i <= kx; i++) {
+            const double xx = xp[i - 1];
+            if (!(xx >= x1 && xx <= x2)) continue;
+            orc_pinpoint(xq, nx, xx, &ii, &xcc);
+            for (int j = 1; j <= ky; j++) {
+                const double yy = yp[j - 1];
+                if (!(yy >= y1 && yy <= y2)) continue;
+                orc_pinpoint(yq, ny, yy, &jj, &ycc);
+                q = (QQ(ii, jj) * xcc + QQ(ii + 1, jj) * (1 - xcc)) * ycc +
+                    (QQ(ii, jj + 1) * xcc + QQ(ii + 1, jj + 1) * (1 - xcc)) * (1 - ycc);
+                PP(i, j) = PP(i, j) + q;
+            }
+        }
+    }
+    if (ny == 1 && nx > 2) {
+        for (int i = 1; i <= kx; i++) {
+            const double xx = xp[i - 1];
+            if (!(xx >= x1 && xx <= x2)) continue;
+            orc_pinpoint(xq, nx, xx, &ii, &xcc);
+            q = QQ(ii, 1) * xcc + QQ(ii + 1, 1) * (1 - xcc);
+            for (int j = 1; j <= ky; j++) PP(i, j) = PP(i, j) + q;
+        }
+    }
+    if (nx == 1 && ny > 2) {
+        for (int j = 1; j <= ky; j++) {
+            const double yy = yp[j - 1];
+            if (!(yy >= y1 && yy <= y2)) continue;
+            orc_pinpoint(yq, ny, yy, &jj, &ycc);
+            q = QQ(1, jj) * ycc + QQ(1, jj + 1) * (1 - ycc);
+            for (int i = 1; i <= kx; i++) PP(i, j) = PP(i, j) + q;
+        }
+    }
+#undef QQ
+#undef PP
+}

@@ -99,6 +99,13 @@ int orc_feed_child(const orc_grid *g, double cuke, double hmin,
                    int *iarv,
                    double *uvq_w, double *uvq_e, double *uvq_s, double *uvq_n);
 
+/* apply_initial_conditions (apply_initial_conditions.f:7-70): linear
+ * (1-D) or bilinear (2-D) interpolation of qq(nx,ny) on lattice xq, yq,
+ * ADDED onto pp(kx,ky) on lattice xp, yp. */
+void orc_apply_initial_conditions(int nx, int ny, const double *xq, const double *yq,
+                                  const double *qq, int kx, int ky, const double *xp,
+                                  const double *yp, double *pp);
+
 #ifdef __cplusplus
 }
 #endif

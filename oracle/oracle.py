@@ -117,6 +117,8 @@ class Oracle(_Base):
                                            C.c_int, C.c_double, C.c_int, C.c_double,
                                            _ip, _dp, _dp, _dp, _dp]
             lib.orc_feed_child.restype = C.c_int
+            lib.orc_apply_initial_conditions.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp,
+                                                         C.c_int, C.c_int, _dp, _dp, _dp]
             cls._lib = lib
         return cls._lib
 
@@ -306,3 +308,17 @@ def feed_child(orc: Oracle, maps: dict, cuke: float, hmin: float, iarv: bool):
     if rc < 0:
         raise MemoryError("orc_feed_child")
     return bool(flag.value), w.T.copy(), e.T.copy(), s.T.copy(), n.T.copy()
+
+
+def apply_initial_conditions(xq, yq, qq, xp, yp, pp):
+    """apply_initial_conditions.f:7-70.  qq (nyq, nxq), pp (nyp, nxp) numpy
+    (Fortran (nx,ny) bytes); returns pp + interpolated qq."""
+    xq = np.ascontiguousarray(xq, dtype=np.float64)
+    yq = np.ascontiguousarray(yq, dtype=np.float64)
+    qq = np.ascontiguousarray(qq, dtype=np.float64)
+    xp = np.ascontiguousarray(xp, dtype=np.float64)
+    yp = np.ascontiguousarray(yp, dtype=np.float64)
+    out = np.array(pp, dtype=np.float64, order="C", copy=True)
+    Oracle.lib().orc_apply_initial_conditions(len(xq), len(yq), _ptr(xq), _ptr(yq), _ptr(qq),
+                                              len(xp), len(yp), _ptr(xp), _ptr(yp), _ptr(out))
+    return out
