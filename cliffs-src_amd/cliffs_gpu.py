@@ -72,6 +72,14 @@ def load_library():
         "cliffs_stream": ([H, C.POINTER(C.c_void_p)], C.c_int),
         "cliffs_info": ([H, _dp, _dp, _ip, _ip, _ip, _ip], C.c_int),
         "cliffs_last_error": ([], C.c_char_p),
+        "cliffs_nest_maps": ([C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp,
+                              _ip, _dp, _ip, _dp, _ip, _dp, _ip, _dp, _ip, _dp, _ip, _dp],
+                             C.c_int),
+        "cliffs_nest_create": ([H, C.c_int, C.c_int, _ip, _dp, _ip, _dp, C.c_int, C.c_double,
+                                C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, C.c_double,
+                                C.POINTER(H)], C.c_int),
+        "cliffs_nest_destroy": ([H], C.c_int),
+        "cliffs_nest_feed": ([H, C.c_double, C.c_double, _ip, _ip, _dp, _dp, _dp, _dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -221,3 +229,63 @@ class Grid:
         s = C.c_void_p()
         _chk(load_library().cliffs_stream(self._h, C.byref(s)))
         return s.value or 0
+
+
+def nest_maps(xp, yp, xc, yc):
+    """Parent index maps of one child grid (NestedGrid.f:104-119), computed by
+    the library's host code.  Returns a dict of 1-based maps, as the NESTED
+    module holds them for one child."""
+    lib = load_library()
+    xp, yp, xc, yc = (_f64(a) for a in (xp, yp, xc, yc))
+    nxc, nyc = len(xc), len(yc)
+    we_ny = np.zeros(nyc, dtype=np.int32)
+    sn_nx = np.zeros(nxc, dtype=np.int32)
+    we_cy = np.zeros(nyc)
+    sn_cx = np.zeros(nxc)
+    sc = [C.c_int(), C.c_double(), C.c_int(), C.c_double(), C.c_int(), C.c_double(),
+          C.c_int(), C.c_double()]
+    _chk(lib.cliffs_nest_maps(len(xp), _p(xp), len(yp), _p(yp), nxc, _p(xc), nyc, _p(yc),
+                              we_ny.ctypes.data_as(_ip), _p(we_cy), sn_nx.ctypes.data_as(_ip),
+                              _p(sn_cx), *[C.byref(s) for s in sc]))
+    return dict(we_ny=we_ny, we_cy=we_cy, sn_nx=sn_nx, sn_cx=sn_cx,
+                w_nx=sc[0].value, w_cx=sc[1].value, e_nx=sc[2].value, e_cx=sc[3].value,
+                s_ny=sc[4].value, s_cy=sc[5].value, n_ny=sc[6].value, n_cy=sc[7].value)
+
+
+class Nest:
+    """One child grid's boundary feed from a parent Grid (feed_children_grid)."""
+
+    def __init__(self, parent: Grid, maps: dict):
+        lib = load_library()
+        self.parent = parent
+        self.nyc, self.nxc = len(maps["we_ny"]), len(maps["sn_nx"])
+        we_ny = np.ascontiguousarray(maps["we_ny"], dtype=np.int32)
+        sn_nx = np.ascontiguousarray(maps["sn_nx"], dtype=np.int32)
+        we_cy, sn_cx = _f64(maps["we_cy"]), _f64(maps["sn_cx"])
+        h = C.c_void_p()
+        _chk(lib.cliffs_nest_create(parent._h, self.nxc, self.nyc, we_ny.ctypes.data_as(_ip),
+                                    _p(we_cy), sn_nx.ctypes.data_as(_ip), _p(sn_cx),
+                                    int(maps["w_nx"]), float(maps["w_cx"]), int(maps["e_nx"]),
+                                    float(maps["e_cx"]), int(maps["s_ny"]), float(maps["s_cy"]),
+                                    int(maps["n_ny"]), float(maps["n_cy"]), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().cliffs_nest_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def feed(self, cuke, hmin, iarv=False):
+        """-> (arrived, fed, uvq_w (nyc,3), uvq_e, uvq_s (nxc,3), uvq_n)."""
+        a, f = C.c_int(1 if iarv else 0), C.c_int(0)
+        w, e = np.zeros((3, self.nyc)), np.zeros((3, self.nyc))
+        s, n = np.zeros((3, self.nxc)), np.zeros((3, self.nxc))
+        _chk(load_library().cliffs_nest_feed(self._h, float(cuke), float(hmin), C.byref(a),
+                                             C.byref(f), _p(w), _p(e), _p(s), _p(n)))
+        return bool(a.value), bool(f.value), w.T.copy(), e.T.copy(), s.T.copy(), n.T.copy()

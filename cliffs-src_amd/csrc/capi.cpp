@@ -20,6 +20,8 @@ hipError_t launch_sweep_y(const YArgs& a, hipStream_t st);
 hipError_t launch_line_flags(const DevGrid& g, const Params& p, unsigned char* fout, int dir,
                              unsigned* fclr, int nclr, hipStream_t st);
 hipError_t launch_maxwave(const DevGrid& g, hipStream_t st);
+hipError_t launch_nest_arrival(const NestArgs& a, hipStream_t st);
+hipError_t launch_nest_feed(const NestArgs& a, hipStream_t st);
 }  // namespace cliffs
 
 using namespace cliffs;
@@ -42,6 +44,21 @@ int hipfail(hipError_t e, const char* where) {
     } while (0)
 
 constexpr int kEdgeSlots = 16;
+
+// Left-next lattice index by bisection and its interpolation weight
+// (NestedGrid.f:129-148): 1-based ip with xx(ip) < xp <= xx(ip+1), clamped
+// to [1, nx-1]; cp = (xx(ip+1)-xp)/(xx(ip+1)-xx(ip)), 0.5 for nx = 1.  The
+// bisection order is kept so that maps equal the reference's bit for bit.
+void pinpoint(const double* xx, int nx, double xp, int* ip_out, double* cp) {
+    int ip = 1, ipp = nx;
+    while (ip + 2 <= ipp) {
+        const int med = (ip + ipp) / 2;
+        if (xp > xx[med - 1]) ip = med;
+        else ipp = med;
+    }
+    *cp = nx > 1 ? (xx[ip] - xp) / (xx[ip] - xx[ip - 1]) : 0.5;
+    *ip_out = ip;
+}
 
 }  // namespace
 
@@ -89,6 +106,22 @@ struct cliffs_handle {
         return g;
     }
     unsigned char* fbuf(int i) { return flags + (size_t)(((i % 3) + 3) % 3) * fbytes; }
+};
+
+struct cliffs_nest {
+    cliffs_handle* parent = nullptr;
+    int nxc = 0, nyc = 0;
+    int* we_ny = nullptr;   // device, 1-based parent rows
+    int* sn_nx = nullptr;   // device, 1-based parent columns
+    double* we_cy = nullptr;
+    double* sn_cx = nullptr;
+    int w_nx = 0, e_nx = 0, s_ny = 0, n_ny = 0;
+    double w_cx = 0, e_cx = 0, s_cy = 0, n_cy = 0;
+    int ii1 = 0, ii2 = 0, kk1 = 0, kk2 = 0;
+    unsigned* flag = nullptr;
+    double* uvq = nullptr;
+    size_t nuvq = 0;
+    std::vector<double> huvq;
 };
 
 namespace {
@@ -646,6 +679,142 @@ int cliffs_gauges(cliffs_handle* h, int n, const int* I, const int* J, double* e
         u[q] = uu;
         v[q] = vv;
     }
+    return CLIFFS_OK;
+}
+
+// ---- one-way nesting -------------------------------------------------------
+
+int cliffs_nest_maps(int nxp, const double* xp, int nyp, const double* yp, int nxc,
+                     const double* xc, int nyc, const double* yc, int* we_ny, double* we_cy,
+                     int* sn_nx, double* sn_cx, int* w_nx, double* w_cx, int* e_nx, double* e_cx,
+                     int* s_ny, double* s_cy, int* n_ny, double* n_cy) {
+    if (nxp < 2 || nyp < 2 || nxc < 1 || nyc < 1 || !xp || !yp || !xc || !yc || !we_ny ||
+        !we_cy || !sn_nx || !sn_cx || !w_nx || !w_cx || !e_nx || !e_cx || !s_ny || !s_cy ||
+        !n_ny || !n_cy)
+        return fail(CLIFFS_ERR_ARG, "bad nest map arguments (nests need a 2-D parent)");
+    // containment check of NestedGrid.f:84-96
+    if (xc[0] < xp[0] || xc[nxc - 1] > xp[nxp - 1] || yc[0] < yp[0] || yc[nyc - 1] > yp[nyp - 1])
+        return fail(CLIFFS_ERR_ARG, "child grid not within the parent grid limits");
+    pinpoint(yp, nyp, yc[0], s_ny, s_cy);
+    pinpoint(yp, nyp, yc[nyc - 1], n_ny, n_cy);
+    for (int i = 0; i < nxc; i++) pinpoint(xp, nxp, xc[i], &sn_nx[i], &sn_cx[i]);
+    pinpoint(xp, nxp, xc[0], w_nx, w_cx);
+    pinpoint(xp, nxp, xc[nxc - 1], e_nx, e_cx);
+    for (int i = 0; i < nyc; i++) pinpoint(yp, nyp, yc[i], &we_ny[i], &we_cy[i]);
+    return CLIFFS_OK;
+}
+
+int cliffs_nest_create(cliffs_handle* h, int nxc, int nyc, const int* we_ny, const double* we_cy,
+                       const int* sn_nx, const double* sn_cx, int w_nx, double w_cx, int e_nx,
+                       double e_cx, int s_ny, double s_cy, int n_ny, double n_cy,
+                       cliffs_nest** out) {
+    if (!h || !out || nxc < 1 || nyc < 1 || !we_ny || !we_cy || !sn_nx || !sn_cx)
+        return fail(CLIFFS_ERR_ARG, "bad nest arguments");
+    *out = nullptr;
+    auto okx = [&](int i) { return i >= 1 && i <= h->nx - 1; };
+    auto oky = [&](int j) { return j >= 1 && j <= h->ny - 1; };
+    for (int q = 0; q < nyc; q++)
+        if (!oky(we_ny[q])) return fail(CLIFFS_ERR_ARG, "we_ny out of the parent grid");
+    for (int q = 0; q < nxc; q++)
+        if (!okx(sn_nx[q])) return fail(CLIFFS_ERR_ARG, "sn_nx out of the parent grid");
+    if (!okx(w_nx) || !okx(e_nx) || !oky(s_ny) || !oky(n_ny))
+        return fail(CLIFFS_ERR_ARG, "edge line index out of the parent grid");
+    HIPCHK(hipSetDevice(h->dev));
+    cliffs_nest* n = new (std::nothrow) cliffs_nest;
+    if (!n) return fail(CLIFFS_ERR_NOMEM, "host allocation");
+    n->parent = h;
+    n->nxc = nxc;
+    n->nyc = nyc;
+    n->w_nx = w_nx; n->e_nx = e_nx; n->s_ny = s_ny; n->n_ny = n_ny;
+    n->w_cx = w_cx; n->e_cx = e_cx; n->s_cy = s_cy; n->n_cy = n_cy;
+    n->ii1 = sn_nx[0];
+    n->ii2 = sn_nx[nxc - 1] + 1;
+    n->kk1 = we_ny[0];
+    n->kk2 = we_ny[nyc - 1] + 1;
+    n->nuvq = (size_t)6 * nyc + (size_t)6 * nxc;
+    if (hipMalloc(&n->we_ny, sizeof(int) * nyc) != hipSuccess ||
+        hipMalloc(&n->sn_nx, sizeof(int) * nxc) != hipSuccess ||
+        hipMalloc(&n->we_cy, sizeof(double) * nyc) != hipSuccess ||
+        hipMalloc(&n->sn_cx, sizeof(double) * nxc) != hipSuccess ||
+        hipMalloc(&n->flag, sizeof(unsigned)) != hipSuccess ||
+        hipMalloc(&n->uvq, sizeof(double) * n->nuvq) != hipSuccess) {
+        cliffs_nest_destroy(n);
+        return fail(CLIFFS_ERR_NOMEM, "nest device allocation");
+    }
+    if (hipMemcpy(n->we_ny, we_ny, sizeof(int) * nyc, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(n->sn_nx, sn_nx, sizeof(int) * nxc, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(n->we_cy, we_cy, sizeof(double) * nyc, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(n->sn_cx, sn_cx, sizeof(double) * nxc, hipMemcpyHostToDevice) != hipSuccess) {
+        cliffs_nest_destroy(n);
+        return fail(CLIFFS_ERR_HIP, "nest map upload");
+    }
+    n->huvq.resize(n->nuvq);
+    *out = n;
+    return CLIFFS_OK;
+}
+
+int cliffs_nest_destroy(cliffs_nest* n) {
+    if (!n) return CLIFFS_OK;
+    if (n->parent) (void)hipSetDevice(n->parent->dev);
+    (void)hipFree(n->we_ny);
+    (void)hipFree(n->sn_nx);
+    (void)hipFree(n->we_cy);
+    (void)hipFree(n->sn_cx);
+    (void)hipFree(n->flag);
+    (void)hipFree(n->uvq);
+    delete n;
+    return CLIFFS_OK;
+}
+
+int cliffs_nest_feed(cliffs_nest* n, double cuke, double hmin, int* iarv, int* fed,
+                     double* uvq_w, double* uvq_e, double* uvq_s, double* uvq_n) {
+    if (!n || !iarv || !fed || !uvq_w || !uvq_e || !uvq_s || !uvq_n)
+        return fail(CLIFFS_ERR_ARG, "NULL argument");
+    cliffs_handle* h = n->parent;
+    if (!h->state) return fail(CLIFFS_ERR_STATE, "no state uploaded");
+    // the driver passes `ground` (cliffs_main.f:297, 362); the device decides
+    // dep > ground exactly, any other threshold is not supported
+    if (hmin != h->k.ground) return fail(CLIFFS_ERR_ARG, "hmin must equal the grid's ground");
+    HIPCHK(hipSetDevice(h->dev));
+    NestArgs a{};
+    a.g = h->grid();
+    a.p = h->p;
+    a.grav = h->k.grav;
+    a.nxc = n->nxc;
+    a.nyc = n->nyc;
+    a.we_ny = n->we_ny;
+    a.we_cy = n->we_cy;
+    a.sn_nx = n->sn_nx;
+    a.sn_cx = n->sn_cx;
+    a.w_nx = n->w_nx; a.e_nx = n->e_nx; a.s_ny = n->s_ny; a.n_ny = n->n_ny;
+    a.w_cx = n->w_cx; a.e_cx = n->e_cx; a.s_cy = n->s_cy; a.n_cy = n->n_cy;
+    a.ii1 = n->ii1; a.ii2 = n->ii2; a.kk1 = n->kk1; a.kk2 = n->kk2;
+    a.gthr = (float)(h->k.grav * ((double)0.1f * cuke));  // feed_children_grid.f:30
+    a.cuke = (float)cuke;
+    a.flag = n->flag;
+    a.uvq = n->uvq;
+    *fed = 0;
+    if (!*iarv) {
+        HIPCHK(hipMemsetAsync(n->flag, 0, sizeof(unsigned), h->st));
+        hipError_t e = launch_nest_arrival(a, h->st);
+        if (e != hipSuccess) return hipfail(e, "nest_arrival");
+        unsigned f = 0;
+        HIPCHK(hipMemcpyAsync(&f, n->flag, sizeof(unsigned), hipMemcpyDeviceToHost, h->st));
+        HIPCHK(hipStreamSynchronize(h->st));
+        *iarv = f ? 1 : 0;
+    }
+    if (!*iarv) return CLIFFS_OK;
+    hipError_t e = launch_nest_feed(a, h->st);
+    if (e != hipSuccess) return hipfail(e, "nest_feed");
+    HIPCHK(hipMemcpyAsync(n->huvq.data(), n->uvq, sizeof(double) * n->nuvq, hipMemcpyDeviceToHost,
+                          h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    const size_t ly = (size_t)3 * n->nyc, lx = (size_t)3 * n->nxc;
+    std::memcpy(uvq_w, n->huvq.data(), sizeof(double) * ly);
+    std::memcpy(uvq_e, n->huvq.data() + ly, sizeof(double) * ly);
+    std::memcpy(uvq_s, n->huvq.data() + 2 * ly, sizeof(double) * lx);
+    std::memcpy(uvq_n, n->huvq.data() + 2 * ly + lx, sizeof(double) * lx);
+    *fed = 1;
     return CLIFFS_OK;
 }
 

@@ -68,4 +68,23 @@ struct YArgs {
     int fuse_max;         // fuse the max-wave update (2-D step)
 };
 
+// One-way nest feed (nest.hip)
+struct NestArgs {
+    DevGrid g;
+    Params p;
+    double grav;
+    int nxc, nyc;
+    const int* we_ny;      // 1-based parent rows, per child row
+    const double* we_cy;
+    const int* sn_nx;      // 1-based parent columns, per child column
+    const double* sn_cx;
+    int w_nx, e_nx, s_ny, n_ny;
+    double w_cx, e_cx, s_cy, n_cy;
+    int ii1, ii2, kk1, kk2;  // footprint (1-based, inclusive)
+    float gthr;              // g * 0.1f * cuke
+    float cuke;
+    unsigned* flag;
+    double* uvq;             // west (nyc,3), east (nyc,3), south (nxc,3), north (nxc,3)
+};
+
 }  // namespace cliffs

@@ -1,0 +1,93 @@
+// One-way nest feed on the device (feed_children_grid.f:10-179).
+//
+// arrival_kernel: any wet-enough parent cell of the child footprint with
+//   |eta| > 0.1*cuke or (|u|+|v|) c > cuke  (feed_children_grid.f:26-46);
+//   dry cells (dep <= hmin, here the device predicate s <= celmin, which
+//   equals dep <= ground bit for bit) are skipped.  Any thread that finds
+//   one stores 1 into the flag (only 1s are ever stored).
+// feed_kernel: one thread per child edge node; the two-point blend across
+//   the parent lines on either side of the child edge followed by the
+//   two-point blend along the edge (feed_children_grid.f:53-80 and the
+//   three mirrored blocks), in FP64 from the FP32 state, dry parent cells
+//   contributing 0.
+#include <hip/hip_runtime.h>
+
+#include "cliffs_dev.h"
+
+namespace cliffs {
+
+__global__ void nest_arrival_kernel(const NestArgs a) {
+    const int i = a.ii1 + blockIdx.x * blockDim.x + threadIdx.x;  // 1-based column
+    const int k = a.kk1 + blockIdx.y;                              // 1-based row
+    if (i > a.ii2 || k > a.kk2) return;
+    const size_t o = (size_t)(k - 1) * a.g.pitch + kOff + (i - 1);
+    const float s = a.g.S[o];
+    if (!is_deep(s, a.p)) return;
+    const float cp = a.g.CP[o];
+    const float ge = geta_of(s, cp);
+    const float c = c0_of(s) + cp;
+    const bool hit = fabsf(ge) > a.gthr ||
+                     (fabsf(a.g.U[o]) + fabsf(a.g.V[o])) * c > a.cuke;
+    if (hit) *a.flag = 1u;
+}
+
+// value triple (u, v, eta) of parent cell (i,k), 1-based, times weight w
+__device__ __forceinline__ void feed_val(const NestArgs& a, int i, int k, double w, double out[3]) {
+    const size_t o = (size_t)(k - 1) * a.g.pitch + kOff + (i - 1);
+    const float s = a.g.S[o];
+    if (is_deep(s, a.p)) {
+        out[0] = w * (double)a.g.U[o];
+        out[1] = w * (double)a.g.V[o];
+        out[2] = w * ((double)geta_of(s, a.g.CP[o]) / a.grav);
+    } else {
+        out[0] = out[1] = out[2] = 0.0;
+    }
+}
+
+__global__ void nest_feed_kernel(const NestArgs a) {
+    const int side = blockIdx.y;  // 0 west, 1 east, 2 south, 3 north
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int len = side < 2 ? a.nyc : a.nxc;
+    if (q >= len) return;
+    double v0[3], v1[3], w0[3], w1[3];
+    double* out;
+    double cc;
+    if (side < 2) {
+        const int ii = side == 0 ? a.w_nx : a.e_nx;
+        const double c0 = side == 0 ? a.w_cx : a.e_cx;
+        const int kk = a.we_ny[q];
+        cc = a.we_cy[q];
+        // clm(kk) and clm(kk+1): blend of parent columns ii and ii+1
+        feed_val(a, ii, kk, c0, v0);
+        feed_val(a, ii + 1, kk, 1.0 - c0, v1);
+        feed_val(a, ii, kk + 1, c0, w0);
+        feed_val(a, ii + 1, kk + 1, 1.0 - c0, w1);
+        out = a.uvq + (size_t)side * 3 * a.nyc;
+    } else {
+        const int kk = side == 2 ? a.s_ny : a.n_ny;
+        const double c0 = side == 2 ? a.s_cy : a.n_cy;
+        const int ii = a.sn_nx[q];
+        cc = a.sn_cx[q];
+        feed_val(a, ii, kk, c0, v0);
+        feed_val(a, ii, kk + 1, 1.0 - c0, v1);
+        feed_val(a, ii + 1, kk, c0, w0);
+        feed_val(a, ii + 1, kk + 1, 1.0 - c0, w1);
+        out = a.uvq + (size_t)6 * a.nyc + (size_t)(side - 2) * 3 * a.nxc;
+    }
+#pragma unroll
+    for (int f = 0; f < 3; f++) out[q + (size_t)len * f] = (v0[f] + v1[f]) * cc + (w0[f] + w1[f]) * (1.0 - cc);
+}
+
+hipError_t launch_nest_arrival(const NestArgs& a, hipStream_t st) {
+    const int nxb = a.ii2 - a.ii1 + 1, nyb = a.kk2 - a.kk1 + 1;
+    hipLaunchKernelGGL(nest_arrival_kernel, dim3((nxb + 255) / 256, nyb), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_nest_feed(const NestArgs& a, hipStream_t st) {
+    const int len = a.nxc > a.nyc ? a.nxc : a.nyc;
+    hipLaunchKernelGGL(nest_feed_kernel, dim3((len + 127) / 128, 4), dim3(128), 0, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace cliffs

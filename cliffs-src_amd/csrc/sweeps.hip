@@ -380,30 +380,55 @@ __global__ __launch_bounds__(256) void sweep_y_kernel(const YArgs a) {
     float ocp0 = 0.f, ou0 = 0.f, ov0 = 0.f, s0 = 0.f;
     float lc0 = 0.f, lu0 = 0.f, lcm1 = 0.f, lum1 = 0.f;
     unsigned rowwet = 0u;  // lane l: bit0 -> row j0+l, bit1 -> row j0+64+l
+    int okrun = 0;         // consecutive loaded rows that are wet and deeper than ground
     bool colwet = false;
 
     for (int r = j0 - 4; r < j1; ++r) {
         // 1. new row r+3
         const Raw rC = load_raw(a, col, r + 3, col_ok);
-        const bool reC = rC.land && !rB.land && floods(rB.ge, rC.s, p) && rB.u >= 0.0f;
-        const bool startC = !rC.land && rB.land && !reB;
-        // 2. expansion decisions and post-expansion state of row r+2
-        const bool leB = rB.valid && startC && floods(rC.ge, rB.s, p) && rC.u <= 0.0f;
-        const bool nlB = rB.land && !reB && !leB;
-        const float c0B = c0_of(rB.s);
+        okrun = (rC.valid && !rC.land && rC.deep) ? okrun + 1 : 0;
+        const int row1 = r + 1;
+        // fast path: every swept lane has wet, deep rows r-1..r+3 and row r+1
+        // is interior -- no expansion, ghost, limiter or edge can apply
+        const bool fast = row1 >= 1 && row1 <= ny - 2 && wave_all(!swept || okrun >= 5);
+        bool reC, leB, nlB;
         Node peB;
-        peB.s = rB.s;
-        peB.cp = (reB || leB) ? p.celmin - c0B : (nlB ? -c0B : rB.cp);
-        peB.u = reB ? uA : (leB ? rC.u : (nlB ? 0.0f : rB.u));
-        peB.v = reB ? vA : (leB ? rC.v : (nlB ? 0.0f : rB.v));
-        const float pcB = c0B + peB.cp;
-        const float pgB = geta_of(peB.s, peB.cp);
+        float pcB, pgB;
+        if (fast) {
+            reC = leB = nlB = false;
+            peB = Node{rB.s, rB.cp, rB.u, rB.v};
+            pcB = rB.c;
+            pgB = rB.ge;
+        } else {
+            reC = rC.land && !rB.land && floods(rB.ge, rC.s, p) && rB.u >= 0.0f;
+            const bool startC = !rC.land && rB.land && !reB;
+            // 2. expansion decisions and post-expansion state of row r+2
+            leB = rB.valid && startC && floods(rC.ge, rB.s, p) && rC.u <= 0.0f;
+            nlB = rB.land && !reB && !leB;
+            const float c0B = c0_of(rB.s);
+            peB.s = rB.s;
+            peB.cp = (reB || leB) ? p.celmin - c0B : (nlB ? -c0B : rB.cp);
+            peB.u = reB ? uA : (leB ? rC.u : (nlB ? 0.0f : rB.u));
+            peB.v = reB ? vA : (leB ? rC.v : (nlB ? 0.0f : rB.v));
+            pcB = c0B + peB.cp;
+            pgB = geta_of(peB.s, peB.cp);
+        }
         // 3. residual across rows r+1 -> r+2, update of row r+1
         const EdgeRes E1 = edge_res(pe1, pc1, pg1, peB, pcB, pgB);
-        const int row1 = r + 1;
         float ocp1 = pe1.cp, ou1 = pe1.u, ov1 = pe1.v;
         const bool end1 = (row1 == 0) || (row1 == ny - 1);
-        if (swept && !nl1 && row1 >= 0 && row1 < ny) {
+        if (fast) {
+            if (swept) {
+                const float4 ym = a.ym[row1];
+                Metric m;
+                m.rfm = ym.x;
+                m.rfp = ym.y;
+                m.rsum = ym.z;
+                m.fpm = ym.w;
+                m.scl = a.zeta[row1];
+                interior_update(pe0, pc0, pe1, pc1, peB, pcB, E0, E1, true, m, p, ocp1, ou1, ov1);
+            }
+        } else if (swept && !nl1 && row1 >= 0 && row1 < ny) {
             if (row1 >= 1 && row1 <= ny - 2) {
                 const float4 ym = a.ym[row1];
                 Metric m;

@@ -64,6 +64,15 @@ struct Metric {
     float scl;   // zeta(i) in the y-sweep, 0 in the x-sweep
 };
 
+// Reciprocal: hardware v_rcp_f32 (1 ulp) on the device, a division on the host.
+CLIFFS_HD float fast_rcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+
 CLIFFS_HD float c0_of(float s) { return fmaxf(s, 0.0f); }
 CLIFFS_HD float cel_of(const Node& n) { return c0_of(n.s) + n.cp; }
 CLIFFS_HD float geta_of(float s, float cp) {
@@ -112,10 +121,11 @@ CLIFFS_HD void interior_update(const Node& L, float cL, const Node& X, float cX,
     const float u = X.u, v = X.v, c = cX;
     const float dtx = p.dt * m.rsum;
 
-    float F = 0.0f;  // almost-Manning friction g n^2 u sqrt((u^2+v^2)/H)/H
+    float F = 0.0f;  // almost-Manning friction g n^2 u sqrt((u^2+v^2)/H)/H, H = c^2/g
     if (p.friction) {
-        const float H = c * c / p.grav;
-        F = p.gn2 * u * sqrtf((u * u + v * v) / H) / H;
+        // = g n^2 * g * u * sqrt(g (u^2+v^2)) / c^3   (c >= celmin > 0 here)
+        const float rc = fast_rcp(c);
+        F = p.gn2 * p.grav * u * sqrtf(p.grav * (u * u + v * v)) * (rc * rc * rc);
     }
     const float S = 8.0f * c * u * m.scl;  // (P-Q)(P+Q) zeta
 

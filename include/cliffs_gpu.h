@@ -111,6 +111,39 @@ int cliffs_run_timed(cliffs_handle *h, int nsteps, double *ms_total, double *ms_
 int cliffs_gauges(cliffs_handle *h, int n, const int *I, const int *J, double *eta,
                   double *u, double *v);
 
+/* ---- one-way nesting: parent -> child boundary feed -------------------- */
+
+typedef struct cliffs_nest cliffs_nest;
+
+/* Parent index maps of one child grid (NestedGrid.f:104-119), computed on the
+ * host with the reference's bisection `pinpoint` (NestedGrid.f:129-148) in
+ * FP64.  xp/yp: parent lattice (nXn, nYn); xc/yc: child lattice (nxc, nyc).
+ * Outputs are 1-based parent indices and FP64 weights, laid out as the
+ * NESTED module holds them for one child: we_ny/we_cy (nyc), sn_nx/sn_cx
+ * (nxc), and the scalars of the four edges. */
+int cliffs_nest_maps(int nxp, const double *xp, int nyp, const double *yp, int nxc,
+                     const double *xc, int nyc, const double *yc, int *we_ny, double *we_cy,
+                     int *sn_nx, double *sn_cx, int *w_nx, double *w_cx, int *e_nx,
+                     double *e_cx, int *s_ny, double *s_cy, int *n_ny, double *n_cy);
+
+/* Attach one child's maps (as produced by NestedGrid or cliffs_nest_maps)
+ * to a parent handle; the maps are copied to the device. */
+int cliffs_nest_create(cliffs_handle *parent, int nxc, int nyc, const int *we_ny,
+                       const double *we_cy, const int *sn_nx, const double *sn_cx, int w_nx,
+                       double w_cx, int e_nx, double e_cx, int s_ny, double s_cy, int n_ny,
+                       double n_cy, cliffs_nest **out);
+int cliffs_nest_destroy(cliffs_nest *n);
+
+/* feed_children_grid (feed_children_grid.f:10-179) on the device state:
+ * while *iarv == 0, test wave arrival over the child footprint (|eta| >
+ * 0.1*cuke, or (|u|+|v|) c > cuke, on cells with dep > hmin) and latch *iarv;
+ * once arrived, interpolate (u, v, eta) onto the four child edges, dry parent
+ * cells contributing 0.  uvq_w/uvq_e are (nyc,3), uvq_s/uvq_n (nxc,3),
+ * column-major -- the arrays output_feed writes (output_feed.f:3-24).
+ * *fed is set to 1 when the edges were filled. */
+int cliffs_nest_feed(cliffs_nest *n, double cuke, double hmin, int *iarv, int *fed,
+                     double *uvq_w, double *uvq_e, double *uvq_s, double *uvq_n);
+
 int cliffs_sync(cliffs_handle *h);
 /* The handle's hipStream_t, for callers that enqueue their own work. */
 int cliffs_stream(cliffs_handle *h, void **stream);
