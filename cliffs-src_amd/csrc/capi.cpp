@@ -20,6 +20,7 @@ hipError_t launch_sweep_y(const YArgs& a, hipStream_t st);
 hipError_t launch_line_flags(const DevGrid& g, const Params& p, unsigned char* fout, int dir,
                              unsigned* fclr, int nclr, hipStream_t st);
 hipError_t launch_maxwave(const DevGrid& g, hipStream_t st);
+hipError_t launch_copy_lines(const DevGrid& g, int dir, int first, int last, hipStream_t st);
 hipError_t launch_nest_arrival(const NestArgs& a, hipStream_t st);
 hipError_t launch_nest_feed(const NestArgs& a, hipStream_t st);
 }  // namespace cliffs
@@ -74,7 +75,8 @@ struct cliffs_handle {
     bool xrun = false, yrun = false;
     int ii1 = 0, ii2 = 0, jj1 = 0, jj2 = 0, mx = 1;
     // device memory
-    float* fields = nullptr;   // S, CP, U, V, HM, VM
+    float* fields = nullptr;   // S, CP0, U0, V0, CP1, U1, V1, HM, VM
+    int cur = 0;               // which of the two state copies is current
     size_t fstride = 0;        // floats per field
     float4* xm = nullptr;
     float2* xr = nullptr;
@@ -98,11 +100,16 @@ struct cliffs_handle {
         g.ny = ny;
         g.pitch = pitch;
         g.S = fields;
-        g.CP = fields + fstride;
-        g.U = fields + 2 * fstride;
-        g.V = fields + 3 * fstride;
-        g.HM = fields + 4 * fstride;
-        g.VM = fields + 5 * fstride;
+        float* a = fields + (1 + 3 * (size_t)cur) * fstride;      // current copy
+        float* b = fields + (1 + 3 * (size_t)(1 - cur)) * fstride;  // copy to write
+        g.CP = a;
+        g.U = a + fstride;
+        g.V = a + 2 * fstride;
+        g.CPn = b;
+        g.Un = b + fstride;
+        g.Vn = b + 2 * fstride;
+        g.HM = fields + 7 * fstride;
+        g.VM = fields + 8 * fstride;
         return g;
     }
     unsigned char* fbuf(int i) { return flags + (size_t)(((i % 3) + 3) % 3) * fbytes; }
@@ -259,6 +266,7 @@ int chained_x(cliffs_handle* h, const float* edges, bool one_d) {
     const hipError_t e = launch_sweep_x(a, h->st);
     if (e != hipSuccess) return hipfail(e, "sweep_x");
     h->m++;
+    h->cur ^= 1;
     return CLIFFS_OK;
 }
 
@@ -283,6 +291,7 @@ int chained_y(cliffs_handle* h, const float* edges, bool one_d) {
     const hipError_t e = launch_sweep_y(a, h->st);
     if (e != hipSuccess) return hipfail(e, "sweep_y");
     h->m++;
+    h->cur ^= 1;
     return CLIFFS_OK;
 }
 
@@ -403,7 +412,7 @@ int cliffs_create(const cliffs_grid_desc* d, cliffs_handle** out) {
 
     // ---- device allocations
     h->fstride = (size_t)h->pitch * ny + 64;
-    if (hipMalloc(&h->fields, 6 * h->fstride * sizeof(float)) != hipSuccess)
+    if (hipMalloc(&h->fields, 9 * h->fstride * sizeof(float)) != hipSuccess)
         return bail(fail(CLIFFS_ERR_NOMEM, "device state allocation"));
     if (hipMalloc(&h->xm, sizeof(float4) * nx) != hipSuccess ||
         hipMalloc(&h->xr, sizeof(float2) * ny) != hipSuccess ||
@@ -426,7 +435,7 @@ int cliffs_create(const cliffs_grid_desc* d, cliffs_handle** out) {
         hipMemcpy(h->ym, ym.data(), sizeof(float4) * ny, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->zeta, zt.data(), sizeof(float) * ny, hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(CLIFFS_ERR_HIP, "metric upload"));
-    if (hipMemset(h->fields, 0, 6 * h->fstride * sizeof(float)) != hipSuccess)
+    if (hipMemset(h->fields, 0, 9 * h->fstride * sizeof(float)) != hipSuccess)
         return bail(fail(CLIFFS_ERR_HIP, "state clear"));
 
     // background celerity
@@ -572,6 +581,9 @@ int cliffs_sweep(cliffs_handle* h, int iu, int first, int last, const double* ed
         if (e != hipSuccess) return hipfail(e, "sweep_y");
     }
     h->m++;
+    // only lines first..last were written: copy them back into the current copy
+    const hipError_t e = launch_copy_lines(h->grid(), iu == 1 ? 0 : 1, first - 1, last - 1, h->st);
+    if (e != hipSuccess) return hipfail(e, "copy_lines");
     h->flags_valid = false;
     return CLIFFS_OK;
 }

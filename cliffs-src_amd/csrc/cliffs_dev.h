@@ -8,6 +8,7 @@
 // Fields: S (background celerity, read-only), CP (celerity perturbation),
 // U (x-velocity), V (y-velocity), HM (max CP), VM (max u^2+v^2).
 //
+// CP, U and V exist twice (double buffer, see DevGrid).
 // Line flags ("line has a wet node", cliffs.f:113) are byte arrays, one
 // byte per row or column, produced by the previous kernel in the chain:
 // the x-sweep produces column flags for the y-sweep, the y-sweep produces
@@ -24,12 +25,19 @@ constexpr int kXStrip = 248;    // output nodes per x-sweep wave (window 256)
 constexpr int kXRows = 8;       // rows per x-sweep wave
 constexpr int kYRows = 128;     // output rows per y-sweep block
 
+// The state CP, U, V is double-buffered: a sweep reads the current copy and
+// writes the other one (neighbouring work items read each other's halo
+// lines, so updating in place would race); lines a sweep leaves untouched
+// are copied.  CPn/Un/Vn point at the copy being written.
 struct DevGrid {
     int nx, ny, pitch;
     float* S;
     float* CP;
     float* U;
     float* V;
+    float* CPn;
+    float* Un;
+    float* Vn;
     float* HM;
     float* VM;
 };

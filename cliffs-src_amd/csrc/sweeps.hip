@@ -103,22 +103,19 @@ __global__ __launch_bounds__(256) void sweep_x_kernel(const XArgs a) {
 
     for (int j = r0; j <= r1; ++j) {
         const size_t ro = (size_t)j * a.g.pitch + kOff;
-        float* CPr = a.g.CP + ro;
-        float* Ur = a.g.U + ro;
-        float* Vr = a.g.V + ro;
+        const float* CPr = a.g.CP + ro;
+        const float* Ur = a.g.U + ro;
+        const float* Vr = a.g.V + ro;
         const float* Sr = a.g.S + ro;
         const bool active = j >= a.act_lo && j <= a.act_hi;
         const bool sweep = active && a.fin[j];
 
         Q4 s = zero4(), cp = zero4(), u = zero4(), v = zero4();
-        const bool need_uv = sweep || a.fuse_max;
         if (ld_lane) {
             s = ld4(Sr + i0);
             cp = ld4(CPr + i0);
-            if (need_uv) {
-                u = ld4(Ur + i0);
-                v = ld4(Vr + i0);
-            }
+            u = ld4(Ur + i0);
+            v = ld4(Vr + i0);
         }
         Q4 c, ge;
         int land = 0, deep = 0;
@@ -270,11 +267,13 @@ __global__ __launch_bounds__(256) void sweep_x_kernel(const XArgs a) {
                     }
                 }
             }
-            if (out_lane && ld_lane) {
-                st4(CPr + i0, ocp);
-                st4(Ur + i0, ou);
-                st4(Vr + i0, ov);
-            }
+        }
+        // every visited row is written to the other buffer: swept rows with
+        // their new values, the others (boundary rows, dry rows) unchanged
+        if (out_lane && ld_lane) {
+            st4(a.g.CPn + ro + i0, ocp);
+            st4(a.g.Un + ro + i0, ou);
+            st4(a.g.Vn + ro + i0, ov);
         }
 
         // wet bits of the new state for the flags consumed by the next sweep
@@ -469,15 +468,17 @@ __global__ __launch_bounds__(256) void sweep_y_kernel(const YArgs a) {
             if (swept) {
                 fx = ov0;
                 fy = ou0;
-                a.g.CP[o] = fcp;
-                a.g.U[o] = fx;
-                a.g.V[o] = fy;
-            } else if (col_ok && a.fuse_max) {
+            } else if (col_ok) {  // boundary or dry column: carried over unchanged
                 fcp = a.g.CP[o];
                 fx = a.g.U[o];
                 fy = a.g.V[o];
             } else {
                 fx = fy = 0.0f;
+            }
+            if (col_ok) {
+                a.g.CPn[o] = fcp;
+                a.g.Un[o] = fx;
+                a.g.Vn[o] = fy;
             }
             bool wet = false;
             if (col_ok) {
@@ -542,6 +543,28 @@ __global__ void line_flags_kernel(DevGrid g, Params p, unsigned char* fout, int 
     if (!is_land(c0_of(s) + g.CP[o], p)) fout[dir == 0 ? j : i] = 1;
 }
 
+// Copy lines first..last (rows if dir 0, columns if dir 1) of the state from
+// the written buffer (CPn, Un, Vn) back into the current one (CP, U, V):
+// used after a sweep over a subset of lines, which leaves the other lines
+// of the written buffer stale.
+__global__ void copy_lines_kernel(DevGrid g, int dir, int first, int last) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j;
+    if (dir == 0) {
+        i = t;
+        j = first + (int)blockIdx.y;
+        if (i >= g.nx) return;
+    } else {
+        i = first + t;
+        j = blockIdx.y;
+        if (i > last) return;
+    }
+    const size_t o = (size_t)j * g.pitch + kOff + i;
+    g.CP[o] = g.CPn[o];
+    g.U[o] = g.Un[o];
+    g.V[o] = g.Vn[o];
+}
+
 // max-wave update over all cells (cliffs_main.f:347-355)
 __global__ void maxwave_kernel(DevGrid g) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -574,6 +597,13 @@ hipError_t launch_line_flags(const DevGrid& g, const Params& p, unsigned char* f
                              unsigned* fclr, int nclr, hipStream_t st) {
     const dim3 grid((g.nx + 255) / 256, g.ny);
     hipLaunchKernelGGL(line_flags_kernel, grid, dim3(256), 0, st, g, p, fout, dir, fclr, nclr);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_lines(const DevGrid& g, int dir, int first, int last, hipStream_t st) {
+    const int n = last - first + 1;
+    const dim3 grid = dir == 0 ? dim3((g.nx + 255) / 256, n) : dim3((n + 255) / 256, g.ny);
+    hipLaunchKernelGGL(copy_lines_kernel, grid, dim3(256), 0, st, g, dir, first, last);
     return hipGetLastError();
 }
 

@@ -17,4 +17,7 @@ if [ "$1" = "test" ]; then
   $FC -O2 -c "$ROOT/tests/fortran/fdriver.f90" -o fdriver.o
   $FC -O2 -o "$ROOT/tests/fortran/fdriver" cliffs_gpu_mod.o fdriver_modules.o cliffs_dropin.o \
       fdriver.o -L"$PKG" -lcliffs_gpu -Wl,-rpath,"\$ORIGIN/../../cliffs-src_amd"
+  $FC -O2 -c "$ROOT/tests/fortran/fnest.f90" -o fnest.o
+  $FC -O2 -o "$ROOT/tests/fortran/fnest" cliffs_gpu_mod.o fdriver_modules.o cliffs_dropin.o \
+      fnest.o -L"$PKG" -lcliffs_gpu -Wl,-rpath,"\$ORIGIN/../../cliffs-src_amd"
 fi

@@ -30,6 +30,7 @@ module cliffs_gpu_mod
   public :: cliffs_download_max, cliffs_reset_max, cliffs_sweep, cliffs_maxwave
   public :: cliffs_step, cliffs_run, cliffs_run_timed, cliffs_gauges, cliffs_sync
   public :: cliffs_stream, cliffs_info, cliffs_last_error
+  public :: cliffs_nest_maps, cliffs_nest_create, cliffs_nest_destroy, cliffs_nest_feed
   public :: cliffs_gpu_open, cliffs_error_message
 
   interface
@@ -111,6 +112,37 @@ module cliffs_gpu_mod
     end function
     type(c_ptr) function cliffs_last_error() bind(C, name="cliffs_last_error")
       import :: c_ptr
+    end function
+    integer(c_int) function cliffs_nest_maps(nxp, xp, nyp, yp, nxc, xc, nyc, yc, we_ny, we_cy, &
+        sn_nx, sn_cx, w_nx, w_cx, e_nx, e_cx, s_ny, s_cy, n_ny, n_cy) bind(C, name="cliffs_nest_maps")
+      import :: c_int, c_double
+      integer(c_int), value :: nxp, nyp, nxc, nyc
+      real(c_double), intent(in) :: xp(*), yp(*), xc(*), yc(*)
+      integer(c_int), intent(out) :: we_ny(*), sn_nx(*), w_nx, e_nx, s_ny, n_ny
+      real(c_double), intent(out) :: we_cy(*), sn_cx(*), w_cx, e_cx, s_cy, n_cy
+    end function
+    integer(c_int) function cliffs_nest_create(parent, nxc, nyc, we_ny, we_cy, sn_nx, sn_cx, &
+        w_nx, w_cx, e_nx, e_cx, s_ny, s_cy, n_ny, n_cy, nest) bind(C, name="cliffs_nest_create")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: parent
+      integer(c_int), value :: nxc, nyc, w_nx, e_nx, s_ny, n_ny
+      integer(c_int), intent(in) :: we_ny(*), sn_nx(*)
+      real(c_double), intent(in) :: we_cy(*), sn_cx(*)
+      real(c_double), value :: w_cx, e_cx, s_cy, n_cy
+      type(c_ptr), intent(out) :: nest
+    end function
+    integer(c_int) function cliffs_nest_destroy(nest) bind(C, name="cliffs_nest_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: nest
+    end function
+    integer(c_int) function cliffs_nest_feed(nest, cuke, hmin, iarv, fed, uvq_w, uvq_e, uvq_s, &
+        uvq_n) bind(C, name="cliffs_nest_feed")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: nest
+      real(c_double), value :: cuke, hmin
+      integer(c_int), intent(inout) :: iarv
+      integer(c_int), intent(out) :: fed
+      real(c_double), intent(out) :: uvq_w(*), uvq_e(*), uvq_s(*), uvq_n(*)
     end function
   end interface
 

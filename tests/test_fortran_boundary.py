@@ -63,3 +63,48 @@ def test_fortran_driver_matches_oracle(tmp_path, gpu_lib, cartesian):
     assert np.abs(ref[1] - got[1])[both].max() < 1e-4
     hw = (ref[3] > cm) & (got[3] > cm)
     assert np.abs((ref[3] ** 2 - got[3] ** 2) / o.grav)[hw].max() <= 1e-5 * case.meta["amplitude"]
+
+
+FNEST = os.path.join(ROOT, "tests", "fortran", "fnest")
+
+
+def _read_feed(path):
+    raw = open(path, "rb").read()
+    off, recs = 0, []
+    while True:
+        ncid, nn, it = np.frombuffer(raw, dtype=np.int32, count=3, offset=off)
+        off += 12
+        if ncid == -1:
+            return recs, bool(nn), int(it)
+        t = np.frombuffer(raw, dtype=np.float64, count=1, offset=off)[0]
+        off += 8
+        vals = np.frombuffer(raw, dtype=np.float64, count=3 * nn, offset=off).reshape(3, nn).T
+        off += 24 * nn
+        recs.append((int(ncid), int(it), float(t), vals.copy()))
+
+
+@pytest.mark.gpu
+def test_fortran_feed_children_grid_matches_oracle(tmp_path, gpu_lib):
+    case = synth.coastal(nx=180, ny=140, seed=41)
+    s1, s2, zeta = case.metrics()
+    xc = np.linspace(case.xcrd[90], case.xcrd[170], 241)
+    yc = np.linspace(case.ycrd[20], case.ycrd[110], 271)
+    cuke = 0.01
+    fin, fout = tmp_path / "in.bin", tmp_path / "feed.bin"
+    with open(fin, "wb") as f:
+        np.array([case.shape[1], case.shape[0], len(xc), len(yc)], dtype=np.int32).tofile(f)
+        np.array([case.dt, case.ground, case.crough, cuke], dtype=np.float64).tofile(f)
+        for a in (case.dep, case.cel, case.xvel, case.yvel, s1, s2, zeta, case.xcrd, case.ycrd,
+                  xc, yc):
+            np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+    print("[fnest] coastal 180x140, child 241x271", flush=True)
+    subprocess.run([FNEST, str(fin), str(fout)], check=True, timeout=120)
+    recs, arrived, irec = _read_feed(fout)
+    o = O.Oracle(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground)
+    o.set_state(case.cel, case.xvel, case.yvel)
+    maps = O.nest_maps(case.xcrd, case.ycrd, xc, yc)
+    want = O.feed_child(o, maps, cuke, case.ground, False)
+    assert arrived and want[0] and irec == 1
+    assert [r[0] for r in recs] == [11, 12, 13, 14] and all(r[1] == 1 for r in recs)
+    for k in range(4):
+        assert np.abs(recs[k][3] - want[1 + k]).max() <= 1e-5 * case.meta["amplitude"]
