@@ -101,22 +101,25 @@ __global__ __launch_bounds__(256) void sweep_x_kernel(const XArgs a) {
     const int outmask = out_lane ? vmask : 0;
     int colwet = 0;
 
+    // the next row's loads are issued before the current row is computed
+    Q4 ns = zero4(), ncp = zero4(), nu = zero4(), nv = zero4();
+    auto load_row = [&](int j) {
+        if (ld_lane) {
+            const size_t ro = (size_t)j * a.g.pitch + kOff + i0;
+            ns = ld4(a.g.S + ro);
+            ncp = ld4(a.g.CP + ro);
+            nu = ld4(a.g.U + ro);
+            nv = ld4(a.g.V + ro);
+        }
+    };
+    load_row(r0);
     for (int j = r0; j <= r1; ++j) {
         const size_t ro = (size_t)j * a.g.pitch + kOff;
-        const float* CPr = a.g.CP + ro;
-        const float* Ur = a.g.U + ro;
-        const float* Vr = a.g.V + ro;
-        const float* Sr = a.g.S + ro;
         const bool active = j >= a.act_lo && j <= a.act_hi;
         const bool sweep = active && a.fin[j];
 
-        Q4 s = zero4(), cp = zero4(), u = zero4(), v = zero4();
-        if (ld_lane) {
-            s = ld4(Sr + i0);
-            cp = ld4(CPr + i0);
-            u = ld4(Ur + i0);
-            v = ld4(Vr + i0);
-        }
+        const Q4 s = ns, cp = ncp, u = nu, v = nv;
+        if (j < r1) load_row(j + 1);
         Q4 c, ge;
         int land = 0, deep = 0;
 #pragma unroll
@@ -316,28 +319,47 @@ __global__ __launch_bounds__(256) void sweep_x_kernel(const XArgs a) {
 // ---------------------------------------------------------------------------
 // y-sweep
 
+// Raw state of one row of the lane's column as loaded from HBM.
+struct RawIn {
+    float s, cp, u, v;
+};
+
 struct Raw {
     float s, cp, u, v, c, ge;
     bool land, valid, deep;
 };
 
-__device__ __forceinline__ Raw load_raw(const YArgs& a, int col, int row, bool col_ok) {
-    Raw r;
-    r.valid = col_ok && row >= 0 && row < a.g.ny;
-    r.s = r.cp = r.u = r.v = 0.0f;
-    if (r.valid) {
+__device__ __forceinline__ RawIn load_in(const YArgs& a, int col, int row, bool ok) {
+    RawIn r{0.f, 0.f, 0.f, 0.f};
+    if (ok) {
         const size_t o = (size_t)row * a.g.pitch + kOff + col;
         r.s = a.g.S[o];
         r.cp = a.g.CP[o];
         r.u = a.g.V[o];  // normal velocity of the y-sweep is yvel (cliffs.f:49-50)
         r.v = a.g.U[o];
     }
-    r.c = c0_of(r.s) + r.cp;
-    r.ge = geta_of(r.s, r.cp);
-    r.land = !r.valid || is_land(r.c, a.p);
-    r.deep = r.valid && is_deep(r.s, a.p);
     return r;
 }
+
+__device__ __forceinline__ Raw derive(const RawIn& in, bool valid, const Params& p) {
+    Raw r;
+    r.valid = valid;
+    r.s = in.s;
+    r.cp = in.cp;
+    r.u = in.u;
+    r.v = in.v;
+    r.c = c0_of(r.s) + r.cp;
+    r.ge = geta_of(r.s, r.cp);
+    r.land = !valid || is_land(r.c, p);
+    r.deep = valid && is_deep(r.s, p);
+    return r;
+}
+
+// Rows are processed in batches of kYBatch; the loads of the next batch (state
+// three rows ahead, max-wave fields of the rows being finalised) are issued
+// before the current batch is computed, so each lane keeps 8 rows of loads in
+// flight instead of stalling on one row at a time.
+constexpr int kYBatch = 8;
 
 __global__ __launch_bounds__(256) void sweep_y_kernel(const YArgs a) {
     if (blockIdx.x == 0 && blockIdx.y == 0) clear_words(a.fclr, a.nclr, threadIdx.x, blockDim.x);
@@ -360,16 +382,22 @@ __global__ __launch_bounds__(256) void sweep_y_kernel(const YArgs a) {
         vbn = a.edges[4 * nx + col];
         gbn = a.edges[5 * nx + col];
     }
+    auto row_ok = [&](int row) { return col_ok && row >= 0 && row < ny; };
 
     // pipeline registers (see header): rows relative to iteration r
-    Raw rB = load_raw(a, col, j0 - 2, col_ok);       // r+2
+    Raw rB = derive(load_in(a, col, j0 - 2, row_ok(j0 - 2)), row_ok(j0 - 2), p);  // r+2
     float uA, vA;                                     // r+1 (u, v)
     bool reB;                                         // RE_{r+2}
+    float pcp0, pu0, pv0, pcp1, pu1, pv1;             // loaded state of rows r, r+1
     {
-        const Raw rA = load_raw(a, col, j0 - 3, col_ok);
+        const Raw rA = derive(load_in(a, col, j0 - 3, row_ok(j0 - 3)), row_ok(j0 - 3), p);
         uA = rA.u;
         vA = rA.v;
         reB = rB.land && !rA.land && floods(rA.ge, rB.s, p) && rA.u >= 0.0f;
+        pcp0 = pu0 = pv0 = 0.f;
+        pcp1 = rA.cp;
+        pu1 = rA.u;
+        pv1 = rA.v;
     }
     Node pe1{0.f, 0.f, 0.f, 0.f}, pe0{0.f, 0.f, 0.f, 0.f};
     float pc1 = 0.f, pg1 = 0.f, pc0 = 0.f;
@@ -382,9 +410,10 @@ __global__ __launch_bounds__(256) void sweep_y_kernel(const YArgs a) {
     int okrun = 0;         // consecutive loaded rows that are wet and deeper than ground
     bool colwet = false;
 
-    for (int r = j0 - 4; r < j1; ++r) {
+    // one iteration: new row r+3 (rin), max-wave fields of row r (hm, vm)
+    auto iterate = [&](int r, const RawIn& rin, float hm, float vm) {
         // 1. new row r+3
-        const Raw rC = load_raw(a, col, r + 3, col_ok);
+        const Raw rC = derive(rin, row_ok(r + 3), p);
         okrun = (rC.valid && !rC.land && rC.deep) ? okrun + 1 : 0;
         const int row1 = r + 1;
         // fast path: every swept lane has wet, deep rows r-1..r+3 and row r+1
@@ -458,33 +487,26 @@ __global__ __launch_bounds__(256) void sweep_y_kernel(const YArgs a) {
         const float lu1 = end1 ? pe1.u : ou1;
 
         // 4. finalise row r: limiter, store, max-wave, flags
-        if (r >= j0) {
+        if (r >= j0 && r < j1) {
             if (swept && r >= 1 && r <= ny - 2) {
                 if (le0 && nlm1) limiter(s0, ocp0, ou0, lc1, lu1, true, p);
                 if (re0 && nl1) limiter(s0, ocp0, ou0, lcm1, lum1, false, p);
             }
             const size_t o = (size_t)r * a.g.pitch + kOff + col;
-            float fcp = ocp0, fx, fy;  // final cp, x-velocity, y-velocity
-            if (swept) {
-                fx = ov0;
-                fy = ou0;
-            } else if (col_ok) {  // boundary or dry column: carried over unchanged
-                fcp = a.g.CP[o];
-                fx = a.g.U[o];
-                fy = a.g.V[o];
-            } else {
-                fx = fy = 0.0f;
-            }
+            // final cp, x-velocity, y-velocity; boundary and dry columns are
+            // carried over unchanged
+            const float fcp = swept ? ocp0 : pcp0;
+            const float fx = swept ? ov0 : pv0;
+            const float fy = swept ? ou0 : pu0;
+            bool wet = false;
             if (col_ok) {
                 a.g.CPn[o] = fcp;
                 a.g.Un[o] = fx;
                 a.g.Vn[o] = fy;
-            }
-            bool wet = false;
-            if (col_ok) {
-                if (a.fuse_max) {  // fused max-wave (cliffs_main.f:347-355)
-                    a.g.HM[o] = fmaxf(a.g.HM[o], fcp);
-                    a.g.VM[o] = fmaxf(a.g.VM[o], fx * fx + fy * fy);
+                if (a.fuse_max) {  // fused max-wave (cliffs_main.f:347-355): store on change only
+                    const float sp2 = fx * fx + fy * fy;
+                    if (fcp > hm) a.g.HM[o] = fcp;
+                    if (sp2 > vm) a.g.VM[o] = sp2;
                 }
                 wet = !is_land(c0_of(s0) + fcp, p);
                 colwet |= wet;
@@ -518,8 +540,47 @@ __global__ __launch_bounds__(256) void sweep_y_kernel(const YArgs a) {
         E0 = E1;
         uA = rB.u;
         vA = rB.v;
+        pcp0 = pcp1;
+        pu0 = pu1;
+        pv0 = pv1;
+        pcp1 = rB.cp;
+        pu1 = rB.u;
+        pv1 = rB.v;
         rB = rC;
         reB = reC;
+    };
+
+    const int r_first = j0 - 4;
+    const int n_it = j1 - r_first;
+    auto hv_ok = [&](int r) { return a.fuse_max && col_ok && r >= j0 && r < j1; };
+    RawIn cur[kYBatch], nxt[kYBatch];
+    float hc[kYBatch], vc[kYBatch], hn[kYBatch], vn[kYBatch];
+#pragma unroll
+    for (int k = 0; k < kYBatch; k++) {
+        const int r = r_first + k;
+        cur[k] = load_in(a, col, r + 3, row_ok(r + 3));
+        const size_t o = (size_t)max(r, 0) * a.g.pitch + kOff + col;
+        hc[k] = hv_ok(r) ? a.g.HM[o] : 0.f;
+        vc[k] = hv_ok(r) ? a.g.VM[o] : 0.f;
+    }
+    for (int it = 0; it < n_it; it += kYBatch) {
+#pragma unroll
+        for (int k = 0; k < kYBatch; k++) {  // prefetch the next batch
+            const int r = r_first + it + kYBatch + k;
+            nxt[k] = load_in(a, col, r + 3, row_ok(r + 3));
+            const size_t o = (size_t)max(r, 0) * a.g.pitch + kOff + col;
+            hn[k] = hv_ok(r) ? a.g.HM[o] : 0.f;
+            vn[k] = hv_ok(r) ? a.g.VM[o] : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < kYBatch; k++)
+            if (it + k < n_it) iterate(r_first + it + k, cur[k], hc[k], vc[k]);
+#pragma unroll
+        for (int k = 0; k < kYBatch; k++) {
+            cur[k] = nxt[k];
+            hc[k] = hn[k];
+            vc[k] = vn[k];
+        }
     }
 
     if (a.flag_cols) {
