@@ -7,7 +7,10 @@ non-dispersive; default 10,000 timed steps.  One step = x-sweep over rows,
 y-sweep over columns, max-wave update (cliffs_main.f:317-355).  State is
 resident in HBM before the timed region; no host copies inside it.
 
-Prints ONE JSON line (see DESIGN.md for the roofline bookkeeping).
+Prints ONE JSON line.  roofline.achieved counts the algorithmic bytes of the
+dominant sweep kernel (28 B per cell for the x-sweep: read s, cp, u, v and
+write cp, u, v in FP32; 44 B for the y-sweep, which also reads and writes the
+two max-wave fields) over its mean HIP-event duration in the timed region.
 """
 import argparse
 import json
@@ -28,6 +31,21 @@ Y_BYTES = 44   # y-sweep: the same 28 B + fused max-wave 16 B (read+write hmax, 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def pmc_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` on the n x n workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/), already
+    scaled by the plain-copy calibration; None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_open_ocean_{n}.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            t = line.split()
+            if len(t) >= 2 and t[0] == f"{kernel}_bytes_per_launch":
+                return int(t[1])
+    return None
 
 
 def cpu_baseline(n, seed, steps):
@@ -61,9 +79,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--timing-steps", type=int, default=200,
-                    help="steps of the per-kernel HIP-event pass")
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (x5) to calibrate "
@@ -75,16 +91,23 @@ def main():
         n = args.copy_calib * (1 << 20) // 4
         a = torch.ones(n, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
+        b.copy_(a)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
         for _ in range(5):
             b.copy_(a)
+        e1.record()
         torch.cuda.synchronize()
-        print(json.dumps({"copy_calib_mib": args.copy_calib, "launches": 5,
-                          "bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n}))
+        ms = e0.elapsed_time(e1) / 5
+        print(json.dumps({"copy_calib_mib": args.copy_calib, "launches": 6,
+                          "bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n,
+                          "ms_per_launch": round(ms, 4),
+                          "copy_GBps": round(8 * n / (ms * 1e-3) / 1e9, 1)}))
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; the path shards by scenario or by "
-              "y-slab across GPUs (DESIGN.md), but no multi-GPU run is made here",
+              "y-slab across GPUs, but no multi-GPU run is made here",
               file=sys.stderr)
         sys.exit(2)
 
@@ -103,17 +126,15 @@ def main():
     g.run(args.warmup)
     g.sync()
 
-    log(f"[bench] timed {args.steps} steps")
+    log(f"[bench] timed {args.steps} steps (HIP events around every sweep)")
+    g.sync()
     t0 = time.perf_counter()
-    g.run(args.steps)
+    ms_dev, ms_x, ms_y = g.run_timed(args.steps)
     g.sync()
     wall = time.perf_counter() - t0
     cells = n * n
     value = cells * args.steps / wall / 1e9
-    log(f"[bench] {wall:.3f} s, {value:.2f} Gcell-updates/s")
-
-    log(f"[bench] per-kernel HIP-event pass, {args.timing_steps} steps")
-    ms_tot, ms_x, ms_y = g.run_timed(args.timing_steps)
+    log(f"[bench] {wall:.3f} s wall, {ms_dev:.1f} ms device, {value:.2f} Gcell-updates/s")
     c, u, v = g.download()
     finite = bool(np.isfinite(c).all() and np.isfinite(u).all())
     g.close()
@@ -125,7 +146,7 @@ def main():
     step_bw56 = 56 * value
     roof = {
         "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+        "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(dom, n),
         "kernel": dom, "bytes_per_cell": byt, "ms_per_launch": round(ms, 4),
         "other_kernel": {"sweep_x" if dom == "sweep_y" else "sweep_y": {
             "ms_per_launch": round(ms_x if dom == "sweep_y" else ms_y, 4),

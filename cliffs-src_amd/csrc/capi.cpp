@@ -168,7 +168,7 @@ int get_field(cliffs_handle* h, const float* src, std::vector<float>& dst) {
     return CLIFFS_OK;
 }
 inline size_t hidx(const cliffs_handle* h, int i, int j) {
-    return (size_t)j * h->pitch + kOff + i;
+    return (size_t)j * h->pitch + i;
 }
 
 // Edge arrays (mx,3) FP64 -> device [6][n] (ub, vb, g*eta for edge1, then
@@ -340,7 +340,11 @@ int cliffs_create(const cliffs_grid_desc* d, cliffs_handle** out) {
     h->dev = d->device;
     h->nx = nx;
     h->ny = ny;
-    h->pitch = ((nx + 2 * kOff) + 63) / 64 * 64;
+    // rows are whole 256-byte segments (the x-sweep's float4 accesses stay
+    // inside the row); strides that are multiples of 2 KiB get one more
+    // segment so consecutive rows do not fall on the same HBM channels
+    h->pitch = (nx + 63) / 64 * 64;
+    if (h->pitch % 512 == 0) h->pitch += 64;
     h->k = make_consts(d->ground);
     h->p.dt = (float)d->dt;
     h->p.grav = 9.8f;
@@ -620,6 +624,21 @@ int cliffs_run(cliffs_handle* h, int nsteps) {
     return CLIFFS_OK;
 }
 
+// Events bracket every sweep.  They live in a ring of kTimedSlots steps: before
+// a slot is reused its (long finished) events are read and accumulated, so the
+// host never waits on recent work and the event count stays bounded.
+constexpr int kTimedSlots = 256;
+
+namespace {
+struct EventSet {
+    std::vector<hipEvent_t> ev;
+    ~EventSet() {
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+}  // namespace
+
 int cliffs_run_timed(cliffs_handle* h, int nsteps, double* ms_total, double* ms_x, double* ms_y) {
     if (!h || !ms_total || !ms_x || !ms_y) return fail(CLIFFS_ERR_ARG, "NULL argument");
     if (!h->state) return fail(CLIFFS_ERR_STATE, "no state uploaded");
@@ -627,45 +646,55 @@ int cliffs_run_timed(cliffs_handle* h, int nsteps, double* ms_total, double* ms_
     HIPCHK(hipSetDevice(h->dev));
     int rc;
     if ((rc = ensure_flags(h, (h->xrun || !h->yrun) ? 0 : 1))) return rc;
-    std::vector<hipEvent_t> ev(4 * (size_t)nsteps + 2);
-    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-    size_t q = 0;
-    HIPCHK(hipEventRecord(ev[q++], h->st));
-    for (int s = 0; s < nsteps; s++) {
+    const int slots = std::min(nsteps, kTimedSlots);
+    EventSet es;
+    es.ev.assign(4 * (size_t)slots + 2, nullptr);
+    for (auto& e : es.ev) HIPCHK(hipEventCreate(&e));
+    hipEvent_t* ev = es.ev.data();
+    double sx = 0, sy = 0;
+    float t;
+    auto harvest = [&](int q) -> int {  // accumulate the sweep times of slot q
+        hipEvent_t* e = ev + 2 + 4 * (size_t)q;
         if (h->xrun) {
-            HIPCHK(hipEventRecord(ev[2 + 4 * (size_t)s], h->st));
-            if ((rc = chained_x(h, nullptr, !h->yrun))) return rc;
-            HIPCHK(hipEventRecord(ev[3 + 4 * (size_t)s], h->st));
+            HIPCHK(hipEventSynchronize(e[1]));
+            HIPCHK(hipEventElapsedTime(&t, e[0], e[1]));
+            sx += t;
         }
         if (h->yrun) {
-            HIPCHK(hipEventRecord(ev[4 + 4 * (size_t)s], h->st));
+            HIPCHK(hipEventSynchronize(e[3]));
+            HIPCHK(hipEventElapsedTime(&t, e[2], e[3]));
+            sy += t;
+        }
+        return CLIFFS_OK;
+    };
+    HIPCHK(hipEventRecord(ev[0], h->st));
+    for (int s = 0; s < nsteps; s++) {
+        const int q = s % slots;
+        if (s >= slots && (rc = harvest(q))) return rc;
+        hipEvent_t* e = ev + 2 + 4 * (size_t)q;
+        if (h->xrun) {
+            HIPCHK(hipEventRecord(e[0], h->st));
+            if ((rc = chained_x(h, nullptr, !h->yrun))) return rc;
+            HIPCHK(hipEventRecord(e[1], h->st));
+        }
+        if (h->yrun) {
+            HIPCHK(hipEventRecord(e[2], h->st));
             if ((rc = chained_y(h, nullptr, !h->xrun))) return rc;
-            HIPCHK(hipEventRecord(ev[5 + 4 * (size_t)s], h->st));
+            HIPCHK(hipEventRecord(e[3], h->st));
         }
         if (!h->xrun && !h->yrun) {
-            const hipError_t e = launch_maxwave(h->grid(), h->st);
-            if (e != hipSuccess) return hipfail(e, "maxwave");
+            const hipError_t err = launch_maxwave(h->grid(), h->st);
+            if (err != hipSuccess) return hipfail(err, "maxwave");
         }
     }
     HIPCHK(hipEventRecord(ev[1], h->st));
     HIPCHK(hipEventSynchronize(ev[1]));
-    float t;
     HIPCHK(hipEventElapsedTime(&t, ev[0], ev[1]));
     *ms_total = t;
-    double sx = 0, sy = 0;
-    for (int s = 0; s < nsteps; s++) {
-        if (h->xrun) {
-            HIPCHK(hipEventElapsedTime(&t, ev[2 + 4 * (size_t)s], ev[3 + 4 * (size_t)s]));
-            sx += t;
-        }
-        if (h->yrun) {
-            HIPCHK(hipEventElapsedTime(&t, ev[4 + 4 * (size_t)s], ev[5 + 4 * (size_t)s]));
-            sy += t;
-        }
-    }
+    for (int s = std::max(nsteps - slots, 0); s < nsteps; s++)
+        if ((rc = harvest(s % slots))) return rc;
     *ms_x = h->xrun ? sx / nsteps : 0.0;
     *ms_y = h->yrun ? sy / nsteps : 0.0;
-    for (auto& e : ev) (void)hipEventDestroy(e);
     return CLIFFS_OK;
 }
 

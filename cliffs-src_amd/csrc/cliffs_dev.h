@@ -1,10 +1,10 @@
 // Device-side data layout and kernel argument blocks.
 //
 // HBM layout (one allocation per field, FP32):
-//   row j of field F starts at F + j*pitch; node i of the row sits at
-//   column off + i, with off = 4 (16-byte aligned, room for the x-sweep's
-//   left halo) and pitch = roundup(nx + 8, 64) floats (256-byte rows).
-//   x is contiguous, exactly the reference's Fortran (nx, ny) order.
+//   node (i, j) of field F is F[j*pitch + i]; pitch is a multiple of 64
+//   floats, so every row starts on a 256-byte boundary and a wave's 64
+//   consecutive columns are whole 128-byte lines.  x is contiguous, exactly
+//   the reference's Fortran (nx, ny) order.
 // Fields: S (background celerity, read-only), CP (celerity perturbation),
 // U (x-velocity), V (y-velocity), HM (max CP), VM (max u^2+v^2).
 //
@@ -20,7 +20,6 @@
 
 namespace cliffs {
 
-constexpr int kOff = 4;         // column offset of node 0 within a row
 constexpr int kXStrip = 248;    // output nodes per x-sweep wave (window 256)
 constexpr int kXRows = 8;       // rows per x-sweep wave
 constexpr int kYRows = 128;     // output rows per y-sweep block

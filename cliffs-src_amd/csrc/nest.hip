@@ -20,7 +20,7 @@ __global__ void nest_arrival_kernel(const NestArgs a) {
     const int i = a.ii1 + blockIdx.x * blockDim.x + threadIdx.x;  // 1-based column
     const int k = a.kk1 + blockIdx.y;                              // 1-based row
     if (i > a.ii2 || k > a.kk2) return;
-    const size_t o = (size_t)(k - 1) * a.g.pitch + kOff + (i - 1);
+    const size_t o = (size_t)(k - 1) * a.g.pitch + (i - 1);
     const float s = a.g.S[o];
     if (!is_deep(s, a.p)) return;
     const float cp = a.g.CP[o];
@@ -33,7 +33,7 @@ __global__ void nest_arrival_kernel(const NestArgs a) {
 
 // value triple (u, v, eta) of parent cell (i,k), 1-based, times weight w
 __device__ __forceinline__ void feed_val(const NestArgs& a, int i, int k, double w, double out[3]) {
-    const size_t o = (size_t)(k - 1) * a.g.pitch + kOff + (i - 1);
+    const size_t o = (size_t)(k - 1) * a.g.pitch + (i - 1);
     const float s = a.g.S[o];
     if (is_deep(s, a.p)) {
         out[0] = w * (double)a.g.U[o];

@@ -87,7 +87,7 @@ __global__ __launch_bounds__(256) void sweep_x_kernel(const XArgs a) {
     const int nx = a.g.nx, ny = a.g.ny;
     const int i0 = strip * kXStrip - 4 + 4 * lane;  // node held in slot 0
     const bool out_lane = lane >= 1 && lane <= 62;
-    const bool ld_lane = i0 < nx;
+    const bool ld_lane = i0 >= 0 && i0 < nx;  // lane 0 of strip 0 holds nodes -4..-1: nothing to load
 
     int vmask = 0;
     float4 mt[4];
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void sweep_x_kernel(const XArgs a) {
     Q4 ns = zero4(), ncp = zero4(), nu = zero4(), nv = zero4();
     auto load_row = [&](int j) {
         if (ld_lane) {
-            const size_t ro = (size_t)j * a.g.pitch + kOff + i0;
+            const size_t ro = (size_t)j * a.g.pitch + i0;
             ns = ld4(a.g.S + ro);
             ncp = ld4(a.g.CP + ro);
             nu = ld4(a.g.U + ro);
@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256) void sweep_x_kernel(const XArgs a) {
     };
     load_row(r0);
     for (int j = r0; j <= r1; ++j) {
-        const size_t ro = (size_t)j * a.g.pitch + kOff;
+        const size_t ro = (size_t)j * a.g.pitch;
         const bool active = j >= a.act_lo && j <= a.act_hi;
         const bool sweep = active && a.fin[j];
 
@@ -332,7 +332,7 @@ struct Raw {
 __device__ __forceinline__ RawIn load_in(const YArgs& a, int col, int row, bool ok) {
     RawIn r{0.f, 0.f, 0.f, 0.f};
     if (ok) {
-        const size_t o = (size_t)row * a.g.pitch + kOff + col;
+        const size_t o = (size_t)row * a.g.pitch + col;
         r.s = a.g.S[o];
         r.cp = a.g.CP[o];
         r.u = a.g.V[o];  // normal velocity of the y-sweep is yvel (cliffs.f:49-50)
@@ -492,7 +492,7 @@ __global__ __launch_bounds__(256) void sweep_y_kernel(const YArgs a) {
                 if (le0 && nlm1) limiter(s0, ocp0, ou0, lc1, lu1, true, p);
                 if (re0 && nl1) limiter(s0, ocp0, ou0, lcm1, lum1, false, p);
             }
-            const size_t o = (size_t)r * a.g.pitch + kOff + col;
+            const size_t o = (size_t)r * a.g.pitch + col;
             // final cp, x-velocity, y-velocity; boundary and dry columns are
             // carried over unchanged
             const float fcp = swept ? ocp0 : pcp0;
@@ -558,8 +558,8 @@ __global__ __launch_bounds__(256) void sweep_y_kernel(const YArgs a) {
 #pragma unroll
     for (int k = 0; k < kYBatch; k++) {
         const int r = r_first + k;
-        cur[k] = load_in(a, col, r + 3, row_ok(r + 3));
-        const size_t o = (size_t)max(r, 0) * a.g.pitch + kOff + col;
+        cur[k] = load_in(a, col, r + 3, k < n_it && row_ok(r + 3));
+        const size_t o = (size_t)max(r, 0) * a.g.pitch + col;
         hc[k] = hv_ok(r) ? a.g.HM[o] : 0.f;
         vc[k] = hv_ok(r) ? a.g.VM[o] : 0.f;
     }
@@ -567,8 +567,8 @@ __global__ __launch_bounds__(256) void sweep_y_kernel(const YArgs a) {
 #pragma unroll
         for (int k = 0; k < kYBatch; k++) {  // prefetch the next batch
             const int r = r_first + it + kYBatch + k;
-            nxt[k] = load_in(a, col, r + 3, row_ok(r + 3));
-            const size_t o = (size_t)max(r, 0) * a.g.pitch + kOff + col;
+            nxt[k] = load_in(a, col, r + 3, it + kYBatch + k < n_it && row_ok(r + 3));
+            const size_t o = (size_t)max(r, 0) * a.g.pitch + col;
             hn[k] = hv_ok(r) ? a.g.HM[o] : 0.f;
             vn[k] = hv_ok(r) ? a.g.VM[o] : 0.f;
         }
@@ -599,7 +599,7 @@ __global__ void line_flags_kernel(DevGrid g, Params p, unsigned char* fout, int 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (i >= g.nx) return;
-    const size_t o = (size_t)j * g.pitch + kOff + i;
+    const size_t o = (size_t)j * g.pitch + i;
     const float s = g.S[o];
     if (!is_land(c0_of(s) + g.CP[o], p)) fout[dir == 0 ? j : i] = 1;
 }
@@ -620,7 +620,7 @@ __global__ void copy_lines_kernel(DevGrid g, int dir, int first, int last) {
         j = blockIdx.y;
         if (i > last) return;
     }
-    const size_t o = (size_t)j * g.pitch + kOff + i;
+    const size_t o = (size_t)j * g.pitch + i;
     g.CP[o] = g.CPn[o];
     g.U[o] = g.Un[o];
     g.V[o] = g.Vn[o];
@@ -631,7 +631,7 @@ __global__ void maxwave_kernel(DevGrid g) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (i >= g.nx) return;
-    const size_t o = (size_t)j * g.pitch + kOff + i;
+    const size_t o = (size_t)j * g.pitch + i;
     const float u = g.U[o], v = g.V[o];
     g.HM[o] = fmaxf(g.HM[o], g.CP[o]);
     g.VM[o] = fmaxf(g.VM[o], u * u + v * v);

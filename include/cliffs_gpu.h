@@ -100,8 +100,10 @@ int cliffs_step(cliffs_handle *h, const double *xedge1, const double *xedge2,
 int cliffs_run(cliffs_handle *h, int nsteps);
 
 /* Like cliffs_run, bracketing every sweep launch with HIP events on the
- * handle's stream.  Returns wall time of the whole enqueue (ms_total) and the
- * mean device duration of the x- and y-sweep kernels. */
+ * handle's stream.  Returns the device time from the first to the last
+ * launch (ms_total) and the mean device duration of the x- and y-sweep
+ * kernels.  Events are reused in a ring of 256 steps, so any nsteps works;
+ * the call returns when the work has finished. */
 int cliffs_run_timed(cliffs_handle *h, int nsteps, double *ms_total, double *ms_xsweep,
                      double *ms_ysweep);
 

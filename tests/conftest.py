@@ -55,7 +55,7 @@ def gpu_available() -> bool:
 
 @pytest.fixture(scope="session")
 def gpu_lib():
-    """The HIP library, loaded; fails loudly when it is missing on a GPU box."""
+    """The HIP library, loaded; fails loudly when it is missing on a machine with a GPU."""
     if not gpu_available():
         pytest.skip("no GPU")
     import cliffs_gpu
