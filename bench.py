@@ -8,9 +8,12 @@ y-sweep over columns, max-wave update (cliffs_main.f:317-355).  State is
 resident in HBM before the timed region; no host copies inside it.
 
 Prints ONE JSON line.  roofline.achieved counts the algorithmic bytes of the
-dominant sweep kernel (28 B per cell for the x-sweep: read s, cp, u, v and
-write cp, u, v in FP32; 44 B for the y-sweep, which also reads and writes the
-two max-wave fields) over its mean HIP-event duration in the timed region.
+dominant kernel over its mean HIP-event duration in the timed region.  The
+default step is one fused kernel (x-sweep, y-sweep and max-wave of a step in
+one pass): 44 B per cell-update, i.e. read s, cp, u, v, hmax, velmax and
+write cp, u, v, hmax, velmax in FP32.  With --unfused the step is two sweep
+kernels: 28 B per cell for the x-sweep (read s, cp, u, v; write cp, u, v) and
+44 B for the y-sweep, which also carries the max-wave update.
 """
 import argparse
 import json
@@ -27,6 +30,7 @@ METRIC = ("Gcell-updates/s & %HBM roofline; ensemble scenarios/hr at ≤2% "
 HBM_PEAK_GBS = 8000.0
 X_BYTES = 28   # x-sweep: read s, cp, u, v; write cp, u, v (FP32)
 Y_BYTES = 44   # y-sweep: the same 28 B + fused max-wave 16 B (read+write hmax, velmax)
+F_BYTES = 44   # fused step: 28 B of state + 16 B of max-wave per cell-update
 
 
 def log(*a):
@@ -81,6 +85,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--unfused", action="store_true",
+                    help="run the step as two sweep kernels instead of one fused kernel")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (x5) to calibrate "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE counters, then exit")
@@ -121,6 +127,7 @@ def main():
     s1, s2, zeta = case.metrics()
     g = cliffs_gpu.Grid(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground,
                         crough=case.crough, device=0)
+    g.set_fused(not args.unfused)
     g.upload(case.cel, case.xvel, case.yvel)
     log(f"[bench] warmup {args.warmup} steps")
     g.run(args.warmup)
@@ -139,21 +146,29 @@ def main():
     finite = bool(np.isfinite(c).all() and np.isfinite(u).all())
     g.close()
 
-    ach_x = X_BYTES * cells / (ms_x * 1e-3) / 1e9
-    ach_y = Y_BYTES * cells / (ms_y * 1e-3) / 1e9
-    dom = "sweep_y" if ms_y >= ms_x else "sweep_x"
-    ach, byt, ms = (ach_y, Y_BYTES, ms_y) if dom == "sweep_y" else (ach_x, X_BYTES, ms_x)
-    step_bw56 = 56 * value
+    if args.unfused:
+        ach_x = X_BYTES * cells / (ms_x * 1e-3) / 1e9
+        ach_y = Y_BYTES * cells / (ms_y * 1e-3) / 1e9
+        dom = "sweep_y" if ms_y >= ms_x else "sweep_x"
+        ach, byt, ms = (ach_y, Y_BYTES, ms_y) if dom == "sweep_y" else (ach_x, X_BYTES, ms_x)
+        other = {"sweep_x" if dom == "sweep_y" else "sweep_y": {
+            "ms_per_launch": round(ms_x if dom == "sweep_y" else ms_y, 4),
+            "achieved": round(ach_x if dom == "sweep_y" else ach_y, 1),
+            "bytes_per_cell": X_BYTES if dom == "sweep_y" else Y_BYTES}}
+        step_bytes = X_BYTES + Y_BYTES
+    else:
+        dom, byt, ms = "step_fused", F_BYTES, ms_x
+        ach = F_BYTES * cells / (ms * 1e-3) / 1e9
+        other = {"step_fixup": {"ms_per_launch": round(ms_y, 4)}}
+        step_bytes = F_BYTES
     roof = {
         "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(dom, n),
         "kernel": dom, "bytes_per_cell": byt, "ms_per_launch": round(ms, 4),
-        "other_kernel": {"sweep_x" if dom == "sweep_y" else "sweep_y": {
-            "ms_per_launch": round(ms_x if dom == "sweep_y" else ms_y, 4),
-            "achieved": round(ach_x if dom == "sweep_y" else ach_y, 1),
-            "bytes_per_cell": X_BYTES if dom == "sweep_y" else Y_BYTES}},
-        "step_56B_frac": round(step_bw56 / HBM_PEAK_GBS, 4),
-        "step_72B_frac": round(72 * value / HBM_PEAK_GBS, 4),
+        "other_kernel": other,
+        "step_bytes_per_cell": step_bytes,
+        "step_frac": round(step_bytes * value / HBM_PEAK_GBS, 4),
+        "step_28B_frac": round(28 * value / HBM_PEAK_GBS, 4),
     }
     cpu = None
     if not args.no_cpu:

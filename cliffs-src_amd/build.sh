@@ -7,6 +7,7 @@ ARCH=${CLIFFS_ARCH:-gfx950}
 FLAGS="-O3 -std=c++17 -fPIC --offload-arch=$ARCH -Xclang -target-feature -Xclang -packed-fp32-ops -Wall"
 cd "$HERE/csrc"
 $HIPCC $FLAGS -c sweeps.hip -o "$HERE/csrc/sweeps.o"
+$HIPCC $FLAGS -c step_fused.hip -o "$HERE/csrc/step_fused.o"
 $HIPCC $FLAGS -c nest.hip -o "$HERE/csrc/nest.o"
 $HIPCC $FLAGS -c capi.cpp -o "$HERE/csrc/capi.o"
-$HIPCC $FLAGS -shared -o "$HERE/libcliffs_gpu.so" "$HERE/csrc/sweeps.o" "$HERE/csrc/nest.o" "$HERE/csrc/capi.o"
+$HIPCC $FLAGS -shared -o "$HERE/libcliffs_gpu.so" "$HERE/csrc/sweeps.o" "$HERE/csrc/step_fused.o" "$HERE/csrc/nest.o" "$HERE/csrc/capi.o"

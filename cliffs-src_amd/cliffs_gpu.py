@@ -67,6 +67,7 @@ def load_library():
         "cliffs_step": ([H, _dp, _dp, _dp, _dp, C.c_int], C.c_int),
         "cliffs_run": ([H, C.c_int], C.c_int),
         "cliffs_run_timed": ([H, C.c_int, _dp, _dp, _dp], C.c_int),
+        "cliffs_set_fused": ([H, C.c_int], C.c_int),
         "cliffs_gauges": ([H, C.c_int, _ip, _ip, _dp, _dp, _dp], C.c_int),
         "cliffs_sync": ([H], C.c_int),
         "cliffs_stream": ([H, C.POINTER(C.c_void_p)], C.c_int),
@@ -205,9 +206,14 @@ class Grid:
     def run(self, nsteps: int):
         _chk(load_library().cliffs_run(self._h, int(nsteps)))
 
+    def set_fused(self, on: bool):
+        """2-D steps as one fused kernel (default) or as two sweep kernels."""
+        _chk(load_library().cliffs_set_fused(self._h, 1 if on else 0))
+
     def run_timed(self, nsteps: int):
-        """-> (ms_total, ms_xsweep_mean, ms_ysweep_mean), HIP events on the
-        handle's stream."""
+        """-> (ms_total, ms_a, ms_b), HIP events on the handle's stream: mean
+        durations of the x- and y-sweep kernels, or of the fused step kernel
+        and its flag-check kernel when the step runs fused."""
         t, x, y = C.c_double(), C.c_double(), C.c_double()
         _chk(load_library().cliffs_run_timed(self._h, int(nsteps), C.byref(t), C.byref(x),
                                              C.byref(y)))

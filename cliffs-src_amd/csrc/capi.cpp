@@ -23,6 +23,13 @@ hipError_t launch_maxwave(const DevGrid& g, hipStream_t st);
 hipError_t launch_copy_lines(const DevGrid& g, int dir, int first, int last, hipStream_t st);
 hipError_t launch_nest_arrival(const NestArgs& a, hipStream_t st);
 hipError_t launch_nest_feed(const NestArgs& a, hipStream_t st);
+hipError_t launch_step_fused(const FArgs& a, hipStream_t st);
+hipError_t launch_step_fixup(const FArgs& a, hipStream_t st);
+hipError_t launch_fused_flags(const FArgs& a, unsigned char* ctr0, unsigned char* ctr1,
+                              hipStream_t st);
+hipError_t fused_blocks_per_cu(int* n);
+void fused_geometry(int nx, int ny, int cus, int blocks_per_cu, int* nstrips, int* nchunks,
+                    int* rows);
 }  // namespace cliffs
 
 using namespace cliffs;
@@ -91,6 +98,14 @@ struct cliffs_handle {
     size_t fbytes = 0;
     int m = 0;                 // kernel sequence number
     bool flags_valid = false;
+    // fused 2-D step (step_fused.hip)
+    bool fused = true;         // 2-D steps run as one fused kernel (+ fixup)
+    bool fused_ok = false;     // the nine fields span < 4 GiB (32-bit offsets)
+    int f_nstrips = 0, f_nchunks = 0, f_rows = 0;
+    int f_cstride = 0;         // bytes per column of cpr: roundup(nchunks, 16)
+    unsigned char* fflags = nullptr;  // rfl[2][nstrips][ny], cpr[2][nx][cstride], ctr[2][nx]
+    int fn = 0;                // fused step parity
+    bool fused_valid = false;
     bool state = false;
     std::vector<float> hbuf;   // pitched host staging
 
@@ -113,6 +128,13 @@ struct cliffs_handle {
         return g;
     }
     unsigned char* fbuf(int i) { return flags + (size_t)(((i % 3) + 3) % 3) * fbytes; }
+    unsigned char* rfl(int b) { return fflags + (size_t)b * f_nstrips * ny; }
+    size_t rfl_bytes() const { return ((size_t)f_nstrips * ny + 255) / 256 * 256; }
+    size_t cpr_bytes() const { return (size_t)f_cstride * nx; }
+    unsigned char* cpr(int b) { return fflags + 2 * rfl_bytes() + (size_t)b * cpr_bytes(); }
+    unsigned char* ctr(int b) {
+        return fflags + 2 * rfl_bytes() + 2 * cpr_bytes() + (size_t)b * nx;
+    }
 };
 
 struct cliffs_nest {
@@ -146,6 +168,7 @@ void free_handle(cliffs_handle* h) {
     (void)hipFree(h->zeta);
     (void)hipFree(h->edges);
     (void)hipFree(h->flags);
+    (void)hipFree(h->fflags);
     if (h->hedges) (void)hipHostFree(h->hedges);
     if (h->st) (void)hipStreamDestroy(h->st);
     delete h;
@@ -295,18 +318,93 @@ int chained_y(cliffs_handle* h, const float* edges, bool one_d) {
     return CLIFFS_OK;
 }
 
-int do_step(cliffs_handle* h, const float* xe, const float* ye) {
+FArgs fargs(cliffs_handle* h, const float* xe, const float* ye) {
+    FArgs a{};
+    a.g = h->grid();
+    a.fs = (unsigned)h->fstride;
+    a.cur = h->cur;
+    a.xm = h->xm;
+    a.xr = h->xr;
+    a.xedges = xe;
+    a.ym = h->ym;
+    a.zeta = h->zeta;
+    a.yedges = ye;
+    a.p = h->p;
+    a.nstrips = h->f_nstrips;
+    a.nchunks = h->f_nchunks;
+    a.rows = h->f_rows;
+    a.ntiles = h->f_nstrips * h->f_nchunks;
+    a.cstride = h->f_cstride;
+    const int b = h->fn & 1;
+    a.rfl_in = h->rfl(b);
+    a.rfl_out = h->rfl(b ^ 1);
+    a.cpr_in = h->cpr(b);
+    a.cpr_out = h->cpr(b ^ 1);
+    a.ctr = h->ctr(b);
+    a.ctr_next = h->ctr(b ^ 1);
+    a.row_lo = h->jj1;
+    a.row_hi = h->jj2;
+    a.col_lo = h->ii1;
+    a.col_hi = h->ii2;
+    return a;
+}
+
+// Line flags of the current state for the next fused step.
+int ensure_fused(cliffs_handle* h) {
+    if (h->fused_valid) return CLIFFS_OK;
+    FArgs a = fargs(h, nullptr, nullptr);
+    a.rfl_out = h->rfl(h->fn & 1);
+    a.cpr_out = h->cpr(h->fn & 1);
+    const hipError_t e = launch_fused_flags(a, h->ctr(0), h->ctr(1), h->st);
+    if (e != hipSuccess) return hipfail(e, "fused_flags");
+    h->fused_valid = true;
+    return CLIFFS_OK;
+}
+
+// Optional events bracketing the kernels of a step: ev[0], ev[1] around the
+// x-sweep (or the fused step kernel), ev[2], ev[3] around the y-sweep (or
+// the fixup kernel).
+int record(hipEvent_t* ev, int k, hipStream_t st) {
+    if (ev) HIPCHK(hipEventRecord(ev[k], st));
+    return CLIFFS_OK;
+}
+
+int do_step(cliffs_handle* h, const float* xe, const float* ye, hipEvent_t* ev = nullptr) {
     int rc;
+    if (h->xrun && h->yrun && h->fused && h->fused_ok) {
+        if ((rc = ensure_fused(h))) return rc;
+        const FArgs a = fargs(h, xe, ye);
+        if ((rc = record(ev, 0, h->st))) return rc;
+        hipError_t e = launch_step_fused(a, h->st);
+        if (e != hipSuccess) return hipfail(e, "step_fused");
+        if ((rc = record(ev, 1, h->st)) || (rc = record(ev, 2, h->st))) return rc;
+        e = launch_step_fixup(a, h->st);
+        if (e != hipSuccess) return hipfail(e, "step_fixup");
+        if ((rc = record(ev, 3, h->st))) return rc;
+        h->cur ^= 1;
+        h->fn++;
+        h->flags_valid = false;
+        return CLIFFS_OK;
+    }
+    h->fused_valid = false;
     if (h->xrun && h->yrun) {
         if ((rc = ensure_flags(h, 0))) return rc;
-        if ((rc = chained_x(h, xe, false))) return rc;
-        if ((rc = chained_y(h, ye, false))) return rc;
+        if ((rc = record(ev, 0, h->st)) || (rc = chained_x(h, xe, false)) ||
+            (rc = record(ev, 1, h->st)))
+            return rc;
+        if ((rc = record(ev, 2, h->st)) || (rc = chained_y(h, ye, false)) ||
+            (rc = record(ev, 3, h->st)))
+            return rc;
     } else if (h->xrun) {
         if ((rc = ensure_flags(h, 0))) return rc;
-        if ((rc = chained_x(h, xe, true))) return rc;
+        if ((rc = record(ev, 0, h->st)) || (rc = chained_x(h, xe, true)) ||
+            (rc = record(ev, 1, h->st)))
+            return rc;
     } else if (h->yrun) {
         if ((rc = ensure_flags(h, 1))) return rc;
-        if ((rc = chained_y(h, ye, true))) return rc;
+        if ((rc = record(ev, 2, h->st)) || (rc = chained_y(h, ye, true)) ||
+            (rc = record(ev, 3, h->st)))
+            return rc;
     } else {
         const hipError_t e = launch_maxwave(h->grid(), h->st);
         if (e != hipSuccess) return hipfail(e, "maxwave");
@@ -416,6 +514,9 @@ int cliffs_create(const cliffs_grid_desc* d, cliffs_handle** out) {
 
     // ---- device allocations
     h->fstride = (size_t)h->pitch * ny + 64;
+    // the fused step addresses all fields by 32-bit byte offsets from one base;
+    // larger grids (> ~119M cells) step as two sweep kernels instead
+    h->fused_ok = 9 * h->fstride * sizeof(float) < ((size_t)1 << 32);
     if (hipMalloc(&h->fields, 9 * h->fstride * sizeof(float)) != hipSuccess)
         return bail(fail(CLIFFS_ERR_NOMEM, "device state allocation"));
     if (hipMalloc(&h->xm, sizeof(float4) * nx) != hipSuccess ||
@@ -433,6 +534,18 @@ int cliffs_create(const cliffs_grid_desc* d, cliffs_handle** out) {
     h->fbytes = ((size_t)std::max(nx, ny) + 8 + 255) / 256 * 256;
     if (hipMalloc(&h->flags, 3 * h->fbytes) != hipSuccess)
         return bail(fail(CLIFFS_ERR_NOMEM, "flag allocation"));
+    {  // fused step: tile geometry for this device, per-tile line flags
+        int cus = 0, bpc = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->dev) != hipSuccess ||
+            fused_blocks_per_cu(&bpc) != hipSuccess || cus < 1 || bpc < 1)
+            return bail(fail(CLIFFS_ERR_HIP, "device occupancy query"));
+        fused_geometry(nx, ny, cus, bpc, &h->f_nstrips, &h->f_nchunks, &h->f_rows);
+        h->f_cstride = (h->f_nchunks + 15) / 16 * 16;
+        const size_t fb = 2 * (h->rfl_bytes() + h->cpr_bytes() + (size_t)nx);
+        // cpr bytes beyond nchunks are read as zeros and never written
+        if (hipMalloc(&h->fflags, fb) != hipSuccess || hipMemset(h->fflags, 0, fb) != hipSuccess)
+            return bail(fail(CLIFFS_ERR_NOMEM, "flag allocation"));
+    }
 
     if (hipMemcpy(h->xm, xm.data(), sizeof(float4) * nx, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->xr, xr.data(), sizeof(float2) * ny, hipMemcpyHostToDevice) != hipSuccess ||
@@ -481,6 +594,7 @@ int cliffs_upload_state(cliffs_handle* h, const double* cel, const double* xvel,
     if ((rc = put_field(h, h->grid().V, b))) return rc;
     h->state = true;
     h->flags_valid = false;
+    h->fused_valid = false;
     return CLIFFS_OK;
 }
 
@@ -589,6 +703,7 @@ int cliffs_sweep(cliffs_handle* h, int iu, int first, int last, const double* ed
     const hipError_t e = launch_copy_lines(h->grid(), iu == 1 ? 0 : 1, first - 1, last - 1, h->st);
     if (e != hipSuccess) return hipfail(e, "copy_lines");
     h->flags_valid = false;
+    h->fused_valid = false;
     return CLIFFS_OK;
 }
 
@@ -645,7 +760,6 @@ int cliffs_run_timed(cliffs_handle* h, int nsteps, double* ms_total, double* ms_
     if (nsteps < 1) return fail(CLIFFS_ERR_ARG, "nsteps < 1");
     HIPCHK(hipSetDevice(h->dev));
     int rc;
-    if ((rc = ensure_flags(h, (h->xrun || !h->yrun) ? 0 : 1))) return rc;
     const int slots = std::min(nsteps, kTimedSlots);
     EventSet es;
     es.ev.assign(4 * (size_t)slots + 2, nullptr);
@@ -671,21 +785,7 @@ int cliffs_run_timed(cliffs_handle* h, int nsteps, double* ms_total, double* ms_
     for (int s = 0; s < nsteps; s++) {
         const int q = s % slots;
         if (s >= slots && (rc = harvest(q))) return rc;
-        hipEvent_t* e = ev + 2 + 4 * (size_t)q;
-        if (h->xrun) {
-            HIPCHK(hipEventRecord(e[0], h->st));
-            if ((rc = chained_x(h, nullptr, !h->yrun))) return rc;
-            HIPCHK(hipEventRecord(e[1], h->st));
-        }
-        if (h->yrun) {
-            HIPCHK(hipEventRecord(e[2], h->st));
-            if ((rc = chained_y(h, nullptr, !h->xrun))) return rc;
-            HIPCHK(hipEventRecord(e[3], h->st));
-        }
-        if (!h->xrun && !h->yrun) {
-            const hipError_t err = launch_maxwave(h->grid(), h->st);
-            if (err != hipSuccess) return hipfail(err, "maxwave");
-        }
+        if ((rc = do_step(h, nullptr, nullptr, ev + 2 + 4 * (size_t)q))) return rc;
     }
     HIPCHK(hipEventRecord(ev[1], h->st));
     HIPCHK(hipEventSynchronize(ev[1]));
@@ -695,6 +795,12 @@ int cliffs_run_timed(cliffs_handle* h, int nsteps, double* ms_total, double* ms_
         if ((rc = harvest(s % slots))) return rc;
     *ms_x = h->xrun ? sx / nsteps : 0.0;
     *ms_y = h->yrun ? sy / nsteps : 0.0;
+    return CLIFFS_OK;
+}
+
+int cliffs_set_fused(cliffs_handle* h, int on) {
+    if (!h) return fail(CLIFFS_ERR_ARG, "NULL handle");
+    h->fused = on != 0;
     return CLIFFS_OK;
 }
 

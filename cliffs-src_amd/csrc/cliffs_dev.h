@@ -75,6 +75,38 @@ struct YArgs {
     int fuse_max;         // fuse the max-wave update (2-D step)
 };
 
+// Fused 2-D step (step_fused.hip): tiles of kFStrip columns x `rows` rows.
+constexpr int kFStrip = 248;    // output columns per tile (x window of 256)
+constexpr int kFMaxRows = 1024; // rows per tile, at most
+constexpr int kXCDs = 8;        // MI355X: 8 accelerator complex dies, one L2 each
+
+// Line flags of the fused step are per-tile contributions (one byte per tile
+// and line, written 0 or 1 by the tile that owns it), so a tile recomputed by
+// the fixup kernel overwrites its own contribution; a line's flag is the OR
+// over the tiles that cross it.
+struct FArgs {
+    DevGrid g;                    // g.S is the base of all nine fields (one allocation)
+    unsigned fs;                  // floats from one field to the next (9 * fs * 4 < 2^32)
+    int cur;                      // current state copy (0 or 1)
+    const float4* xm;     // x-sweep metric (XArgs::xm), xr, edges [6][ny] or null
+    const float2* xr;
+    const float* xedges;
+    const float4* ym;     // y-sweep metric (YArgs::ym), zeta, edges [6][nx] or null
+    const float* zeta;
+    const float* yedges;
+    Params p;
+    int nstrips, nchunks, rows, ntiles;
+    int cstride;                  // bytes per column of cpr: roundup(nchunks, 16)
+    const unsigned char* rfl_in;  // [nstrips][ny]: row j wet within strip s, state entering the step
+    unsigned char* rfl_out;       // the same for the state leaving the step
+    const unsigned char* cpr_in;  // [nx][cstride]: column i wet within chunk c, entering the step
+    unsigned char* cpr_out;       // the same, leaving the step
+    unsigned char* ctr;           // [nx]: column i wet after the x-sweep (only 1s stored)
+    unsigned char* ctr_next;      // [nx]: the next step's ctr, cleared by the fixup kernel
+    int row_lo, row_hi;           // rows the x-sweep covers (0-based, inclusive)
+    int col_lo, col_hi;           // columns the y-sweep covers
+};
+
 // One-way nest feed (nest.hip)
 struct NestArgs {
     DevGrid g;

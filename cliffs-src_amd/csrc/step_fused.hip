@@ -1,0 +1,372 @@
+// Fused VTCS-2 time step for CDNA4 (gfx950): the x-sweep over rows, the
+// y-sweep over columns and the max-wave update of one step
+// (cliffs_main.f:317-355) in a single pass over HBM.
+//
+// The reference's y-sweep must see the complete output of the x-sweep, but
+// both sweeps are local (stencil radius 3 along the line, plus one flag per
+// line), so a tile of output cells needs only its own x-swept rows and
+// three halo rows on each side.  A tile is kFStrip = 248 columns by `rows`
+// rows; its 256-thread block
+//   X: x-sweeps four rows at a time, one per wave (x_row: 256-node window,
+//      4 nodes per lane), into a double-buffered LDS ring of four rows;
+//   Y: feeds those four rows to the column pipelines (YPipe: one lane per
+//      column) and stores the final state and the max-wave fields.
+// HBM sees each cell's state read once (plus 3.2 % x-halo and 6 halo rows
+// per tile) and written once: 28 B of state plus <= 16 B of max-wave per
+// cell-update, instead of 72 B for two separate sweep kernels.
+//
+// Line flags.  A row is x-swept iff it has a wet node (cliffs.f:113); the
+// flag comes from the previous step (rfl, per strip).  A column is y-swept
+// iff it has a wet node after the x-sweep of the whole grid, which a tile
+// cannot know.  The step kernel predicts it from the state entering the
+// step (cpr, per chunk: a column changes that status only when a whole
+// column floods or dries) and records the true flag (ctr).  The fixup
+// kernel then recomputes, from the unchanged input buffer, every tile
+// holding a column whose prediction was wrong, with the true flags; the
+// result is identical to the unfused sweeps.  For columns predicted dry the
+// max-wave update is deferred to the fixup kernel, so a wrong prediction
+// never leaves a wrong maximum behind.  (Sweeping an all-dry column only
+// resets its nodes to c = u = v = 0, which cannot raise a maximum, so the
+// opposite mistake needs no deferral.)
+#include <hip/hip_runtime.h>
+
+#include "sweep_core.h"
+
+namespace cliffs {
+
+// Column i was wet within some chunk (cpr row i: cstride bytes, one per chunk,
+// zero beyond nchunks), read with 16-byte loads that are all in flight at once.
+__device__ __forceinline__ bool column_any(const unsigned char* cpr, int cstride, int col) {
+    const uint4* q = reinterpret_cast<const uint4*>(cpr + (size_t)col * cstride);
+    unsigned any = 0u;
+    for (int k = 0; k < cstride / 16; k++) {
+        const uint4 v = q[k];
+        any |= v.x | v.y | v.z | v.w;
+    }
+    return any != 0u;
+}
+
+// Tiles are assigned to XCDs in contiguous ranges of (chunk, strip) order, so
+// neighbouring tiles -- which read each other's halo lines and share the
+// 128-byte lines at their borders -- run under the same L2.  Workgroups are
+// dispatched round-robin over the XCDs: block b runs on XCD b % kXCDs.
+__device__ __forceinline__ int tile_of_block(int b, int ntiles) {
+    const int x = b % kXCDs, k = b / kXCDs;
+    const int lo = (int)(((long)ntiles * x) / kXCDs);
+    const int hi = (int)(((long)ntiles * (x + 1)) / kXCDs);
+    return lo + k < hi ? lo + k : -1;
+}
+
+struct FShared {
+    float row[2][4][4][256];      // [buffer][row of the group][s, cp, u, v][window node]
+    float4 xmt[256];              // x metric of the tile's window nodes
+    unsigned rowany[kFMaxRows];   // row of the tile has a wet node after the step
+};
+
+// Field access by 32-bit element offset from the common base: global
+// accesses with a scalar base and a 32-bit vector offset, no 64-bit address
+// arithmetic per access.
+struct Fields {
+    char* base;
+    __device__ __forceinline__ float ld(unsigned o) const {
+        return *reinterpret_cast<const float*>(base + o * 4u);
+    }
+    __device__ __forceinline__ Q4 ld4(unsigned o) const {
+        const float4 t = *reinterpret_cast<const float4*>(base + o * 4u);
+        return Q4{{t.x, t.y, t.z, t.w}};
+    }
+    __device__ __forceinline__ void st(unsigned o, float v) const {
+        *reinterpret_cast<float*>(base + o * 4u) = v;
+    }
+};
+
+// One tile of the step.  kExact = false: predicted column flags (step
+// kernel); kExact = true: the true flags recorded by the step kernel (fixup).
+template <bool kExact>
+__device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh) {
+    const Params& p = a.p;
+    const int nx = a.g.nx, ny = a.g.ny;
+    const unsigned pitch = (unsigned)a.g.pitch;
+    const Fields F{reinterpret_cast<char*>(a.g.S)};
+    const unsigned fs = a.fs;
+    const unsigned oCP = (1u + 3u * a.cur) * fs, oU = oCP + fs, oV = oCP + 2u * fs;
+    const unsigned oCPn = (4u - 3u * a.cur) * fs, oUn = oCPn + fs, oVn = oCPn + 2u * fs;
+    const unsigned oHM = 7u * fs, oVM = 8u * fs;
+    const int strip = tile % a.nstrips, chunk = tile / a.nstrips;
+    const int cs = strip * kFStrip;
+    const int j0 = chunk * a.rows, j1 = min(j0 + a.rows, ny);
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+
+    // x side: lane holds window nodes cs-4+4*lane .. +3
+    const int i0 = cs - 4 + 4 * lane;
+    {
+        const int i = cs - 4 + t;
+        sh.xmt[t] = (i >= 0 && i < nx) ? a.xm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    const XLane XL{i0, xlane_vmask(nx, i0), &sh.xmt[4 * lane]};
+    // per-lane constants as integers (vector registers, not scalar lane masks)
+    const int ld_lane = (i0 >= 0 && i0 < nx) ? 1 : 0;
+    const int outmask = (lane >= 1 && lane <= 62) ? XL.vmask : 0;
+    int colwet_x = 0;
+
+    // y side: thread t owns column cs+t (window node t+4)
+    const int col = cs + t;
+    const int col_ok = (t < kFStrip && col < nx) ? 1 : 0;
+    const int in_range = (col_ok && col >= a.col_lo && col <= a.col_hi) ? 1 : 0;
+    int yflag = 0;
+    if (kExact) {
+        yflag = (in_range && a.ctr[col] != 0) ? 1 : 0;
+    } else if (in_range) {
+        yflag = column_any(a.cpr_in, a.cstride, col) ? 1 : 0;
+    }
+    const int keepmax = (col_ok && (kExact || !in_range || yflag)) ? 1 : 0;  // else deferred
+    YCol cc;
+    cc.swept = in_range && yflag;
+    cc.ubs = cc.vbs = cc.gbs = cc.ubn = cc.vbn = cc.gbn = 0.f;
+    if (a.yedges && col_ok) {
+        cc.ubs = a.yedges[col];
+        cc.vbs = a.yedges[nx + col];
+        cc.gbs = a.yedges[2 * nx + col];
+        cc.ubn = a.yedges[3 * nx + col];
+        cc.vbn = a.yedges[4 * nx + col];
+        cc.gbn = a.yedges[5 * nx + col];
+    }
+    YPipe pipe;
+    pipe.init();
+    int colwet_f = 0;
+    for (int k = t; k < j1 - j0; k += 256) sh.rowany[k] = 0u;
+    __syncthreads();  // xmt and rowany are complete
+
+    // rows j0-3 .. j1+2 stream through; group g holds stream rows 4g .. 4g+3
+    const int nstream = j1 - j0 + 6;
+    const int ngroups = (nstream + 3) / 4;
+
+    // the wave's next x row is loaded one group ahead
+    Q4 ns = zero4(), ncp = zero4(), nu = zero4(), nv = zero4();
+    int nflag = 0;
+    auto x_valid = [&](int g) {
+        const int q = j0 - 3 + 4 * g + w;
+        return 4 * g + w < nstream && q >= 0 && q < ny;
+    };
+    auto x_load = [&](int g) {
+        if (!x_valid(g)) return;
+        const int q = j0 - 3 + 4 * g + w;
+        if (ld_lane) {
+            const unsigned o = (unsigned)q * pitch + (unsigned)i0;
+            ns = F.ld4(o);
+            ncp = F.ld4(oCP + o);
+            nu = F.ld4(oU + o);
+            nv = F.ld4(oV + o);
+        }
+        int f = 0;  // row has a wet node in some strip (previous step)
+        for (int s0 = 0; s0 < a.nstrips; s0 += 64) {
+            const int s = s0 + lane;
+            f |= (s < a.nstrips && a.rfl_in[(size_t)s * ny + q] != 0) ? 1 : 0;
+        }
+        nflag = f;
+    };
+    // max-wave fields of the rows group g finalises (rows j0-6+4g .. +3),
+    // loaded after the group's barrier: prefetching them a group ahead costs
+    // more in registers (occupancy) than it saves in latency
+    float hc[4], vc[4];
+    auto hv_load = [&](int g, float* h, float* v) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int r = j0 - 6 + 4 * g + k;
+            const bool ok = keepmax && r >= j0 && r < j1;
+            const unsigned o = (unsigned)max(r, 0) * pitch + (unsigned)col;
+            h[k] = ok ? F.ld(oHM + o) : 0.f;
+            v[k] = ok ? F.ld(oVM + o) : 0.f;
+        }
+    };
+
+    x_load(0);
+    for (int g = 0; g < ngroups; g++) {
+        float(*buf)[4][256] = sh.row[g & 1];
+        // ---- X: wave w sweeps grid row q into the ring
+        {
+            const int q = j0 - 3 + 4 * g + w;
+            const bool valid = x_valid(g);
+            const Q4 s = ns, cp = ncp, u = nu, v = nv;
+            const bool fl = __ballot(nflag != 0) != 0;
+            if (g + 1 < ngroups) x_load(g + 1);
+            if (valid) {
+                const bool sweep = q >= a.row_lo && q <= a.row_hi && fl;
+                Q4 ocp, ou, ov;
+                x_row(XL, p, nx, ny, q, sweep, sweep ? a.xr[q] : make_float2(0.f, 0.f), a.xedges, s,
+                      cp, u, v, ocp, ou, ov);
+                st4(&buf[w][0][4 * lane], s);
+                st4(&buf[w][1][4 * lane], ocp);
+                st4(&buf[w][2][4 * lane], ou);
+                st4(&buf[w][3][4 * lane], ov);
+                if (!kExact && q >= j0 && q < j1) colwet_x |= wet_bits(s, ocp, outmask, p);
+            }
+        }
+        __syncthreads();
+        hv_load(g, hc, vc);
+        // ---- Y: four rows through the column pipelines
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int idx = 4 * g + k;
+            if (idx < nstream) {
+                const int r = j0 - 6 + idx;
+                const int q = r + 3;
+                const bool rv = col_ok && q >= 0 && q < ny;
+                RawIn rin{0.f, 0.f, 0.f, 0.f};
+                if (rv) {
+                    rin.s = buf[k][0][t + 4];
+                    rin.cp = buf[k][1][t + 4];
+                    rin.u = buf[k][3][t + 4];  // normal velocity of the y-sweep: yvel
+                    rin.v = buf[k][2][t + 4];
+                }
+                const YOut f = pipe.advance(r, rin, rv, cc, ny, a.ym, a.zeta, p);
+                if (r >= j0 && r < j1) {
+                    bool wet = false;
+                    if (col_ok) {
+                        const unsigned o = (unsigned)r * pitch + (unsigned)col;
+                        F.st(oCPn + o, f.cp);
+                        F.st(oUn + o, f.x);
+                        F.st(oVn + o, f.y);
+                        if (keepmax) {  // max-wave (cliffs_main.f:347-355): store on change only
+                            const float sp2 = f.x * f.x + f.y * f.y;
+                            if (f.cp > hc[k]) F.st(oHM + o, f.cp);
+                            if (sp2 > vc[k]) F.st(oVM + o, sp2);
+                        }
+                        wet = !is_land(c0_of(f.s) + f.cp, p);
+                        colwet_f |= wet ? 1 : 0;
+                    }
+                    if (__ballot(wet) != 0 && lane == 0) atomicOr(&sh.rowany[r - j0], 1u);
+                }
+            }
+        }
+        // no second barrier: the next X phase fills the other buffer, and the
+        // one after it waits at the next barrier for this Y phase to finish
+    }
+    __syncthreads();
+    for (int k = t; k < j1 - j0; k += 256) a.rfl_out[(size_t)strip * ny + j0 + k] = sh.rowany[k] ? 1 : 0;
+    if (col_ok) a.cpr_out[(size_t)col * a.cstride + chunk] = colwet_f ? 1 : 0;
+    if (!kExact && outmask && ld_lane && colwet_x) {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (bit(colwet_x, k)) a.ctr[i0 + k] = 1;
+    }
+}
+
+// 3 waves per SIMD (<= 168 VGPRs): 12 waves per CU hide the memory latency
+// that one barrier-synchronised block cannot.
+__global__ __launch_bounds__(256, 3) void step_fused_kernel(const FArgs a) {
+    __shared__ FShared sh;
+    const int tile = tile_of_block(blockIdx.x, a.ntiles);
+    if (tile < 0) return;
+    fused_tile<false>(a, tile, sh);
+}
+
+// Checks the step kernel's column predictions tile by tile: recomputes the
+// tiles with a wrong prediction, applies the deferred max-wave update of
+// columns that stayed dry, and clears the next step's ctr.
+__global__ __launch_bounds__(256) void step_fixup_kernel(const FArgs a) {
+    __shared__ FShared sh;
+    const int tile = tile_of_block(blockIdx.x, a.ntiles);
+    if (tile < 0) return;
+    const int nx = a.g.nx, ny = a.g.ny;
+    const int strip = tile % a.nstrips, chunk = tile / a.nstrips;
+    const int t = threadIdx.x;
+    const int col = strip * kFStrip + t;
+    const bool col_ok = t < kFStrip && col < nx;
+    const bool in_range = col_ok && col >= a.col_lo && col <= a.col_hi;
+    const bool pred = in_range && column_any(a.cpr_in, a.cstride, col);
+    const bool tru = in_range && a.ctr[col] != 0;
+    if (chunk == 0 && col_ok) a.ctr_next[col] = 0;
+    if (__syncthreads_or(in_range && pred != tru)) {
+        fused_tile<true>(a, tile, sh);
+        return;
+    }
+    if (in_range && !pred) {  // column left unswept: max-wave of its final state
+        const int j0 = chunk * a.rows, j1 = min(j0 + a.rows, ny);
+        for (int r = j0; r < j1; r++) {
+            const size_t o = (size_t)r * a.g.pitch + col;
+            const float u = a.g.Un[o], v = a.g.Vn[o];
+            a.g.HM[o] = fmaxf(a.g.HM[o], a.g.CPn[o]);
+            a.g.VM[o] = fmaxf(a.g.VM[o], u * u + v * v);
+        }
+    }
+}
+
+// Line flags of the current state (written to rfl_out, cpr_out) and cleared
+// ctr buffers, before the first fused step after any other operation.
+__global__ void fused_flags_kernel(const FArgs a, unsigned char* ctr0, unsigned char* ctr1) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = a.g.nx, ny = a.g.ny;
+    const long nr = (long)a.nstrips * ny;
+    const long nc = (long)a.nchunks * nx;
+    if (t < nx) {
+        ctr0[t] = 0;
+        ctr1[t] = 0;
+    }
+    if (t < nr) {
+        const int s = (int)(t / ny), j = (int)(t % ny);
+        const int i1 = min((s + 1) * kFStrip, nx);
+        bool any = false;
+        for (int i = s * kFStrip; i < i1 && !any; i++) {
+            const size_t o = (size_t)j * a.g.pitch + i;
+            any = !is_land(c0_of(a.g.S[o]) + a.g.CP[o], a.p);
+        }
+        a.rfl_out[t] = any ? 1 : 0;
+    } else if (t < nr + nc) {
+        const long u = t - nr;
+        const int c = (int)(u / nx), i = (int)(u % nx);  // consecutive threads: consecutive columns
+        const int j1 = min((c + 1) * a.rows, ny);
+        bool any = false;
+        for (int j = c * a.rows; j < j1 && !any; j++) {
+            const size_t o = (size_t)j * a.g.pitch + i;
+            any = !is_land(c0_of(a.g.S[o]) + a.g.CP[o], a.p);
+        }
+        a.cpr_out[(size_t)i * a.cstride + c] = any ? 1 : 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+static int fused_blocks(const FArgs& a) { return (a.ntiles + kXCDs - 1) / kXCDs * kXCDs; }
+
+hipError_t launch_step_fused(const FArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(step_fused_kernel, dim3(fused_blocks(a)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_fixup(const FArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(step_fixup_kernel, dim3(fused_blocks(a)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fused_flags(const FArgs& a, unsigned char* ctr0, unsigned char* ctr1,
+                              hipStream_t st) {
+    const long n = (long)a.nstrips * a.g.ny + (long)a.nchunks * a.g.nx;
+    const long m = n > a.g.nx ? n : a.g.nx;
+    hipLaunchKernelGGL(fused_flags_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, a,
+                       ctr0, ctr1);
+    return hipGetLastError();
+}
+
+hipError_t fused_blocks_per_cu(int* n) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, step_fused_kernel, 256, 0);
+}
+
+// Tile geometry: enough tiles to fill every CU with `blocks_per_cu` blocks
+// once, chunks of at most kFMaxRows rows and at least 32.
+void fused_geometry(int nx, int ny, int cus, int blocks_per_cu, int* nstrips, int* nchunks,
+                    int* rows) {
+    const int ns = (nx + kFStrip - 1) / kFStrip;
+    const int target = cus * blocks_per_cu;
+    int nc = target / ns;
+    if (nc < 1) nc = 1;
+    int r = (ny + nc - 1) / nc;
+    if (r < 32) r = 32;
+    if (r > kFMaxRows) r = kFMaxRows;
+    if (r > ny) r = ny;
+    *nstrips = ns;
+    *rows = r;
+    *nchunks = (ny + r - 1) / r;
+}
+
+}  // namespace cliffs
