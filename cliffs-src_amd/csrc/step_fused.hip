@@ -122,15 +122,7 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
     const int keepmax = (col_ok && (kExact || !in_range || yflag)) ? 1 : 0;  // else deferred
     YCol cc;
     cc.swept = in_range && yflag;
-    cc.ubs = cc.vbs = cc.gbs = cc.ubn = cc.vbn = cc.gbn = 0.f;
-    if (a.yedges && col_ok) {
-        cc.ubs = a.yedges[col];
-        cc.vbs = a.yedges[nx + col];
-        cc.gbs = a.yedges[2 * nx + col];
-        cc.ubn = a.yedges[3 * nx + col];
-        cc.vbn = a.yedges[4 * nx + col];
-        cc.gbn = a.yedges[5 * nx + col];
-    }
+    cc.load_edges(a.yedges, nx, col, col_ok);
     YPipe pipe;
     pipe.init();
     int colwet_f = 0;

@@ -29,17 +29,19 @@
 
 namespace cliffs {
 
+// Neighbour lane values by DPP wave shifts; bound_ctrl writes 0 into the lane
+// that has no source (lane 0 from the left, lane 63 from the right).
 __device__ __forceinline__ float from_left(float x) {  // lane t <- lane t-1
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x138, 0xF, 0xF, true));
 }
 __device__ __forceinline__ float from_right(float x) {  // lane t <- lane t+1
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x130, 0xF, 0xF, true));
 }
 __device__ __forceinline__ int from_left_i(int x) {
-    return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xF, 0xF, false);
+    return __builtin_amdgcn_mov_dpp(x, 0x138, 0xF, 0xF, true);
 }
 __device__ __forceinline__ int from_right_i(int x) {
-    return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xF, 0xF, false);
+    return __builtin_amdgcn_mov_dpp(x, 0x130, 0xF, 0xF, true);
 }
 
 // Four consecutive nodes held by one lane; neighbour shifts along the row.
@@ -165,7 +167,7 @@ __device__ __forceinline__ void x_row_general(const XLane& L, const Params& p, i
             m.rsum = L.mt[k].z * rr.x;
             m.fpm = L.mt[k].w * rr.y;
             m.scl = 0.0f;
-            interior_update(Ln, cL, X, pc.v[k], Rn, cR, em, ep, W, m, p, xc, xu, xv);
+            interior_update<false>(Ln, cL, X, pc.v[k], Rn, cR, em, ep, W, m, p, xc, xu, xv);
         } else if (!bit(nl, k) && i == 0) {
             if (!bit(Rnl, k) && floods(gbw, X.s, p))
                 edge_first(X, pc.v[k], Rr, EdgeRes{Ep.v[k], Eq.v[k]}, L.mt[k].y * rr.x, 0.0f, ubw,
@@ -235,7 +237,7 @@ __device__ __forceinline__ void x_row(const XLane& L, const Params& p, int nx, i
             m.rsum = L.mt[k].z * rr.x;
             m.fpm = L.mt[k].w * rr.y;
             m.scl = 0.0f;
-            interior_update(Node{Ls.v[k], Lcp.v[k], Lu.v[k], Lv.v[k]}, Lc.v[k],
+            interior_update<false>(Node{Ls.v[k], Lcp.v[k], Lu.v[k], Lv.v[k]}, Lc.v[k],
                             Node{s.v[k], cp.v[k], u.v[k], v.v[k]}, c.v[k],
                             Node{Rs.v[k], Rcp.v[k], Ru.v[k], Rv.v[k]}, Rc.v[k],
                             EdgeRes{LEp.v[k], LEq.v[k]}, EdgeRes{Ep.v[k], Eq.v[k]}, true, m, p,
@@ -272,8 +274,13 @@ struct YOut {
 
 // Per-column y-sweep constants.
 struct YCol {
-    bool swept;                               // column swept (range and wet flag)
-    float ubs, vbs, gbs, ubn, vbn, gbn;       // south / north edge values
+    bool swept;            // column swept (range and wet flag)
+    float e[6];            // south ub, vb, g*eta, north ub, vb, g*eta (0 without edges)
+    __device__ __forceinline__ float edge(int k) const { return e[k]; }
+    __device__ __forceinline__ void load_edges(const float* edges, int nx, int col, bool ok) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) e[k] = (edges && ok) ? edges[k * nx + col] : 0.f;
+    }
 };
 
 // Register pipeline of one column.  The rows are fed in order, one per
@@ -397,14 +404,16 @@ struct YPipe {
                 const EdgeRes ep = nlB ? ghost_right_res(pc1, pe1.u) : E1;
                 const bool W = is_deep(pe1.s, p) && is_deep(L.s, p) && is_deep(R.s, p);
                 interior_update(L, cL, pe1, pc1, R, cR, em, ep, W, m, p, ocp1, ou1, ov1);
-            } else if (row1 == 0) {
-                if (!nlB && floods(col.gbs, pe1.s, p))
-                    edge_first(pe1, pc1, peB, E1, ym[0].y, zeta[0], col.ubs, col.vbs, col.gbs, p, ocp1,
-                               ou1, ov1);
-            } else {
-                if (!nl0 && floods(col.gbn, pe1.s, p))
-                    edge_last(pe1, pc1, pe0, E0, ym[ny - 1].x, zeta[ny - 1], col.ubn, col.vbn, col.gbn,
-                              p, ocp1, ou1, ov1);
+            } else if (row1 == 0) {  // south edge
+                const float gbs = col.edge(2);
+                if (!nlB && floods(gbs, pe1.s, p))
+                    edge_first(pe1, pc1, peB, E1, ym[0].y, zeta[0], col.edge(0), col.edge(1), gbs, p,
+                               ocp1, ou1, ov1);
+            } else {  // north edge
+                const float gbn = col.edge(5);
+                if (!nl0 && floods(gbn, pe1.s, p))
+                    edge_last(pe1, pc1, pe0, E0, ym[ny - 1].x, zeta[ny - 1], col.edge(3), col.edge(4),
+                              gbn, p, ocp1, ou1, ov1);
             }
         }
         const float lc1 = c0_of(pe1.s) + (end1 ? pe1.cp : ocp1);

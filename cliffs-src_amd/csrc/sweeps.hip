@@ -135,15 +135,7 @@ __global__ __launch_bounds__(256) void sweep_y_kernel(const YArgs a) {
 
     YCol cc;
     cc.swept = col_ok && col >= a.col_lo && col <= a.col_hi && a.fin[col];
-    cc.ubs = cc.vbs = cc.gbs = cc.ubn = cc.vbn = cc.gbn = 0.f;
-    if (a.edges && col_ok) {
-        cc.ubs = a.edges[col];
-        cc.vbs = a.edges[nx + col];
-        cc.gbs = a.edges[2 * nx + col];
-        cc.ubn = a.edges[3 * nx + col];
-        cc.vbn = a.edges[4 * nx + col];
-        cc.gbn = a.edges[5 * nx + col];
-    }
+    cc.load_edges(a.edges, nx, col, col_ok);
     auto row_ok = [&](int row) { return col_ok && row >= 0 && row < ny; };
     YPipe pipe;
     pipe.init();

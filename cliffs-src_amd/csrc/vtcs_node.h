@@ -114,7 +114,9 @@ CLIFFS_HD EdgeRes ghost_right_res(float c, float u) {
 
 // Interior VTCS-2 update of node X between neighbours L and R (real nodes or
 // ghosts), cliffs.f:200-260.  em/ep are the residuals L->X and X->R.
-// W is (d > ground) for all three nodes.
+// W is (d > ground) for all three nodes.  kScl = false drops the metric term
+// (P-Q)(P+Q) zeta, which is zero in the x-sweep (scl = 0, cliffs.f:39).
+template <bool kScl = true>
 CLIFFS_HD void interior_update(const Node& L, float cL, const Node& X, float cX, const Node& R,
                                float cR, EdgeRes em, EdgeRes ep, bool W, const Metric& m,
                                const Params& p, float& cp_out, float& u_out, float& v_out) {
@@ -127,7 +129,7 @@ CLIFFS_HD void interior_update(const Node& L, float cL, const Node& X, float cX,
         const float rc = fast_rcp(c);
         F = p.gn2 * p.grav * u * sqrtf(p.grav * (u * u + v * v)) * (rc * rc * rc);
     }
-    const float S = 8.0f * c * u * m.scl;  // (P-Q)(P+Q) zeta
+    const float S = kScl ? 8.0f * c * u * m.scl : 0.0f;  // (P-Q)(P+Q) zeta
 
     // P family
     const float qpjP = ep.ep * m.rfp;
@@ -138,21 +140,23 @@ CLIFFS_HD void interior_update(const Node& L, float cL, const Node& X, float cX,
     const float qjmQ = em.eq * m.rfm;
     float qsQ = 0.5f * (qpjQ + qjmQ);
 
-    // transcritical branch (cliffs.f:228-229, 241-242); rare
+    // transcritical branch (cliffs.f:228-229, 241-242); rare.  On the device
+    // the test is one wave vote, so the common case costs no divergent branch.
     const bool tcP = W && (L.u + cL < 0.0f) && (R.u + cR > 0.0f);
     const bool tcQ = W && (L.u - cL < 0.0f) && (R.u - cR > 0.0f);
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (__ballot(tcP || tcQ) != 0) {
+#else
     if (tcP || tcQ) {
+#endif
         const float duRL = R.u - L.u;
         const float dcpRL = R.cp - L.cp;
         const float dsRL = R.s - L.s;
-        if (tcP) {
-            const float w = u + c;
-            qsP = (w * duRL + 2.0f * w * dcpRL + 2.0f * (u + X.cp) * dsRL) * m.rsum;
-        }
-        if (tcQ) {
-            const float w = u - c;
-            qsQ = (w * duRL - 2.0f * w * dcpRL - 2.0f * (u - X.cp) * dsRL) * m.rsum;
-        }
+        const float wP = u + c, wQ = u - c;
+        const float vP = (wP * duRL + 2.0f * wP * dcpRL + 2.0f * (u + X.cp) * dsRL) * m.rsum;
+        const float vQ = (wQ * duRL - 2.0f * wQ * dcpRL - 2.0f * (u - X.cp) * dsRL) * m.rsum;
+        qsP = tcP ? vP : qsP;
+        qsQ = tcQ ? vQ : qsQ;
     }
     qsP = fmaf((u + c) * dtx, qjmP - qpjP, qsP);
     qsQ = fmaf((u - c) * dtx, qjmQ - qpjQ, qsQ);
