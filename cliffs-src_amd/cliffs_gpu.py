@@ -295,3 +295,19 @@ class Nest:
         _chk(load_library().cliffs_nest_feed(self._h, float(cuke), float(hmin), C.byref(a),
                                              C.byref(f), _p(w), _p(e), _p(s), _p(n)))
         return bool(a.value), bool(f.value), w.T.copy(), e.T.copy(), s.T.copy(), n.T.copy()
+
+
+NAN32 = np.float32(-1.0e32)   # the reference's real*4 NAN (global_modules.f:18)
+
+
+def maxwave_fields(hmax, velmax, dep, grav, celmin):
+    """MaxE and MaxV as the reference writes them (output_maxwave.f:14-24):
+    MaxE = hmax**2/g - dep and MaxV = sqrt(velmax) where hmax > celmin, else
+    the real*4 NAN (-1e32); FP64 arithmetic, FP32 results."""
+    hmax = np.asarray(hmax, dtype=np.float64)
+    velmax = np.asarray(velmax, dtype=np.float64)
+    wet = hmax > celmin
+    e = np.where(wet, hmax * hmax / grav - np.asarray(dep, dtype=np.float64), 0.0)
+    v = np.where(wet, np.sqrt(np.maximum(velmax, 0.0)), 0.0)
+    return (np.where(wet, e.astype(np.float32), NAN32),
+            np.where(wet, v.astype(np.float32), NAN32))

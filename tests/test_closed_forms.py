@@ -236,3 +236,26 @@ def test_feed_arrival_gate():
     u[13, 11] = 1.1 * cuke / math.sqrt(G * 20.0)
     o.set_state(np.sqrt(G * dep), u, np.zeros_like(dep))
     assert O.feed_child(o, maps, cuke=cuke, hmin=0.01, iarv=False)[0]
+
+
+def test_maxwave_fields_recover_surface_maximum():
+    """output_maxwave.f:14-24: MaxE = hmax^2/g - d returns the maximum surface
+    elevation for a celerity maximum c = sqrt(g (d + eta)); cells whose
+    maximum never exceeded celmin are the real*4 NAN in both fields."""
+    import cliffs_gpu
+    rng = np.random.default_rng(4)
+    g = synth.GRAV
+    ground = 0.01
+    cm = np.sqrt(g * ground)
+    dep = rng.uniform(-5.0, 4000.0, (40, 50))
+    eta = rng.uniform(-0.5, 3.0, dep.shape)
+    H = dep + eta
+    hmax = np.where(H > ground, np.sqrt(g * np.maximum(H, 0.0)), 0.0)
+    u = rng.normal(0, 0.3, dep.shape)
+    v = rng.normal(0, 0.3, dep.shape)
+    E, V = cliffs_gpu.maxwave_fields(hmax, u * u + v * v, dep, g, cm)
+    wet = hmax > cm
+    assert E.dtype == np.float32 and V.dtype == np.float32
+    assert np.all(E[~wet] == np.float32(-1e32)) and np.all(V[~wet] == np.float32(-1e32))
+    assert np.allclose(E[wet], eta[wet], atol=1e-9 * 4000 + 4e-4)   # FP32 rounding of ~4000 m terms
+    assert np.allclose(V[wet], np.hypot(u, v)[wet].astype(np.float32), rtol=1e-6)
