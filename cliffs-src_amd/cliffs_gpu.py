@@ -69,6 +69,12 @@ def load_library():
         "cliffs_run_timed": ([H, C.c_int, _dp, _dp, _dp], C.c_int),
         "cliffs_set_fused": ([H, C.c_int], C.c_int),
         "cliffs_gauges": ([H, C.c_int, _ip, _ip, _dp, _dp, _dp], C.c_int),
+        "cliffs_ensemble_create": ([C.POINTER(H), C.c_int, C.POINTER(H)], C.c_int),
+        "cliffs_ensemble_destroy": ([H], C.c_int),
+        "cliffs_ensemble_run": ([H, C.c_int], C.c_int),
+        "cliffs_ensemble_run_timed": ([H, C.c_int, _dp], C.c_int),
+        "cliffs_ensemble_tune": ([H, C.c_int, _ip], C.c_int),
+        "cliffs_ensemble_sync": ([H], C.c_int),
         "cliffs_sync": ([H], C.c_int),
         "cliffs_stream": ([H, C.POINTER(C.c_void_p)], C.c_int),
         "cliffs_info": ([H, _dp, _dp, _ip, _ip, _ip, _ip], C.c_int),
@@ -235,6 +241,45 @@ class Grid:
         s = C.c_void_p()
         _chk(load_library().cliffs_stream(self._h, C.byref(s)))
         return s.value or 0
+
+
+class Ensemble:
+    """Grids stepped together: one fused launch per step for all members,
+    replayed from a HIP graph (cliffs_ensemble_*).  Members stay usable."""
+
+    def __init__(self, grids):
+        self.grids = list(grids)
+        arr = (C.c_void_p * len(self.grids))(*[g._h.value for g in self.grids])
+        h = C.c_void_p()
+        _chk(load_library().cliffs_ensemble_create(arr, len(self.grids), C.byref(h)))
+        self._h = h
+
+    def run(self, nsteps: int):
+        _chk(load_library().cliffs_ensemble_run(self._h, int(nsteps)))
+
+    def run_timed(self, nsteps: int) -> float:
+        t = C.c_double()
+        _chk(load_library().cliffs_ensemble_run_timed(self._h, int(nsteps), C.byref(t)))
+        return t.value
+
+    def tune(self, trial_steps: int = 3) -> int:
+        r = C.c_int()
+        _chk(load_library().cliffs_ensemble_tune(self._h, int(trial_steps), C.byref(r)))
+        return r.value
+
+    def sync(self):
+        _chk(load_library().cliffs_ensemble_sync(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().cliffs_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def nest_maps(xp, yp, xc, yc):

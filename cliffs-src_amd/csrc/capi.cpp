@@ -28,6 +28,9 @@ hipError_t launch_step_fixup(const FArgs& a, hipStream_t st);
 hipError_t launch_fused_flags(const FArgs& a, unsigned char* ctr0, unsigned char* ctr1,
                               hipStream_t st);
 hipError_t fused_blocks_per_cu(int* n);
+hipError_t launch_step_batch(const FArgs* args, const int2* bmap, int nblocks, hipStream_t st);
+hipError_t launch_fixup_batch(const FArgs* args, const int2* bmap, int nblocks, hipStream_t st);
+void ensemble_block_map(const int* ntiles, int n, std::vector<int2>& bmap);
 void fused_geometry(int nx, int ny, int cus, int blocks_per_cu, int* nstrips, int* nchunks,
                     int* rows);
 }  // namespace cliffs
@@ -102,6 +105,7 @@ struct cliffs_handle {
     bool fused = true;         // 2-D steps run as one fused kernel (+ fixup)
     bool fused_ok = false;     // the nine fields span < 4 GiB (32-bit offsets)
     int f_nstrips = 0, f_nchunks = 0, f_rows = 0;
+    int f_cus = 0, f_bpc = 0;  // CUs of the device, fused-kernel blocks per CU
     int f_cstride = 0;         // bytes per column of cpr: roundup(nchunks, 16)
     unsigned char* fflags = nullptr;  // rfl[2][nstrips][ny], cpr[2][nx][cstride], ctr[2][nx]
     int fn = 0;                // fused step parity
@@ -346,6 +350,7 @@ FArgs fargs(cliffs_handle* h, const float* xe, const float* ye) {
     a.row_hi = h->jj2;
     a.col_lo = h->ii1;
     a.col_hi = h->ii2;
+    a.maxwave = 1;
     return a;
 }
 
@@ -358,6 +363,35 @@ int ensure_fused(cliffs_handle* h) {
     const hipError_t e = launch_fused_flags(a, h->ctr(0), h->ctr(1), h->st);
     if (e != hipSuccess) return hipfail(e, "fused_flags");
     h->fused_valid = true;
+    return CLIFFS_OK;
+}
+
+// Tile geometry of the fused step: rows per tile (0: enough tiles to fill the
+// device once, for a grid stepped alone) and the per-tile flag buffers.
+int set_fused_rows(cliffs_handle* h, int rows) {
+    int ns, nc, r;
+    if (rows <= 0) {
+        fused_geometry(h->nx, h->ny, h->f_cus, h->f_bpc, &ns, &nc, &r);
+    } else {
+        ns = (h->nx + kFStrip - 1) / kFStrip;
+        r = std::min(std::min(rows, kFMaxRows), h->ny);
+        nc = (h->ny + r - 1) / r;
+    }
+    if (h->fflags && ns == h->f_nstrips && nc == h->f_nchunks && r == h->f_rows) return CLIFFS_OK;
+    if (h->fflags) {
+        HIPCHK(hipStreamSynchronize(h->st));
+        HIPCHK(hipFree(h->fflags));
+        h->fflags = nullptr;
+    }
+    h->f_nstrips = ns;
+    h->f_nchunks = nc;
+    h->f_rows = r;
+    h->f_cstride = (nc + 15) / 16 * 16;
+    const size_t fb = 2 * (h->rfl_bytes() + h->cpr_bytes() + (size_t)h->nx);
+    // cpr bytes beyond nchunks are read as zeros and never written
+    if (hipMalloc(&h->fflags, fb) != hipSuccess) return fail(CLIFFS_ERR_NOMEM, "flag allocation");
+    HIPCHK(hipMemset(h->fflags, 0, fb));
+    h->fused_valid = false;
     return CLIFFS_OK;
 }
 
@@ -535,16 +569,12 @@ int cliffs_create(const cliffs_grid_desc* d, cliffs_handle** out) {
     if (hipMalloc(&h->flags, 3 * h->fbytes) != hipSuccess)
         return bail(fail(CLIFFS_ERR_NOMEM, "flag allocation"));
     {  // fused step: tile geometry for this device, per-tile line flags
-        int cus = 0, bpc = 0;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->dev) != hipSuccess ||
-            fused_blocks_per_cu(&bpc) != hipSuccess || cus < 1 || bpc < 1)
+        if (hipDeviceGetAttribute(&h->f_cus, hipDeviceAttributeMultiprocessorCount, h->dev) !=
+                hipSuccess ||
+            fused_blocks_per_cu(&h->f_bpc) != hipSuccess || h->f_cus < 1 || h->f_bpc < 1)
             return bail(fail(CLIFFS_ERR_HIP, "device occupancy query"));
-        fused_geometry(nx, ny, cus, bpc, &h->f_nstrips, &h->f_nchunks, &h->f_rows);
-        h->f_cstride = (h->f_nchunks + 15) / 16 * 16;
-        const size_t fb = 2 * (h->rfl_bytes() + h->cpr_bytes() + (size_t)nx);
-        // cpr bytes beyond nchunks are read as zeros and never written
-        if (hipMalloc(&h->fflags, fb) != hipSuccess || hipMemset(h->fflags, 0, fb) != hipSuccess)
-            return bail(fail(CLIFFS_ERR_NOMEM, "flag allocation"));
+        const int rc = set_fused_rows(h, 0);
+        if (rc) return bail(rc);
     }
 
     if (hipMemcpy(h->xm, xm.data(), sizeof(float4) * nx, hipMemcpyHostToDevice) != hipSuccess ||
@@ -801,6 +831,264 @@ int cliffs_run_timed(cliffs_handle* h, int nsteps, double* ms_total, double* ms_
 int cliffs_set_fused(cliffs_handle* h, int on) {
     if (!h) return fail(CLIFFS_ERR_ARG, "NULL handle");
     h->fused = on != 0;
+    return CLIFFS_OK;
+}
+
+// ---- ensembles --------------------------------------------------------------
+}  // extern "C"
+
+struct cliffs_ensemble {
+    int dev = 0;
+    hipStream_t st = nullptr;
+    std::vector<cliffs_handle*> m;
+    int rows = 0;                 // common tile height (0: each member's own)
+    FArgs* hargs = nullptr;       // pinned staging, [2][n]: this step's and the next step's
+    FArgs* dargs = nullptr;       // device copy
+    int2* dmap = nullptr;         // block -> (member, tile)
+    int nblocks = 0;
+    hipEvent_t ev = nullptr;      // stream hand-over between the members and the ensemble
+    hipEvent_t argev = nullptr;   // the argument copy has executed (staging reusable)
+    hipGraph_t graph = nullptr;   // two steps (both parities)
+    hipGraphExec_t gexec = nullptr;
+};
+
+namespace {
+
+void free_ensemble(cliffs_ensemble* e) {
+    if (!e) return;
+    (void)hipSetDevice(e->dev);
+    if (e->st) (void)hipStreamSynchronize(e->st);
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    if (e->ev) (void)hipEventDestroy(e->ev);
+    if (e->argev) (void)hipEventDestroy(e->argev);
+    (void)hipFree(e->dargs);
+    (void)hipFree(e->dmap);
+    if (e->hargs) (void)hipHostFree(e->hargs);
+    if (e->st) (void)hipStreamDestroy(e->st);
+    delete e;
+}
+
+// Block map and the captured two-step graph for the members' current geometry.
+int ensemble_rebuild(cliffs_ensemble* e) {
+    const int n = (int)e->m.size();
+    std::vector<int> nt(n);
+    for (int i = 0; i < n; i++) nt[i] = e->m[i]->f_nstrips * e->m[i]->f_nchunks;
+    std::vector<int2> bmap;
+    ensemble_block_map(nt.data(), n, bmap);
+    HIPCHK(hipStreamSynchronize(e->st));
+    if (e->gexec) HIPCHK(hipGraphExecDestroy(e->gexec));
+    if (e->graph) HIPCHK(hipGraphDestroy(e->graph));
+    e->gexec = nullptr;
+    e->graph = nullptr;
+    HIPCHK(hipFree(e->dmap));
+    e->dmap = nullptr;
+    e->nblocks = (int)bmap.size();
+    if (hipMalloc(&e->dmap, sizeof(int2) * bmap.size()) != hipSuccess)
+        return fail(CLIFFS_ERR_NOMEM, "ensemble block map");
+    HIPCHK(hipMemcpy(e->dmap, bmap.data(), sizeof(int2) * bmap.size(), hipMemcpyHostToDevice));
+    // two steps: the argument blocks of this step, then of the next
+    HIPCHK(hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal));
+    hipError_t err = hipSuccess;
+    for (int k = 0; k < 2 && err == hipSuccess; k++) {
+        err = launch_step_batch(e->dargs + (size_t)k * n, e->dmap, e->nblocks, e->st);
+        if (err == hipSuccess) err = launch_fixup_batch(e->dargs + (size_t)k * n, e->dmap, e->nblocks, e->st);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(e->st, &g);
+    if (err != hipSuccess) return hipfail(err, "ensemble capture");
+    if (ec != hipSuccess) return hipfail(ec, "hipStreamEndCapture");
+    e->graph = g;
+    HIPCHK(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
+    return CLIFFS_OK;
+}
+
+// Members' work so far happens before the ensemble's next work; flags of the
+// fused step are made current on each member's stream first.
+int ensemble_take(cliffs_ensemble* e) {
+    for (cliffs_handle* h : e->m) {
+        int rc;
+        if ((rc = ensure_fused(h))) return rc;
+        HIPCHK(hipEventRecord(e->ev, h->st));
+        HIPCHK(hipStreamWaitEvent(e->st, e->ev, 0));
+    }
+    return CLIFFS_OK;
+}
+
+// The ensemble's work so far happens before each member's next work.
+int ensemble_give(cliffs_ensemble* e) {
+    HIPCHK(hipEventRecord(e->ev, e->st));
+    for (cliffs_handle* h : e->m) HIPCHK(hipStreamWaitEvent(h->st, e->ev, 0));
+    return CLIFFS_OK;
+}
+
+// Argument blocks of every member for its current step (slot 0) and the
+// step after it (slot 1), copied to the device on the ensemble's stream.
+// maxwave = 0 (timing trials) leaves the running maxima untouched.
+int ensemble_args(cliffs_ensemble* e, int maxwave = 1) {
+    const int n = (int)e->m.size();
+    HIPCHK(hipEventSynchronize(e->argev));  // the previous copy has left the staging buffer
+    for (int i = 0; i < n; i++) {
+        cliffs_handle* h = e->m[i];
+        e->hargs[i] = fargs(h, nullptr, nullptr);
+        h->cur ^= 1;
+        h->fn++;
+        e->hargs[n + i] = fargs(h, nullptr, nullptr);
+        h->cur ^= 1;
+        h->fn--;
+        e->hargs[i].maxwave = e->hargs[n + i].maxwave = maxwave;
+    }
+    HIPCHK(hipMemcpyAsync(e->dargs, e->hargs, sizeof(FArgs) * 2 * n, hipMemcpyHostToDevice, e->st));
+    HIPCHK(hipEventRecord(e->argev, e->st));
+    return CLIFFS_OK;
+}
+
+int ensemble_single(cliffs_ensemble* e, int slot) {
+    const FArgs* a = e->dargs + (size_t)slot * e->m.size();
+    hipError_t err = launch_step_batch(a, e->dmap, e->nblocks, e->st);
+    if (err != hipSuccess) return hipfail(err, "ensemble step");
+    err = launch_fixup_batch(a, e->dmap, e->nblocks, e->st);
+    if (err != hipSuccess) return hipfail(err, "ensemble fixup");
+    return CLIFFS_OK;
+}
+
+int ensemble_set_rows(cliffs_ensemble* e, int rows) {
+    HIPCHK(hipStreamSynchronize(e->st));
+    for (cliffs_handle* h : e->m) {
+        const int rc = set_fused_rows(h, rows);
+        if (rc) return rc;
+    }
+    e->rows = rows;
+    return ensemble_rebuild(e);
+}
+
+}  // namespace
+
+extern "C" {
+
+int cliffs_ensemble_create(cliffs_handle* const* members, int n, cliffs_ensemble** out) {
+    if (!members || n < 1 || !out) return fail(CLIFFS_ERR_ARG, "bad ensemble arguments");
+    *out = nullptr;
+    for (int i = 0; i < n; i++) {
+        cliffs_handle* h = members[i];
+        if (!h) return fail(CLIFFS_ERR_ARG, "NULL member");
+        if (h->dev != members[0]->dev) return fail(CLIFFS_ERR_ARG, "members on different devices");
+        if (!(h->xrun && h->yrun) || !h->fused_ok)
+            return fail(CLIFFS_ERR_ARG, "ensemble members must be 2-D grids (nx, ny > 2) of < ~119M cells");
+        if (!h->state) return fail(CLIFFS_ERR_STATE, "member without uploaded state");
+        for (int k = 0; k < i; k++)
+            if (members[k] == h) return fail(CLIFFS_ERR_ARG, "member listed twice");
+    }
+    cliffs_ensemble* e = new (std::nothrow) cliffs_ensemble;
+    if (!e) return fail(CLIFFS_ERR_NOMEM, "host allocation");
+    e->dev = members[0]->dev;
+    e->m.assign(members, members + n);
+    auto bail = [&](int rc) {
+        free_ensemble(e);
+        return rc;
+    };
+    if (hipSetDevice(e->dev) != hipSuccess) return bail(fail(CLIFFS_ERR_HIP, "hipSetDevice"));
+    if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e->ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->argev, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(CLIFFS_ERR_HIP, "ensemble stream/events"));
+    if (hipHostMalloc(&e->hargs, sizeof(FArgs) * 2 * n) != hipSuccess ||
+        hipMalloc(&e->dargs, sizeof(FArgs) * 2 * n) != hipSuccess)
+        return bail(fail(CLIFFS_ERR_NOMEM, "ensemble argument blocks"));
+    if (hipEventRecord(e->argev, e->st) != hipSuccess) return bail(fail(CLIFFS_ERR_HIP, "event"));
+    const int rc = ensemble_rebuild(e);
+    if (rc) return bail(rc);
+    *out = e;
+    return CLIFFS_OK;
+}
+
+int cliffs_ensemble_destroy(cliffs_ensemble* e) {
+    free_ensemble(e);
+    return CLIFFS_OK;
+}
+
+int cliffs_ensemble_run(cliffs_ensemble* e, int nsteps) {
+    if (!e || nsteps < 0) return fail(CLIFFS_ERR_ARG, "bad ensemble run arguments");
+    if (nsteps == 0) return CLIFFS_OK;
+    HIPCHK(hipSetDevice(e->dev));
+    int rc;
+    if ((rc = ensemble_take(e)) || (rc = ensemble_args(e))) return rc;
+    for (int k = 0; k < nsteps / 2; k++) HIPCHK(hipGraphLaunch(e->gexec, e->st));
+    if ((nsteps & 1) && (rc = ensemble_single(e, 0))) return rc;
+    for (cliffs_handle* h : e->m) {
+        h->cur ^= (nsteps & 1);
+        h->fn += nsteps;
+        h->flags_valid = false;
+    }
+    return ensemble_give(e);
+}
+
+int cliffs_ensemble_run_timed(cliffs_ensemble* e, int nsteps, double* ms_total) {
+    if (!e || nsteps < 1 || !ms_total) return fail(CLIFFS_ERR_ARG, "bad ensemble run arguments");
+    HIPCHK(hipSetDevice(e->dev));
+    EventSet es;
+    es.ev.assign(2, nullptr);
+    for (auto& x : es.ev) HIPCHK(hipEventCreate(&x));
+    int rc;
+    if ((rc = ensemble_take(e)) || (rc = ensemble_args(e))) return rc;
+    HIPCHK(hipEventRecord(es.ev[0], e->st));
+    for (int k = 0; k < nsteps / 2; k++) HIPCHK(hipGraphLaunch(e->gexec, e->st));
+    if ((nsteps & 1) && (rc = ensemble_single(e, 0))) return rc;
+    HIPCHK(hipEventRecord(es.ev[1], e->st));
+    for (cliffs_handle* h : e->m) {
+        h->cur ^= (nsteps & 1);
+        h->fn += nsteps;
+        h->flags_valid = false;
+    }
+    if ((rc = ensemble_give(e))) return rc;
+    HIPCHK(hipEventSynchronize(es.ev[1]));
+    float t;
+    HIPCHK(hipEventElapsedTime(&t, es.ev[0], es.ev[1]));
+    *ms_total = t;
+    return CLIFFS_OK;
+}
+
+int cliffs_ensemble_tune(cliffs_ensemble* e, int trial_steps, int* rows_out) {
+    if (!e || trial_steps < 1) return fail(CLIFFS_ERR_ARG, "bad tune arguments");
+    HIPCHK(hipSetDevice(e->dev));
+    static const int kCand[] = {32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024};
+    EventSet es;
+    es.ev.assign(2, nullptr);
+    for (auto& x : es.ev) HIPCHK(hipEventCreate(&x));
+    int best = 0, rc;
+    float best_ms = 0.f;
+    for (int c : kCand) {
+        if ((rc = ensemble_set_rows(e, c))) return rc;
+        if ((rc = ensemble_take(e)) || (rc = ensemble_args(e, 0))) return rc;
+        // the same step, repeated: it reads the current state and writes the
+        // other copy, so the state does not advance; the running maxima are
+        // not touched (maxwave = 0) and the line flags are rebuilt afterwards
+        if ((rc = ensemble_single(e, 0))) return rc;
+        HIPCHK(hipEventRecord(es.ev[0], e->st));
+        for (int k = 0; k < trial_steps; k++)
+            if ((rc = ensemble_single(e, 0))) return rc;
+        HIPCHK(hipEventRecord(es.ev[1], e->st));
+        HIPCHK(hipEventSynchronize(es.ev[1]));
+        float t;
+        HIPCHK(hipEventElapsedTime(&t, es.ev[0], es.ev[1]));
+        for (cliffs_handle* h : e->m) h->fused_valid = false;
+        if (best == 0 || t < best_ms) {
+            best = c;
+            best_ms = t;
+        }
+    }
+    if ((rc = ensemble_set_rows(e, best))) return rc;
+    for (cliffs_handle* h : e->m) h->fused_valid = false;
+    if ((rc = ensemble_give(e))) return rc;
+    if (rows_out) *rows_out = best;
+    return CLIFFS_OK;
+}
+
+int cliffs_ensemble_sync(cliffs_ensemble* e) {
+    if (!e) return fail(CLIFFS_ERR_ARG, "NULL ensemble");
+    HIPCHK(hipSetDevice(e->dev));
+    HIPCHK(hipStreamSynchronize(e->st));
+    HIPCHK(hipGetLastError());
     return CLIFFS_OK;
 }
 

@@ -105,6 +105,7 @@ struct FArgs {
     unsigned char* ctr_next;      // [nx]: the next step's ctr, cleared by the fixup kernel
     int row_lo, row_hi;           // rows the x-sweep covers (0-based, inclusive)
     int col_lo, col_hi;           // columns the y-sweep covers
+    int maxwave;                  // update hmax/velmax (0 only for timing trials)
 };
 
 // One-way nest feed (nest.hip)

@@ -30,6 +30,8 @@
 // opposite mistake needs no deferral.)
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "sweep_core.h"
 
 namespace cliffs {
@@ -119,7 +121,7 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
     } else if (in_range) {
         yflag = column_any(a.cpr_in, a.cstride, col) ? 1 : 0;
     }
-    const int keepmax = (col_ok && (kExact || !in_range || yflag)) ? 1 : 0;  // else deferred
+    const int keepmax = (a.maxwave && col_ok && (kExact || !in_range || yflag)) ? 1 : 0;  // else deferred
     YCol cc;
     cc.swept = in_range && yflag;
     cc.load_edges(a.yedges, nx, col, col_ok);
@@ -253,13 +255,10 @@ __global__ __launch_bounds__(256, 3) void step_fused_kernel(const FArgs a) {
     fused_tile<false>(a, tile, sh);
 }
 
-// Checks the step kernel's column predictions tile by tile: recomputes the
-// tiles with a wrong prediction, applies the deferred max-wave update of
+// Checks the step kernel's column predictions of one tile: recomputes the
+// tile if a prediction was wrong, applies the deferred max-wave update of
 // columns that stayed dry, and clears the next step's ctr.
-__global__ __launch_bounds__(256) void step_fixup_kernel(const FArgs a) {
-    __shared__ FShared sh;
-    const int tile = tile_of_block(blockIdx.x, a.ntiles);
-    if (tile < 0) return;
+__device__ __forceinline__ void fixup_tile(const FArgs& a, int tile, FShared& sh) {
     const int nx = a.g.nx, ny = a.g.ny;
     const int strip = tile % a.nstrips, chunk = tile / a.nstrips;
     const int t = threadIdx.x;
@@ -273,7 +272,7 @@ __global__ __launch_bounds__(256) void step_fixup_kernel(const FArgs a) {
         fused_tile<true>(a, tile, sh);
         return;
     }
-    if (in_range && !pred) {  // column left unswept: max-wave of its final state
+    if (a.maxwave && in_range && !pred) {  // column left unswept: max-wave of its final state
         const int j0 = chunk * a.rows, j1 = min(j0 + a.rows, ny);
         for (int r = j0; r < j1; r++) {
             const size_t o = (size_t)r * a.g.pitch + col;
@@ -282,6 +281,31 @@ __global__ __launch_bounds__(256) void step_fixup_kernel(const FArgs a) {
             a.g.VM[o] = fmaxf(a.g.VM[o], u * u + v * v);
         }
     }
+}
+
+__global__ __launch_bounds__(256) void step_fixup_kernel(const FArgs a) {
+    __shared__ FShared sh;
+    const int tile = tile_of_block(blockIdx.x, a.ntiles);
+    if (tile < 0) return;
+    fixup_tile(a, tile, sh);
+}
+
+// Ensembles: one launch steps every member grid.  bmap[b] = (member, tile)
+// or (-1, -1); args[member] is that member's argument block for this step.
+__global__ __launch_bounds__(256, 3) void step_fused_batch_kernel(const FArgs* __restrict__ args,
+                                                                  const int2* __restrict__ bmap) {
+    __shared__ FShared sh;
+    const int2 bm = bmap[blockIdx.x];
+    if (bm.x < 0) return;
+    fused_tile<false>(args[bm.x], bm.y, sh);
+}
+
+__global__ __launch_bounds__(256) void step_fixup_batch_kernel(const FArgs* __restrict__ args,
+                                                               const int2* __restrict__ bmap) {
+    __shared__ FShared sh;
+    const int2 bm = bmap[blockIdx.x];
+    if (bm.x < 0) return;
+    fixup_tile(args[bm.x], bm.y, sh);
 }
 
 // Line flags of the current state (written to rfl_out, cpr_out) and cleared
@@ -331,6 +355,16 @@ hipError_t launch_step_fixup(const FArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
+hipError_t launch_step_batch(const FArgs* args, const int2* bmap, int nblocks, hipStream_t st) {
+    hipLaunchKernelGGL(step_fused_batch_kernel, dim3(nblocks), dim3(256), 0, st, args, bmap);
+    return hipGetLastError();
+}
+
+hipError_t launch_fixup_batch(const FArgs* args, const int2* bmap, int nblocks, hipStream_t st) {
+    hipLaunchKernelGGL(step_fixup_batch_kernel, dim3(nblocks), dim3(256), 0, st, args, bmap);
+    return hipGetLastError();
+}
+
 hipError_t launch_fused_flags(const FArgs& a, unsigned char* ctr0, unsigned char* ctr1,
                               hipStream_t st) {
     const long n = (long)a.nstrips * a.g.ny + (long)a.nchunks * a.g.nx;
@@ -342,6 +376,25 @@ hipError_t launch_fused_flags(const FArgs& a, unsigned char* ctr0, unsigned char
 
 hipError_t fused_blocks_per_cu(int* n) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, step_fused_kernel, 256, 0);
+}
+
+// Block map of an ensemble step: every member's tiles, XCD by XCD.  Member
+// m's tiles t in [lo(m,x), lo(m,x+1)) go to XCD x in order, so each XCD runs
+// contiguous (chunk, strip) ranges of every member (neighbouring tiles share
+// its L2); block b runs entry b / kXCDs of XCD b % kXCDs's list.
+void ensemble_block_map(const int* ntiles, int n, std::vector<int2>& bmap) {
+    std::vector<std::vector<int2>> per(kXCDs);
+    for (int m = 0; m < n; m++)
+        for (int x = 0; x < kXCDs; x++) {
+            const int lo = (int)(((long)ntiles[m] * x) / kXCDs);
+            const int hi = (int)(((long)ntiles[m] * (x + 1)) / kXCDs);
+            for (int t = lo; t < hi; t++) per[x].push_back(make_int2(m, t));
+        }
+    size_t len = 0;
+    for (auto& v : per) len = v.size() > len ? v.size() : len;
+    bmap.assign(len * kXCDs, make_int2(-1, -1));
+    for (int x = 0; x < kXCDs; x++)
+        for (size_t k = 0; k < per[x].size(); k++) bmap[k * kXCDs + x] = per[x][k];
 }
 
 // Tile geometry: enough tiles to fill every CU with `blocks_per_cu` blocks

@@ -116,6 +116,31 @@ int cliffs_run_timed(cliffs_handle *h, int nsteps, double *ms_total, double *ms_
  * step is always two sweeps (cliffs_main.f:317-345). */
 int cliffs_set_fused(cliffs_handle *h, int on);
 
+/* ---- ensembles: many grids stepped together ---------------------------- */
+
+typedef struct cliffs_ensemble cliffs_ensemble;
+
+/* Group n grids (2-D, state uploaded, one device) to advance together: one
+ * step of the ensemble is one time step (cliffs_main.f:317-355, open edges)
+ * of every member, all members' tiles in one launch of the fused step
+ * kernel, and consecutive steps are replayed from a captured HIP graph.
+ * Each member's results are bit-identical to stepping it alone.  No
+ * counterpart in the reference, which runs one scenario per process.  The
+ * members stay owned by the caller and usable between ensemble calls (calls
+ * are ordered across the members' streams and the ensemble's); destroy the
+ * ensemble before any member. */
+int cliffs_ensemble_create(cliffs_handle *const *members, int n, cliffs_ensemble **out);
+int cliffs_ensemble_destroy(cliffs_ensemble *e);
+int cliffs_ensemble_run(cliffs_ensemble *e, int nsteps);
+/* As cliffs_ensemble_run; ms_total = device time of the nsteps. */
+int cliffs_ensemble_run_timed(cliffs_ensemble *e, int nsteps, double *ms_total);
+/* Autotuner: times trial_steps repetitions of one ensemble step for each
+ * candidate tile height (32 .. 1024 rows of 248 columns) and keeps the
+ * fastest (rows_out).  The trials change neither the state nor the running
+ * maxima (their max-wave update is switched off). */
+int cliffs_ensemble_tune(cliffs_ensemble *e, int trial_steps, int *rows_out);
+int cliffs_ensemble_sync(cliffs_ensemble *e);
+
 /* Gauge sampling (output_gages.f:18-29): eta = c^2/g - d, or -1e32 (the
  * reference's real*4 NAN) where c <= celmin; u = xvel, v = yvel.  I, J are
  * 1-based. */

@@ -1,0 +1,104 @@
+"""Ensembles (cliffs_ensemble_*): all members stepped by one fused launch per
+step, replayed from a HIP graph.  Every member must end bit-identical to the
+same grid stepped alone, whatever the tile heights the autotuner picks, the
+step count (graph pairs plus a single step), the members' starting step
+parity and the hand-back to single-grid calls."""
+import numpy as np
+import pytest
+
+import synth
+from test_fused import NAMES, _plane_beach, _transposed
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases():
+    return [
+        synth.coastal(nx=300, ny=260, seed=1),
+        synth.open_ocean(n=256, seed=2, dx=500.0, ny=200),
+        synth.coastal(nx=520, ny=300, seed=3, crough=0.0025),
+        _plane_beach(nx=400, ny=64, seed=4),
+        _transposed(synth.coastal(nx=240, ny=300, seed=5)),
+    ]
+
+
+def _grid(gpu_lib, case):
+    s1, s2, zeta = case.metrics()
+    g = gpu_lib.Grid(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground, crough=case.crough)
+    g.upload(case.cel, case.xvel, case.yvel)
+    return g
+
+
+def _state(g):
+    c, u, v = g.download()
+    hm, vm = g.download_max()
+    return c, u, v, hm, vm
+
+
+def _same(a, b, k):
+    for x, y, name in zip(a, b, NAMES):
+        bad = int((x.view(np.int64) != y.view(np.int64)).sum())
+        assert bad == 0, f"member {k} {name}: {bad} cells differ"
+
+
+@pytest.mark.parametrize("tune", [False, True])
+def test_ensemble_matches_members_alone(gpu_lib, tune):
+    cases = _cases()
+    pre = [0, 0, 1, 0, 0]            # member 2 starts one step ahead (other parity)
+    steps, after = 31, 3
+    want = []
+    for case, p in zip(cases, pre):
+        g = _grid(gpu_lib, case)
+        g.run(p + steps + after)
+        want.append(_state(g))
+        g.close()
+    grids = [_grid(gpu_lib, c) for c in cases]
+    for g, p in zip(grids, pre):
+        g.run(p)
+    ens = gpu_lib.Ensemble(grids)
+    if tune:
+        rows = ens.tune(2)
+        assert 32 <= rows <= 1024
+    ens.run(20)
+    ens.run(steps - 20)              # odd remainder: one step outside the graph
+    for g in grids:                  # hand-back: single-grid steps after the ensemble
+        g.run(after)
+    got = [_state(g) for g in grids]
+    ens.close()
+    for g in grids:
+        g.close()
+    for k, (a, b) in enumerate(zip(got, want)):
+        _same(a, b, k)
+
+
+def test_ensemble_run_timed_and_repeat(gpu_lib):
+    cases = _cases()[:3]
+    grids = [_grid(gpu_lib, c) for c in cases]
+    ens = gpu_lib.Ensemble(grids)
+    ms = ens.run_timed(8)
+    assert ms > 0.0
+    ens.run(8)
+    ens.sync()
+    got = [_state(g) for g in grids]
+    ens.close()
+    for g in grids:
+        g.close()
+    for k, case in enumerate(cases):
+        g = _grid(gpu_lib, case)
+        g.run(16)
+        _same(got[k], _state(g), k)
+        g.close()
+
+
+def test_ensemble_rejects_bad_members(gpu_lib):
+    beach = synth.beach_1d(seed=0, n=300)
+    s1, s2, zeta = beach.metrics()
+    g1 = gpu_lib.Grid(beach.dep, s1, s2, zeta, dt=beach.dt, ground=beach.ground)
+    g1.upload(beach.cel, beach.xvel, beach.yvel)
+    with pytest.raises(gpu_lib.CliffsError):
+        gpu_lib.Ensemble([g1])       # 1-D grids step with the sweep kernels only
+    g2 = _grid(gpu_lib, synth.coastal(nx=100, ny=90, seed=6))
+    with pytest.raises(gpu_lib.CliffsError):
+        gpu_lib.Ensemble([g2, g2])
+    g1.close()
+    g2.close()
