@@ -187,3 +187,98 @@ def coastal(nx=256, ny=192, seed=0, cartesian=True, dx=500.0, steps=200, source_
     z = np.zeros_like(dep)
     return Case("coastal", dep, cel, z.copy(), z.copy(), x, y, cartesian, dt, ground, crough,
                 steps, dict(seed=seed, amplitude=source_amp))
+
+
+def ssl_condition(dep, depmin=1.0, alpha=2.0, passes=50):
+    """Depth conditioning in the manner of depth_ssl.f:236-290: in open water
+    (depth >= depmin, both neighbours too) a node whose sqrt-depth differs
+    from a neighbour's by more than the factor alpha is replaced by the mean
+    sqrt-depth of its two neighbours, along x and along y, until no node
+    violates the bound (vectorised passes instead of the reference's
+    in-place sweep)."""
+    d = np.array(dep, dtype=np.float64)
+    for _ in range(passes):
+        changed = 0
+        for axis in (1, 0):
+            q = np.sqrt(np.maximum(d, 0.0))
+            wet = d >= depmin
+            qm = np.roll(q, 1, axis)
+            qp = np.roll(q, -1, axis)
+            ok = wet & np.roll(wet, 1, axis) & np.roll(wet, -1, axis)
+            if axis == 1:
+                ok[:, 0] = ok[:, -1] = False
+            else:
+                ok[0, :] = ok[-1, :] = False
+            with np.errstate(divide="ignore", invalid="ignore"):
+                bad = ok & ((qp > alpha * q) | (qm > alpha * q))
+            qn = 0.5 * (qm + qp)
+            d = np.where(bad, qn * qn, d)
+            changed += int(bad.sum())
+        if changed == 0:
+            break
+    return d
+
+
+def shelf_coast(nx, ny, seed, dx, source_amp, source_len, steps=4000, cfl=0.7):
+    """Cartesian shelf and coastline for ensemble scenarios: fractal abyssal
+    plain 4-6 km, a continental slope and a shelf whose sqrt-depth varies
+    linearly (200 m shelf, 2 m at the shore), land rising to +300 m behind a
+    seeded coastline, SSL conditioning (ssl_condition); dipole source
+    (+A, -A/2) of the given length with seeded strike and location."""
+    rng = np.random.default_rng(seed)
+    x = np.arange(nx, dtype=np.float64) * dx
+    y = np.arange(ny, dtype=np.float64) * dx
+    X, Y = np.meshgrid(x, y)
+    Lx, Ly = x[-1], y[-1]
+    fr = fractal_field(ny, 1, rng, beta=2.0)[:, 0]
+    xc = (Lx * (0.75 + 0.05 * fr))[:, None]          # coastline position along x
+    dist = xc - X                                     # > 0 offshore
+    shelf_w, slope_w = 0.14 * Lx, 0.08 * Lx
+    abyss = 4000.0 + 2000.0 * (fractal_field(ny, nx, rng) * 0.5 + 0.5)
+    qa, qs, qc = np.sqrt(abyss), np.sqrt(200.0), np.sqrt(2.0)
+    t_slope = np.clip((dist - shelf_w) / slope_w, 0.0, 1.0)     # 0 at the shelf edge
+    t_shelf = np.clip(dist / shelf_w, 0.0, 1.0)                 # 0 at the coast
+    q = np.where(dist > shelf_w, qs + (qa - qs) * t_slope, qc + (qs - qc) * np.sqrt(t_shelf))
+    dep = q * q * (1.0 + 0.05 * fractal_field(ny, nx, rng) * (dist > 0))
+    land = dist <= 0
+    dep = np.where(land, dist / (0.25 * Lx) * 300.0 - 0.5, dep)
+    dep = ssl_condition(dep)
+    sx = rng.uniform(0.35, 0.55) * Lx
+    sy = rng.uniform(0.3, 0.7) * Ly
+    th = rng.uniform(0, np.pi)
+    a = min(0.5 * source_len, 0.2 * Lx)
+    b = 0.5 * a
+    xr = (X - sx) * np.cos(th) + (Y - sy) * np.sin(th)
+    yr = -(X - sx) * np.sin(th) + (Y - sy) * np.cos(th)
+    src = source_amp * np.exp(-((xr - 0.5 * a) / a) ** 2 - (yr / b) ** 2) \
+        - 0.5 * source_amp * np.exp(-((xr + 0.5 * a) / a) ** 2 - (yr / b) ** 2)
+    src = np.where(dep > 0, src, 0.0)
+    ground = 0.01
+    cel = depth_to_celerity(dep + src, ground)
+    dt = cfl * dx / math.sqrt(GRAV * (float(dep.max()) + 2 * source_amp))
+    z = np.zeros_like(dep)
+    return Case("shelf_coast", dep, cel, z.copy(), z.copy(), x, y, True, dt, ground, 0.0, steps,
+                dict(seed=seed, amplitude=source_amp, dx=dx, source_len=source_len))
+
+
+def ensemble_scenario(seed, steps=4000):
+    """One scenario of the batched-ensemble workload: nx, ny drawn in
+    [512, 2048], Cartesian dx in [200, 1000] m, shelf and seeded coastline
+    (shelf_coast), dipole source of length 50-300 km (at most 0.4 of the
+    domain width), seeded strike and location, amplitude 0.5-8 m; run-up on."""
+    rng = np.random.default_rng(10_000 + seed)
+    nx = int(rng.integers(512, 2049))
+    ny = int(rng.integers(512, 2049))
+    dx = float(rng.uniform(200.0, 1000.0))
+    length = float(rng.uniform(50e3, 300e3))
+    amp = float(rng.uniform(0.5, 8.0))
+    c = shelf_coast(nx, ny, seed, dx, amp, length, steps=steps)
+    c.name = "ensemble"
+    return c
+
+
+def max_runup(hmax, dep, celmin):
+    """Highest ground reached: max of -dep over cells whose celerity maximum
+    exceeded celmin (0 where no land cell was flooded)."""
+    flooded = (np.asarray(hmax) > celmin) & (np.asarray(dep) < 0.0)
+    return float((-np.asarray(dep)[flooded]).max()) if flooded.any() else 0.0

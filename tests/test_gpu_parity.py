@@ -257,3 +257,27 @@ def test_gauges(gpu):
         else:
             assert eta[q] == float(np.float32(-1e32))
         assert abs(u[q] - u_o[j, i]) < 1e-4 and abs(v[q] - v_o[j, i]) < 1e-4
+
+
+def test_shelf_coast_runup_parity(gpu):
+    """A small scenario of the ensemble workload (SSL-conditioned shelf and
+    coastline, dipole source, frictionless run-up): the flooded area and the
+    maximum run-up of the device path against the FP64 oracle — run-up within
+    2 % (and within one cell's ground height), flooded cells within 1 %."""
+    case = synth.shelf_coast(nx=220, ny=180, seed=8, dx=700.0, source_amp=4.0,
+                             source_len=60e3, steps=900)
+    orc, g = _pair(case, gpu)
+    for _ in range(3):
+        orc.steps(300)
+        g.run(300)
+    hm_o = orc.get_state()[3]
+    hm_g = g.download_max()[0]
+    cm = orc.celmin
+    ru_o = synth.max_runup(hm_o, case.dep, cm)
+    ru_g = synth.max_runup(hm_g, case.dep, cm)
+    assert ru_o > 0.5, "the scenario must flood land"
+    step = float(np.abs(np.diff(case.dep, axis=1)).max())   # ground height of one cell
+    assert abs(ru_g - ru_o) <= max(0.02 * ru_o, 0.0) or abs(ru_g - ru_o) <= step
+    fl_o = int(((hm_o > cm) & (case.dep < 0)).sum())
+    fl_g = int(((hm_g > cm) & (case.dep < 0)).sum())
+    assert abs(fl_g - fl_o) <= max(1, 0.01 * fl_o), (fl_o, fl_g)
