@@ -87,6 +87,10 @@ def load_library():
                                 C.POINTER(H)], C.c_int),
         "cliffs_nest_destroy": ([H], C.c_int),
         "cliffs_nest_feed": ([H, C.c_double, C.c_double, _ip, _ip, _dp, _dp, _dp, _dp], C.c_int),
+        "cliffs_nest_couple": ([H, H, C.c_int], C.c_int),
+        "cliffs_nest_record": ([H, C.c_double, C.c_double, C.c_double, _ip, _ip], C.c_int),
+        "cliffs_nest_end_feed": ([H], C.c_int),
+        "cliffs_nest_child_step": ([H, C.c_double], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -138,6 +142,10 @@ class Grid:
         _chk(lib.cliffs_create(C.byref(d), C.byref(h)))
         self._h = h
         self._keep = keep
+        # nests and ensembles using this grid, closed before it (strong
+        # references: a weak one is already cleared when the garbage
+        # collector finalises a grid and its nest together)
+        self._deps = set()
         g, cm = C.c_double(), C.c_double()
         r = [C.c_int() for _ in range(4)]
         _chk(lib.cliffs_info(h, C.byref(g), C.byref(cm), *[C.byref(x) for x in r]))
@@ -147,6 +155,8 @@ class Grid:
         self.mx = max(ny if nx > 2 else 1, nx if ny > 2 else 1)
 
     def close(self):
+        for d in list(getattr(self, "_deps", ())):
+            d.close()           # they hold this grid's device pointers
         if getattr(self, "_h", None):
             load_library().cliffs_destroy(self._h)
             self._h = None
@@ -253,6 +263,8 @@ class Ensemble:
         h = C.c_void_p()
         _chk(load_library().cliffs_ensemble_create(arr, len(self.grids), C.byref(h)))
         self._h = h
+        for g in self.grids:
+            g._deps.add(self)
 
     def run(self, nsteps: int):
         _chk(load_library().cliffs_ensemble_run(self._h, int(nsteps)))
@@ -320,6 +332,7 @@ class Nest:
                                     float(maps["e_cx"]), int(maps["s_ny"]), float(maps["s_cy"]),
                                     int(maps["n_ny"]), float(maps["n_cy"]), C.byref(h)))
         self._h = h
+        parent._deps.add(self)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -340,6 +353,26 @@ class Nest:
         _chk(load_library().cliffs_nest_feed(self._h, float(cuke), float(hmin), C.byref(a),
                                              C.byref(f), _p(w), _p(e), _p(s), _p(n)))
         return bool(a.value), bool(f.value), w.T.copy(), e.T.copy(), s.T.copy(), n.T.copy()
+
+    # -- parent and child on one device ------------------------------------
+    def couple(self, child: Grid, capacity: int = 4):
+        """Keep feed records on the device and step `child` from them."""
+        _chk(load_library().cliffs_nest_couple(self._h, child._h, int(capacity)))
+        self.child = child
+        child._deps.add(self)
+
+    def record(self, cuke, hmin, t, iarv=False):
+        """-> (arrived, records so far)."""
+        a, k = C.c_int(1 if iarv else 0), C.c_int()
+        _chk(load_library().cliffs_nest_record(self._h, float(cuke), float(hmin), float(t),
+                                               C.byref(a), C.byref(k)))
+        return bool(a.value), k.value
+
+    def end_feed(self):
+        _chk(load_library().cliffs_nest_end_feed(self._h))
+
+    def child_step(self, t):
+        _chk(load_library().cliffs_nest_child_step(self._h, float(t)))
 
 
 NAN32 = np.float32(-1.0e32)   # the reference's real*4 NAN (global_modules.f:18)

@@ -23,6 +23,8 @@ hipError_t launch_maxwave(const DevGrid& g, hipStream_t st);
 hipError_t launch_copy_lines(const DevGrid& g, int dir, int first, int last, hipStream_t st);
 hipError_t launch_nest_arrival(const NestArgs& a, hipStream_t st);
 hipError_t launch_nest_feed(const NestArgs& a, hipStream_t st);
+hipError_t launch_child_edges(const double* r0, const double* r1, double cc, double omc, double grav,
+                              int nxc, int nyc, int zero, float* xe, float* ye, hipStream_t st);
 hipError_t launch_step_fused(const FArgs& a, hipStream_t st);
 hipError_t launch_step_fixup(const FArgs& a, hipStream_t st);
 hipError_t launch_fused_flags(const FArgs& a, unsigned char* ctr0, unsigned char* ctr1,
@@ -155,6 +157,18 @@ struct cliffs_nest {
     double* uvq = nullptr;
     size_t nuvq = 0;
     std::vector<double> huvq;
+    // device coupling to a child grid (cliffs_nest_couple)
+    cliffs_handle* child = nullptr;
+    int cap = 0;                       // feed records kept on the device (ring)
+    double* ring = nullptr;            // cap records of nuvq doubles
+    std::vector<hipEvent_t> rec_ev;    // record written (parent stream)
+    std::vector<hipEvent_t> use_ev;    // record last read (child stream)
+    std::vector<double> tinput;        // record times, as get_boundary_feed reads them
+    int irec = 1;                      // child's next record (1-based, cliffs_main.f:305-315)
+    bool bndr = true;                  // bndr_input: feed still running for the child
+    bool ended = false;                // no more records will come
+    float* cedges = nullptr;           // child edges: x [6][nyc], y [6][nxc]
+    NestArgs args(double cuke) const;
 };
 
 namespace {
@@ -1190,7 +1204,17 @@ int cliffs_nest_create(cliffs_handle* h, int nxc, int nyc, const int* we_ny, con
 
 int cliffs_nest_destroy(cliffs_nest* n) {
     if (!n) return CLIFFS_OK;
-    if (n->parent) (void)hipSetDevice(n->parent->dev);
+    if (n->parent) {
+        (void)hipSetDevice(n->parent->dev);
+        if (n->parent->st) (void)hipStreamSynchronize(n->parent->st);
+    }
+    if (n->child && n->child->st) (void)hipStreamSynchronize(n->child->st);
+    for (auto e : n->rec_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto e : n->use_ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipFree(n->ring);
+    (void)hipFree(n->cedges);
     (void)hipFree(n->we_ny);
     (void)hipFree(n->sn_nx);
     (void)hipFree(n->we_cy);
@@ -1200,6 +1224,51 @@ int cliffs_nest_destroy(cliffs_nest* n) {
     delete n;
     return CLIFFS_OK;
 }
+
+}  // extern "C"
+
+NestArgs cliffs_nest::args(double cuke) const {
+    const cliffs_handle* h = parent;
+    NestArgs a{};
+    a.g = h->grid();
+    a.p = h->p;
+    a.grav = h->k.grav;
+    a.nxc = nxc;
+    a.nyc = nyc;
+    a.we_ny = we_ny;
+    a.we_cy = we_cy;
+    a.sn_nx = sn_nx;
+    a.sn_cx = sn_cx;
+    a.w_nx = w_nx; a.e_nx = e_nx; a.s_ny = s_ny; a.n_ny = n_ny;
+    a.w_cx = w_cx; a.e_cx = e_cx; a.s_cy = s_cy; a.n_cy = n_cy;
+    a.ii1 = ii1; a.ii2 = ii2; a.kk1 = kk1; a.kk2 = kk2;
+    a.gthr = (float)(h->k.grav * ((double)0.1f * cuke));  // feed_children_grid.f:30
+    a.cuke = (float)cuke;
+    a.flag = flag;
+    a.uvq = uvq;
+    return a;
+}
+
+namespace {
+
+// Arrival test of feed_children_grid.f:26-46 (until the latch is set; the
+// host reads the flag), on the parent's stream.
+int nest_arrival(cliffs_nest* n, const NestArgs& a, int* iarv) {
+    cliffs_handle* h = n->parent;
+    if (*iarv) return CLIFFS_OK;
+    HIPCHK(hipMemsetAsync(n->flag, 0, sizeof(unsigned), h->st));
+    hipError_t e = launch_nest_arrival(a, h->st);
+    if (e != hipSuccess) return hipfail(e, "nest_arrival");
+    unsigned f = 0;
+    HIPCHK(hipMemcpyAsync(&f, n->flag, sizeof(unsigned), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    *iarv = f ? 1 : 0;
+    return CLIFFS_OK;
+}
+
+}  // namespace
+
+extern "C" {
 
 int cliffs_nest_feed(cliffs_nest* n, double cuke, double hmin, int* iarv, int* fed,
                      double* uvq_w, double* uvq_e, double* uvq_s, double* uvq_n) {
@@ -1211,33 +1280,10 @@ int cliffs_nest_feed(cliffs_nest* n, double cuke, double hmin, int* iarv, int* f
     // dep > ground exactly, any other threshold is not supported
     if (hmin != h->k.ground) return fail(CLIFFS_ERR_ARG, "hmin must equal the grid's ground");
     HIPCHK(hipSetDevice(h->dev));
-    NestArgs a{};
-    a.g = h->grid();
-    a.p = h->p;
-    a.grav = h->k.grav;
-    a.nxc = n->nxc;
-    a.nyc = n->nyc;
-    a.we_ny = n->we_ny;
-    a.we_cy = n->we_cy;
-    a.sn_nx = n->sn_nx;
-    a.sn_cx = n->sn_cx;
-    a.w_nx = n->w_nx; a.e_nx = n->e_nx; a.s_ny = n->s_ny; a.n_ny = n->n_ny;
-    a.w_cx = n->w_cx; a.e_cx = n->e_cx; a.s_cy = n->s_cy; a.n_cy = n->n_cy;
-    a.ii1 = n->ii1; a.ii2 = n->ii2; a.kk1 = n->kk1; a.kk2 = n->kk2;
-    a.gthr = (float)(h->k.grav * ((double)0.1f * cuke));  // feed_children_grid.f:30
-    a.cuke = (float)cuke;
-    a.flag = n->flag;
-    a.uvq = n->uvq;
+    const NestArgs a = n->args(cuke);
     *fed = 0;
-    if (!*iarv) {
-        HIPCHK(hipMemsetAsync(n->flag, 0, sizeof(unsigned), h->st));
-        hipError_t e = launch_nest_arrival(a, h->st);
-        if (e != hipSuccess) return hipfail(e, "nest_arrival");
-        unsigned f = 0;
-        HIPCHK(hipMemcpyAsync(&f, n->flag, sizeof(unsigned), hipMemcpyDeviceToHost, h->st));
-        HIPCHK(hipStreamSynchronize(h->st));
-        *iarv = f ? 1 : 0;
-    }
+    int rc;
+    if ((rc = nest_arrival(n, a, iarv))) return rc;
     if (!*iarv) return CLIFFS_OK;
     hipError_t e = launch_nest_feed(a, h->st);
     if (e != hipSuccess) return hipfail(e, "nest_feed");
@@ -1251,6 +1297,105 @@ int cliffs_nest_feed(cliffs_nest* n, double cuke, double hmin, int* iarv, int* f
     std::memcpy(uvq_n, n->huvq.data() + 2 * ly + lx, sizeof(double) * lx);
     *fed = 1;
     return CLIFFS_OK;
+}
+
+int cliffs_nest_couple(cliffs_nest* n, cliffs_handle* child, int capacity) {
+    if (!n || !child || capacity < 2) return fail(CLIFFS_ERR_ARG, "bad coupling arguments");
+    if (n->child) return fail(CLIFFS_ERR_STATE, "nest already coupled");
+    if (child->nx != n->nxc || child->ny != n->nyc)
+        return fail(CLIFFS_ERR_ARG, "child grid size differs from the nest maps");
+    if (child->dev != n->parent->dev) return fail(CLIFFS_ERR_ARG, "child on another device");
+    if (!child->xrun || !child->yrun) return fail(CLIFFS_ERR_ARG, "child must be a 2-D grid");
+    HIPCHK(hipSetDevice(child->dev));
+    n->cap = capacity;
+    n->rec_ev.assign(capacity, nullptr);
+    n->use_ev.assign(capacity, nullptr);
+    for (int k = 0; k < capacity; k++) {
+        HIPCHK(hipEventCreateWithFlags(&n->rec_ev[k], hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&n->use_ev[k], hipEventDisableTiming));
+        HIPCHK(hipEventRecord(n->use_ev[k], child->st));
+    }
+    if (hipMalloc(&n->ring, sizeof(double) * n->nuvq * capacity) != hipSuccess ||
+        hipMalloc(&n->cedges, sizeof(float) * (6 * (size_t)n->nyc + 6 * (size_t)n->nxc)) != hipSuccess)
+        return fail(CLIFFS_ERR_NOMEM, "nest ring allocation");
+    n->child = child;
+    n->tinput.clear();
+    n->irec = 1;
+    n->bndr = true;
+    n->ended = false;
+    return CLIFFS_OK;
+}
+
+int cliffs_nest_record(cliffs_nest* n, double cuke, double hmin, double t, int* iarv, int* nrec) {
+    if (!n || !iarv || !nrec) return fail(CLIFFS_ERR_ARG, "NULL argument");
+    if (!n->child) return fail(CLIFFS_ERR_STATE, "nest not coupled to a child");
+    cliffs_handle* h = n->parent;
+    if (!h->state) return fail(CLIFFS_ERR_STATE, "no state uploaded");
+    if (hmin != h->k.ground) return fail(CLIFFS_ERR_ARG, "hmin must equal the grid's ground");
+    if (n->ended) return fail(CLIFFS_ERR_STATE, "feed already ended");
+    HIPCHK(hipSetDevice(h->dev));
+    NestArgs a = n->args(cuke);
+    int rc;
+    if ((rc = nest_arrival(n, a, iarv))) return rc;
+    *nrec = (int)n->tinput.size();
+    if (!*iarv) return CLIFFS_OK;
+    const int k = (int)n->tinput.size() + 1;  // 1-based record being written
+    // the slot's previous record (k - cap) must be past the child's need:
+    // the child reads records irec-1 and irec
+    if (k - n->cap >= n->irec - 1 && k > n->cap)
+        return fail(CLIFFS_ERR_STATE, "parent feed too far ahead of the child (raise the capacity)");
+    const int slot = (k - 1) % n->cap;
+    a.uvq = n->ring + (size_t)slot * n->nuvq;
+    HIPCHK(hipStreamWaitEvent(h->st, n->use_ev[slot], 0));
+    const hipError_t e = launch_nest_feed(a, h->st);
+    if (e != hipSuccess) return hipfail(e, "nest_feed");
+    HIPCHK(hipEventRecord(n->rec_ev[slot], h->st));
+    n->tinput.push_back(t);
+    *nrec = k;
+    return CLIFFS_OK;
+}
+
+int cliffs_nest_end_feed(cliffs_nest* n) {
+    if (!n || !n->child) return fail(CLIFFS_ERR_STATE, "nest not coupled to a child");
+    n->ended = true;
+    return CLIFFS_OK;
+}
+
+int cliffs_nest_child_step(cliffs_nest* n, double t) {
+    if (!n || !n->child) return fail(CLIFFS_ERR_STATE, "nest not coupled to a child");
+    cliffs_handle* c = n->child;
+    if (!c->state) return fail(CLIFFS_ERR_STATE, "child state not uploaded");
+    const int nrec = (int)n->tinput.size();
+    if (nrec < 1) return fail(CLIFFS_ERR_STATE, "no feed record yet");
+    HIPCHK(hipSetDevice(c->dev));
+    // cliffs_main.f:305-315: tinput(irec-1) <= t < tinput(irec)
+    while (n->bndr && t >= n->tinput[n->irec - 1]) {
+        if (n->irec == nrec) {
+            if (!n->ended) return fail(CLIFFS_ERR_STATE, "child ahead of the parent feed");
+            n->bndr = false;  // feed ended: edges 0 from now on
+        } else {
+            n->irec++;
+        }
+    }
+    const int zero = n->bndr ? 0 : 1;
+    double cc = 0.0;
+    const double *r0 = n->ring, *r1 = n->ring;
+    if (!zero) {
+        if (n->irec < 2) return fail(CLIFFS_ERR_STATE, "child time before the second feed record");
+        cc = (n->tinput[n->irec - 1] - t) / (n->tinput[n->irec - 1] - n->tinput[n->irec - 2]);
+        const int s0 = (n->irec - 2) % n->cap, s1 = (n->irec - 1) % n->cap;
+        r0 = n->ring + (size_t)s0 * n->nuvq;
+        r1 = n->ring + (size_t)s1 * n->nuvq;
+        HIPCHK(hipStreamWaitEvent(c->st, n->rec_ev[s1], 0));  // newest record needed is written
+    }
+    hipError_t e = launch_child_edges(r0, r1, cc, 1.0 - cc, c->k.grav, n->nxc, n->nyc, zero,
+                                      n->cedges, n->cedges + 6 * (size_t)n->nyc, c->st);
+    if (e != hipSuccess) return hipfail(e, "child_edges");
+    if (!zero) {
+        HIPCHK(hipEventRecord(n->use_ev[(n->irec - 2) % n->cap], c->st));
+        HIPCHK(hipEventRecord(n->use_ev[(n->irec - 1) % n->cap], c->st));
+    }
+    return do_step(c, n->cedges, n->cedges + 6 * (size_t)n->nyc);
 }
 
 int cliffs_sync(cliffs_handle* h) {

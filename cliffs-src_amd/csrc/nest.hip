@@ -14,6 +14,10 @@
 
 #include "cliffs_dev.h"
 
+// The FP64 arithmetic of the feed and of the edge interpolation rounds every
+// product and sum as the reference's real*8 code does: no FMA contraction.
+#pragma clang fp contract(off)
+
 namespace cliffs {
 
 __global__ void nest_arrival_kernel(const NestArgs a) {
@@ -76,6 +80,42 @@ __global__ void nest_feed_kernel(const NestArgs a) {
     }
 #pragma unroll
     for (int f = 0; f < 3; f++) out[q + (size_t)len * f] = (v0[f] + v1[f]) * cc + (w0[f] + w1[f]) * (1.0 - cc);
+}
+
+// Child edges at time t from two feed records (cliffs_main.f:318-324,
+// 333-339): edge = rec0*cc + rec1*(1-cc) in FP64 with every product and sum
+// rounded separately (no contraction, as the reference's real*8 code), then
+// staged for the sweep kernels exactly as host edges are: FP32 normal
+// velocity, tangential velocity and g*eta, [6][n] per sweep direction.
+// zero: the feed has ended (edges 0, cliffs_main.f:306-310).
+__global__ void child_edges_kernel(const double* r0, const double* r1, double cc, double omc,
+                                   double grav, int nxc, int nyc, int zero, float* xe, float* ye) {
+    const int side = blockIdx.y;  // 0 west, 1 east, 2 south, 3 north
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int len = side < 2 ? nyc : nxc;
+    if (q >= len) return;
+    const size_t off = side < 2 ? (size_t)side * 3 * nyc : (size_t)6 * nyc + (size_t)(side - 2) * 3 * nxc;
+    double val[3];
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        const size_t o = off + q + (size_t)len * f;
+        const double p0 = r0[o] * cc;   // separate roundings (contraction is off)
+        const double p1 = r1[o] * omc;
+        val[f] = zero ? 0.0 : p0 + p1;
+    }
+    const int cn = side < 2 ? 0 : 1;  // normal velocity: u for the x-sweep, v for the y-sweep
+    float* out = side < 2 ? xe + (size_t)side * 3 * nyc : ye + (size_t)(side - 2) * 3 * nxc;
+    out[q] = (float)val[cn];
+    out[len + q] = (float)val[1 - cn];
+    out[2 * len + q] = (float)(grav * val[2]);
+}
+
+hipError_t launch_child_edges(const double* r0, const double* r1, double cc, double omc, double grav,
+                              int nxc, int nyc, int zero, float* xe, float* ye, hipStream_t st) {
+    const int len = nxc > nyc ? nxc : nyc;
+    hipLaunchKernelGGL(child_edges_kernel, dim3((len + 127) / 128, 4), dim3(128), 0, st, r0, r1, cc, omc,
+                       grav, nxc, nyc, zero, xe, ye);
+    return hipGetLastError();
 }
 
 hipError_t launch_nest_arrival(const NestArgs& a, hipStream_t st) {

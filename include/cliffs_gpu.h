@@ -180,6 +180,33 @@ int cliffs_nest_destroy(cliffs_nest *n);
 int cliffs_nest_feed(cliffs_nest *n, double cuke, double hmin, int *iarv, int *fed,
                      double *uvq_w, double *uvq_e, double *uvq_s, double *uvq_n);
 
+/* ---- one-way nesting on one device: parent and child stepped together -- */
+
+/* Couple the nest to a child grid (nxc x nyc, same device, 2-D) so that the
+ * parent's feed records stay in device memory -- a ring of `capacity`
+ * records -- and the child's edges are interpolated in time on the device.
+ * Replaces the file round trip feed_children_grid -> output_feed ->
+ * get_boundary_feed -> edge interpolation (cliffs_main.f:304-339) of the
+ * reference, which runs parent and child as separate processes. */
+int cliffs_nest_couple(cliffs_nest *n, cliffs_handle *child, int capacity);
+/* (Destroy a nest before its parent and child grids, and an ensemble before
+ * its members: they use the grids' device memory and streams.) */
+/* feed_children_grid at parent time t: arrival test (the latch iarv as in
+ * cliffs_nest_feed), then, once arrived, one more feed record (*nrec = records
+ * so far).  Fails with CLIFFS_ERR_STATE if the record would overwrite one
+ * the child still needs. */
+int cliffs_nest_record(cliffs_nest *n, double cuke, double hmin, double t, int *iarv, int *nrec);
+/* No more records will come: once past the last record the child's edges
+ * become 0 (cliffs_main.f:306-310). */
+int cliffs_nest_end_feed(cliffs_nest *n);
+/* One child step at child time t (after t = t + dt): records irec-1, irec
+ * with tinput(irec-1) <= t < tinput(irec), starting from the first record,
+ * edges = rec(irec-1)*cc + rec(irec)*(1-cc), cc = (tinput(irec)-t) /
+ * (tinput(irec)-tinput(irec-1)) (cliffs_main.f:305-339), then the step.
+ * Fails with CLIFFS_ERR_STATE if t is beyond the parent's last record and
+ * the feed has not ended. */
+int cliffs_nest_child_step(cliffs_nest *n, double t);
+
 int cliffs_sync(cliffs_handle *h);
 /* The handle's hipStream_t, for callers that enqueue their own work. */
 int cliffs_stream(cliffs_handle *h, void **stream);

@@ -1,0 +1,126 @@
+"""Parent and child grids stepped together on one device
+(cliffs_nest_couple/record/child_step): the feed records stay in device
+memory and the child's edges are interpolated in time on the device.  The
+child must end bit-identical to the reference's data path -- feed records
+brought to the host (cliffs_nest_feed), interpolated in FP64 as
+cliffs_main.f:305-339 does, and handed to cliffs_step as edge arrays."""
+import numpy as np
+import pytest
+from scipy.interpolate import RegularGridInterpolator
+
+import oracle as O
+import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(gpu_lib):
+    case = synth.coastal(nx=200, ny=160, seed=31)
+    s1, s2, zeta = case.metrics()
+    x0, x1 = case.xcrd[100], case.xcrd[185]
+    y0, y1 = case.ycrd[30], case.ycrd[120]
+    xc = np.linspace(x0, x1, 3 * 85 + 1)
+    yc = np.linspace(y0, y1, 3 * 90 + 1)
+    maps = O.nest_maps(case.xcrd, case.ycrd, xc, yc)
+    interp = RegularGridInterpolator((case.ycrd, case.xcrd), case.dep)
+    Yc, Xc = np.meshgrid(yc, xc, indexing="ij")
+    depc = interp(np.stack([Yc, Xc], axis=-1))
+    cs1, cs2, czeta = synth.metrics(xc, yc, True)
+
+    def parent():
+        g = gpu_lib.Grid(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground)
+        g.upload(case.cel, case.xvel, case.yvel)
+        return g
+
+    def child():
+        c = gpu_lib.Grid(depc, cs1, cs2, czeta, dt=case.dt / 3, ground=case.ground)
+        z = np.zeros_like(depc)
+        c.upload(synth.depth_to_celerity(depc, case.ground), z, z)
+        return c
+
+    return case, maps, parent, child
+
+
+def _host_edges(recs, state, t, mx):
+    """cliffs_main.f:305-339 in FP64: records (t, west, east, south, north)."""
+    while state["bndr"] and t >= recs[state["irec"] - 1][0]:
+        assert state["irec"] < len(recs), "child ahead of the feed"
+        state["irec"] += 1
+    k = state["irec"]
+    t0, t1 = recs[k - 2][0], recs[k - 1][0]
+    cc = (t1 - t) / (t1 - t0)
+    out = []
+    for side in range(1, 5):
+        e = recs[k - 2][side] * cc + recs[k - 1][side] * (1 - cc)
+        pad = np.zeros((mx, 3))
+        pad[:e.shape[0]] = e
+        out.append(pad)
+    return out
+
+
+def test_coupled_child_matches_host_feed_path(gpu_lib):
+    case, maps, parent, child = _setup(gpu_lib)
+    cuke, bndout, cycles = 0.01, 2, 30
+    gA, cA = parent(), child()
+    gB, cB = parent(), child()
+    nA = gpu_lib.Nest(gA, maps)
+    nB = gpu_lib.Nest(gB, maps)
+    nB.couple(cB, capacity=4)
+    dtp, dtc = case.dt, case.dt / 3
+    recs, state = [], {"irec": 1, "bndr": True}
+    iarvA = iarvB = False
+    tp, tc, child_steps = 0.0, None, 0
+    for _ in range(cycles):
+        gA.run(bndout)
+        gB.run(bndout)
+        tp += bndout * dtp
+        iarvA, fed, w, e, s, n = nA.feed(cuke, case.ground, iarvA)
+        if fed:
+            recs.append((tp, w, e, s, n))
+        iarvB, nrec = nB.record(cuke, case.ground, tp, iarvB)
+        assert iarvA == iarvB and nrec == len(recs)
+        if len(recs) < 2:
+            continue
+        if tc is None:
+            tc = recs[0][0]
+        while tc + dtc < recs[-1][0]:
+            tc += dtc
+            cA.step(*_host_edges(recs, state, tc, cA.mx))
+            nB.child_step(tc)
+            child_steps += 1
+    assert len(recs) >= 10 and child_steps >= 100
+    a = cA.download() + cA.download_max()
+    b = cB.download() + cB.download_max()
+    for x, y, name in zip(a, b, ("cel", "xvel", "yvel", "hmax", "velmax")):
+        bad = int((x.view(np.int64) != y.view(np.int64)).sum())
+        assert bad == 0, f"child {name}: {bad} cells differ"
+    # the parent's wave came in through the child's edges
+    c, u, v = b[:3]
+    assert float(np.abs(u).max()) > 1e-4
+    # the child cannot run ahead of the parent's last record ...
+    with pytest.raises(gpu_lib.CliffsError):
+        nB.child_step(recs[-1][0] + dtc)
+    # ... unless the feed has ended: then its edges are 0 (cliffs_main.f:306-310)
+    nB.end_feed()
+    nB.child_step(recs[-1][0] + dtc)
+    for g in (nA, nB):
+        g.close()
+    for g in (gA, gB, cA, cB):
+        g.close()
+
+
+def test_ring_capacity_guard(gpu_lib):
+    case, maps, parent, child = _setup(gpu_lib)
+    g, c = parent(), child()
+    n = gpu_lib.Nest(g, maps)
+    n.couple(c, capacity=2)
+    t, iarv, k = 0.0, False, 0
+    with pytest.raises(gpu_lib.CliffsError):
+        for _ in range(40):           # the child never steps: the ring fills up
+            g.run(2)
+            t += 2 * case.dt
+            iarv, k = n.record(0.01, case.ground, t, iarv)
+    assert k == 2
+    n.close()
+    g.close()
+    c.close()
