@@ -9,5 +9,6 @@ cd "$HERE/csrc"
 $HIPCC $FLAGS -c sweeps.hip -o "$HERE/csrc/sweeps.o"
 $HIPCC $FLAGS -c step_fused.hip -o "$HERE/csrc/step_fused.o"
 $HIPCC $FLAGS -c nest.hip -o "$HERE/csrc/nest.o"
+$HIPCC $FLAGS -c surrogate.hip -o "$HERE/csrc/surrogate.o"
 $HIPCC $FLAGS -c capi.cpp -o "$HERE/csrc/capi.o"
-$HIPCC $FLAGS -shared -o "$HERE/libcliffs_gpu.so" "$HERE/csrc/sweeps.o" "$HERE/csrc/step_fused.o" "$HERE/csrc/nest.o" "$HERE/csrc/capi.o"
+$HIPCC $FLAGS -shared -o "$HERE/libcliffs_gpu.so" "$HERE/csrc/sweeps.o" "$HERE/csrc/step_fused.o" "$HERE/csrc/nest.o" "$HERE/csrc/surrogate.o" "$HERE/csrc/capi.o"

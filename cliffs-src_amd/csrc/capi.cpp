@@ -21,6 +21,8 @@ hipError_t launch_line_flags(const DevGrid& g, const Params& p, unsigned char* f
                              unsigned* fclr, int nclr, hipStream_t st);
 hipError_t launch_maxwave(const DevGrid& g, hipStream_t st);
 hipError_t launch_copy_lines(const DevGrid& g, int dir, int first, int last, hipStream_t st);
+hipError_t launch_gauge_sample(const DevGrid& g, const Params& p, const int* I, const int* J, int n,
+                               float* series, int slot, hipStream_t st);
 hipError_t launch_nest_arrival(const NestArgs& a, hipStream_t st);
 hipError_t launch_nest_feed(const NestArgs& a, hipStream_t st);
 hipError_t launch_child_edges(const double* r0, const double* r1, double cc, double omc, double grav,
@@ -839,6 +841,42 @@ int cliffs_run_timed(cliffs_handle* h, int nsteps, double* ms_total, double* ms_
         if ((rc = harvest(s % slots))) return rc;
     *ms_x = h->xrun ? sx / nsteps : 0.0;
     *ms_y = h->yrun ? sy / nsteps : 0.0;
+    return CLIFFS_OK;
+}
+
+int cliffs_run_gauges(cliffs_handle* h, int every, int nsamples, int n, const int* I, const int* J,
+                      float* eta) {
+    if (!h || every < 1 || nsamples < 1 || n < 1 || !I || !J || !eta)
+        return fail(CLIFFS_ERR_ARG, "bad gauge-series arguments");
+    if (!h->state) return fail(CLIFFS_ERR_STATE, "no state uploaded");
+    for (int q = 0; q < n; q++)
+        if (I[q] < 1 || I[q] > h->nx || J[q] < 1 || J[q] > h->ny)
+            return fail(CLIFFS_ERR_ARG, "gauge index out of grid");
+    HIPCHK(hipSetDevice(h->dev));
+    int* dij = nullptr;
+    float* ds = nullptr;
+    auto cleanup = [&]() {
+        (void)hipFree(dij);
+        (void)hipFree(ds);
+    };
+    if (hipMalloc(&dij, sizeof(int) * 2 * n) != hipSuccess ||
+        hipMalloc(&ds, sizeof(float) * (size_t)n * nsamples) != hipSuccess) {
+        cleanup();
+        return fail(CLIFFS_ERR_NOMEM, "gauge series allocation");
+    }
+    int rc = CLIFFS_OK;
+    hipError_t e = hipMemcpyAsync(dij, I, sizeof(int) * n, hipMemcpyHostToDevice, h->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dij + n, J, sizeof(int) * n, hipMemcpyHostToDevice, h->st);
+    for (int k = 0; k < nsamples && e == hipSuccess && rc == CLIFFS_OK; k++) {
+        for (int s = 0; s < every && rc == CLIFFS_OK; s++) rc = do_step(h, nullptr, nullptr);
+        if (rc == CLIFFS_OK) e = launch_gauge_sample(h->grid(), h->p, dij, dij + n, n, ds, k, h->st);
+    }
+    if (e == hipSuccess && rc == CLIFFS_OK)
+        e = hipMemcpyAsync(eta, ds, sizeof(float) * (size_t)n * nsamples, hipMemcpyDeviceToHost, h->st);
+    if (e == hipSuccess && rc == CLIFFS_OK) e = hipStreamSynchronize(h->st);
+    cleanup();
+    if (rc) return rc;
+    if (e != hipSuccess) return hipfail(e, "gauge series");
     return CLIFFS_OK;
 }
 

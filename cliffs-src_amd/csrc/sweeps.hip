@@ -254,8 +254,26 @@ __global__ void maxwave_kernel(DevGrid g) {
     g.VM[o] = fmaxf(g.VM[o], u * u + v * v);
 }
 
+// surface elevation at gauges (1-based I, J) into row `slot` of series:
+// eta = g*eta / g on wet cells (output_gages.f:18-29), 0 on dry ones
+__global__ void gauge_sample_kernel(DevGrid g, Params p, const int* I, const int* J, int n, float* series,
+                                    int slot) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const size_t o = (size_t)(J[q] - 1) * g.pitch + (I[q] - 1);
+    const float s = g.S[o], cp = g.CP[o];
+    series[(size_t)slot * n + q] = is_land(c0_of(s) + cp, p) ? 0.0f : geta_of(s, cp) / p.grav;
+}
+
 // ---------------------------------------------------------------------------
 // launchers
+hipError_t launch_gauge_sample(const DevGrid& g, const Params& p, const int* I, const int* J, int n,
+                               float* series, int slot, hipStream_t st) {
+    hipLaunchKernelGGL(gauge_sample_kernel, dim3((n + 255) / 256), dim3(256), 0, st, g, p, I, J, n, series,
+                       slot);
+    return hipGetLastError();
+}
+
 hipError_t launch_sweep_x(const XArgs& a, hipStream_t st) {
     const int rows = a.row_hi - a.row_lo + 1;
     const int chunks = (rows + kXRows - 1) / kXRows;

@@ -109,6 +109,13 @@ int cliffs_run(cliffs_handle *h, int nsteps);
 int cliffs_run_timed(cliffs_handle *h, int nsteps, double *ms_total, double *ms_xsweep,
                      double *ms_ysweep);
 
+/* nsamples x `every` steps with open edges, recording the surface elevation
+ * at n gauges (1-based I, J; eta = c^2/g - d on wet cells, 0 on dry ones)
+ * after every `every` steps: eta[k*n + q] for sample k, gauge q (FP32).
+ * Recorded on the device; one copy to the host at the end. */
+int cliffs_run_gauges(cliffs_handle *h, int every, int nsamples, int n, const int *I, const int *J,
+                      float *eta);
+
 /* 2-D steps (cliffs_step, cliffs_run) run as one fused kernel per step
  * (on = 1, the default) or as an x-sweep kernel followed by a y-sweep kernel
  * (on = 0).  Results are identical; the fused step moves 44 instead of 72
