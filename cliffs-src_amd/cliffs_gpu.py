@@ -77,6 +77,8 @@ def load_library():
         "cliffs_ensemble_run_timed": ([H, C.c_int, _dp], C.c_int),
         "cliffs_ensemble_tune": ([H, C.c_int, _ip], C.c_int),
         "cliffs_ensemble_sync": ([H], C.c_int),
+        "cliffs_ensemble_run_gauges": ([H, C.c_int, C.c_int, C.c_int, _ip, _ip, C.POINTER(C.c_float)],
+                                       C.c_int),
         "cliffs_sync": ([H], C.c_int),
         "cliffs_stream": ([H, C.POINTER(C.c_void_p)], C.c_int),
         "cliffs_info": ([H, _dp, _dp, _ip, _ip, _ip, _ip], C.c_int),
@@ -294,6 +296,19 @@ class Ensemble:
 
     def sync(self):
         _chk(load_library().cliffs_ensemble_sync(self._h))
+
+    def run_gauges(self, every, nsamples, I, J):
+        """nsamples x every steps (every even); -> (members, nsamples, n) FP32
+        surface elevation at each member's gauges (1-based I, J of shape (n,)
+        for all members, or (members, n)) after every `every` steps."""
+        m = len(self.grids)
+        I = np.ascontiguousarray(np.broadcast_to(np.asarray(I, dtype=np.int32), (m, np.shape(I)[-1])))
+        J = np.ascontiguousarray(np.broadcast_to(np.asarray(J, dtype=np.int32), I.shape))
+        out = np.empty((m, int(nsamples), I.shape[1]), dtype=np.float32)
+        _chk(load_library().cliffs_ensemble_run_gauges(self._h, int(every), int(nsamples), I.shape[1],
+                                                       I.ctypes.data_as(_ip), J.ctypes.data_as(_ip),
+                                                       out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
 
     def close(self):
         if getattr(self, "_h", None):

@@ -34,6 +34,8 @@ hipError_t launch_fused_flags(const FArgs& a, unsigned char* ctr0, unsigned char
 hipError_t fused_blocks_per_cu(int* n);
 hipError_t launch_step_batch(const FArgs* args, const int2* bmap, int nblocks, hipStream_t st);
 hipError_t launch_fixup_batch(const FArgs* args, const int2* bmap, int nblocks, hipStream_t st);
+hipError_t launch_ensemble_gauges(const FArgs* args, int nm, const int* ij, int n, int nsamples, int slot,
+                                  float* series, hipStream_t st);
 void ensemble_block_map(const int* ntiles, int n, std::vector<int2>& bmap);
 void fused_geometry(int nx, int ny, int cus, int blocks_per_cu, int* nstrips, int* nchunks,
                     int* rows);
@@ -1133,6 +1135,53 @@ int cliffs_ensemble_tune(cliffs_ensemble* e, int trial_steps, int* rows_out) {
     for (cliffs_handle* h : e->m) h->fused_valid = false;
     if ((rc = ensemble_give(e))) return rc;
     if (rows_out) *rows_out = best;
+    return CLIFFS_OK;
+}
+
+int cliffs_ensemble_run_gauges(cliffs_ensemble* e, int every, int nsamples, int n, const int* I,
+                               const int* J, float* eta) {
+    if (!e || every < 2 || (every & 1) || nsamples < 1 || n < 1 || !I || !J || !eta)
+        return fail(CLIFFS_ERR_ARG, "bad ensemble gauge-series arguments (every must be even)");
+    const int nm = (int)e->m.size();
+    for (int m = 0; m < nm; m++)
+        for (int q = 0; q < n; q++) {
+            const int i = I[(size_t)m * n + q], j = J[(size_t)m * n + q];
+            if (i < 1 || i > e->m[m]->nx || j < 1 || j > e->m[m]->ny)
+                return fail(CLIFFS_ERR_ARG, "gauge index out of a member's grid");
+        }
+    HIPCHK(hipSetDevice(e->dev));
+    int* dij = nullptr;
+    float* ds = nullptr;
+    const size_t nij = (size_t)nm * n, nser = (size_t)nm * nsamples * n;
+    if (hipMalloc(&dij, sizeof(int) * 2 * nij) != hipSuccess ||
+        hipMalloc(&ds, sizeof(float) * nser) != hipSuccess) {
+        (void)hipFree(dij);
+        return fail(CLIFFS_ERR_NOMEM, "ensemble gauge series allocation");
+    }
+    int rc = CLIFFS_OK;
+    hipError_t err = hipSuccess;
+    if ((rc = ensemble_take(e)) == CLIFFS_OK && (rc = ensemble_args(e)) == CLIFFS_OK) {
+        err = hipMemcpyAsync(dij, I, sizeof(int) * nij, hipMemcpyHostToDevice, e->st);
+        if (err == hipSuccess) err = hipMemcpyAsync(dij + nij, J, sizeof(int) * nij, hipMemcpyHostToDevice, e->st);
+        // `every` is even: after each group of steps the state is again the
+        // copy the slot-0 argument blocks read
+        for (int k = 0; k < nsamples && err == hipSuccess; k++) {
+            for (int s = 0; s < every / 2 && err == hipSuccess; s++) err = hipGraphLaunch(e->gexec, e->st);
+            if (err == hipSuccess) err = launch_ensemble_gauges(e->dargs, nm, dij, n, nsamples, k, ds, e->st);
+        }
+        for (cliffs_handle* h : e->m) {
+            h->fn += every * nsamples;
+            h->flags_valid = false;
+        }
+        if (err == hipSuccess) err = hipMemcpyAsync(eta, ds, sizeof(float) * nser, hipMemcpyDeviceToHost, e->st);
+        if (err == hipSuccess) err = hipStreamSynchronize(e->st);
+        if (err == hipSuccess) rc = ensemble_give(e);
+    }
+    (void)hipStreamSynchronize(e->st);
+    (void)hipFree(dij);
+    (void)hipFree(ds);
+    if (rc) return rc;
+    if (err != hipSuccess) return hipfail(err, "ensemble gauge series");
     return CLIFFS_OK;
 }
 

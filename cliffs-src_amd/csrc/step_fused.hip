@@ -341,9 +341,32 @@ __global__ void fused_flags_kernel(const FArgs a, unsigned char* ctr0, unsigned 
     }
 }
 
+// Surface elevation at every member's gauges (output_gages.f:18-29: eta on
+// wet cells, 0 on dry ones) from the state the argument blocks `args` read:
+// member m = blockIdx.y, gauge q; 1-based I = ij[m*n + q], J = ij[(nm + m)*n + q];
+// series[(m * nsamples + slot) * n + q].
+__global__ void ensemble_gauge_kernel(const FArgs* __restrict__ args, int nm, const int* __restrict__ ij,
+                                      int n, int nsamples, int slot, float* __restrict__ series) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blockIdx.y;
+    if (q >= n) return;
+    const FArgs& a = args[m];
+    const int i = ij[(size_t)m * n + q] - 1, j = ij[((size_t)nm + m) * n + q] - 1;
+    const size_t o = (size_t)j * a.g.pitch + i;
+    const float s = a.g.S[o], cp = a.g.CP[o];
+    series[((size_t)m * nsamples + slot) * n + q] = is_land(c0_of(s) + cp, a.p) ? 0.0f : geta_of(s, cp) / a.p.grav;
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 static int fused_blocks(const FArgs& a) { return (a.ntiles + kXCDs - 1) / kXCDs * kXCDs; }
+
+hipError_t launch_ensemble_gauges(const FArgs* args, int nm, const int* ij, int n, int nsamples, int slot,
+                                  float* series, hipStream_t st) {
+    hipLaunchKernelGGL(ensemble_gauge_kernel, dim3((n + 63) / 64, nm), dim3(64), 0, st, args, nm, ij, n,
+                       nsamples, slot, series);
+    return hipGetLastError();
+}
 
 hipError_t launch_step_fused(const FArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(step_fused_kernel, dim3(fused_blocks(a)), dim3(256), 0, st, a);

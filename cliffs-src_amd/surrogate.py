@@ -2,11 +2,13 @@
 
 The network (hand-written BF16 MFMA kernels, csrc/surrogate.hip) maps the
 surface-elevation series of 64 gauges on a nested grid's offshore boundary
-(256 samples) to the grid's 64 x 64 max-inundation map.  This module
-generates training data with the solver (cliffs_gpu), permutes maps to and
-from the network's quadtree order, trains, and gates predictions: two
-independently trained networks must agree, otherwise the scenario is solved
-with the full GPU solve (the fallback and the judge of the surrogate).
+(256 samples) to the nested grid's 64 x 64 max-inundation map.  This module
+defines the nested site, generates data with the solver (parents stepped as
+an ensemble with device gauge recording for the inputs, the coupled
+parent/child nested solve for the targets), permutes maps to and from the
+network's quadtree order, trains, and gates predictions: two independently
+trained networks must agree, otherwise the scenario is solved with the full
+nested solve (the fallback and the judge of the surrogate).
 """
 from __future__ import annotations
 
@@ -169,61 +171,177 @@ def gemm_bf16(A, B, device=0):
     return out
 
 
-# -- training data from the solver ---------------------------------------------
+# -- the nested site and its scenarios ------------------------------------------
+#
+# One site: a parent grid (192 x 192, 900 m, shelf and coastline of
+# synth.shelf_coast with terrain seed SITE_SEED) and a child grid at 5x
+# refinement (180 m, dt/5) over the coastal strip, fed one-way by the parent
+# (cliffs_nest_*).  Scenarios differ only in the source.  The network's input
+# is the parent's surface elevation at 64 gauges along the child's offshore
+# (west) edge, every EVERY parent steps; its output is the child's
+# max-inundation map over the coastal window.
 
-X_SCALE = 0.5     # gauge elevation (m) -> network input
-Y_SCALE = 0.25    # inundation depth (m) -> network output
+SITE_SEED = 5
+PNX = PNY = 192
+PDX = 900.0
+REFINE = 5
+CI0, CI1 = 124, 164          # child spans parent columns CI0..CI1 (0-based)
+CJ0, CJ1 = 32, 161           # and parent rows CJ0..CJ1
+EVERY = 4                    # parent steps between gauge samples and feed records
+CUKE = 1.0e-3                # arrival threshold of the feed (m)
+MAP_I0, MAP_W = 56, 128      # map window: child columns MAP_I0.. (2 per pixel),
+MAP_J0, MAP_H = 3, 640        # rows MAP_J0.. (10 per pixel), clear of the child's edges
+
+X_SCALE = 2.0     # gauge elevation (m) -> network input
+Y_SCALE = 0.125   # max surface elevation (m) -> network output
+
+_site = None
 
 
-def scenario(seed, nx=320, ny=256, dx=150.0):
-    """One nested-grid scenario: shelf and coastline (synth.shelf_coast),
-    dipole source of length 8-30 km and amplitude 0.5-4 m at a seeded place
-    and strike."""
+class Site:
+    """Terrain, child grid and nest maps of the site (built once)."""
+
+    def __init__(self):
+        base = synth.shelf_coast(PNX, PNY, SITE_SEED, PDX, 4.0, 10e3)
+        self.dt = base.dt                           # CFL step for the largest source amplitude
+        self.dep, self.ground = base.dep, base.ground
+        self.x, self.y = base.xcrd, base.ycrd
+        self.xc = np.linspace(self.x[CI0], self.x[CI1], REFINE * (CI1 - CI0) + 1)
+        self.yc = np.linspace(self.y[CJ0], self.y[CJ1], REFINE * (CJ1 - CJ0) + 1)
+        self.depc = _bilinear(self.dep, self.x, self.y, self.xc, self.yc)
+        self.cmetrics = synth.metrics(self.xc, self.yc, True)
+        self.maps = cliffs_gpu.nest_maps(self.x, self.y, self.xc, self.yc)
+        self.Xc, self.Yc = np.meshgrid(self.xc, self.yc)
+        # gauges: parent cells on the child's west edge (1-based)
+        self.gI = np.full(G, CI0 + 1, dtype=np.int32)
+        self.gJ = np.round(np.linspace(CJ0 + 1, CJ1 + 1, G)).astype(np.int32)
+        # land ground heights per map pixel, sorted, -inf for sea cells
+        w = _window(np.where(self.depc < 0, -self.depc, -np.inf))
+        self.pix_ground = np.sort(w.reshape(MAP, MAP_H // MAP, MAP, MAP_W // MAP)
+                                  .transpose(0, 2, 1, 3).reshape(NOUT, -1), axis=1)
+
+
+def site():
+    global _site
+    if _site is None:
+        _site = Site()
+    return _site
+
+
+def _bilinear(f, x, y, xq, yq):
+    """f (ny, nx) on the grid (y, x) at the points yq x xq."""
+    fi = (np.interp(xq, x, np.arange(len(x))))
+    fj = (np.interp(yq, y, np.arange(len(y))))
+    i0 = np.minimum(fi.astype(int), len(x) - 2)
+    j0 = np.minimum(fj.astype(int), len(y) - 2)
+    wx, wy = (fi - i0)[None, :], (fj - j0)[:, None]
+    a = f[j0][:, i0]
+    b = f[j0][:, i0 + 1]
+    c = f[j0 + 1][:, i0]
+    d = f[j0 + 1][:, i0 + 1]
+    return (1 - wy) * ((1 - wx) * a + wx * b) + wy * ((1 - wx) * c + wx * d)
+
+
+def _window(a):
+    return a[MAP_J0:MAP_J0 + MAP_H, MAP_I0:MAP_I0 + MAP_W]
+
+
+def scenario(seed):
+    """The site with one source: amplitude 0.5-4 m, length 8-30 km, seeded
+    strike and place (x in 0.3-0.5 of the parent's width)."""
+    S = site()
     rng = np.random.default_rng(50_000 + seed)
-    return synth.shelf_coast(nx, ny, seed, dx, float(rng.uniform(0.5, 4.0)),
-                             float(rng.uniform(8e3, 30e3)), steps=1536)
+    return synth.shelf_coast(PNX, PNY, SITE_SEED, PDX, float(rng.uniform(0.5, 4.0)),
+                             float(rng.uniform(8e3, 30e3)), steps=EVERY * T,
+                             source_seed=60_000 + seed, source_x=(0.3, 0.5), dt=S.dt)
 
 
-def gauge_line(case):
-    """64 gauges along the offshore line x = 0.58 Lx (1-based indices)."""
-    ny, nx = case.dep.shape
-    I = np.full(G, int(0.58 * nx) + 1, dtype=np.int32)
-    J = np.round(np.linspace(1, ny, G)).astype(np.int32)
-    return I, J
-
-
-def inundation_map(hmax, dep, celmin, cols=128):
-    """Max water depth on flooded land (H = hmax^2 / g where hmax > celmin,
-    dep < 0) over the last `cols` columns, block-averaged to 64 x 64."""
-    ny, nx = dep.shape
-    h = np.where((hmax > celmin) & (dep < 0.0), hmax * hmax / synth.GRAV, 0.0)[:, nx - cols:]
-    return h.reshape(MAP, ny // MAP, MAP, cols // MAP).mean(axis=(1, 3))
-
-
-def solve(case, every=6, device=0):
-    """The full GPU solve of one scenario: -> (gauge series (256, 64) in m,
-    inundation map (64, 64) in m, max run-up in m)."""
+def _parent(case, device):
     s1, s2, zeta = case.metrics()
     g = cliffs_gpu.Grid(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground, device=device)
     g.upload(case.cel, case.xvel, case.yvel)
-    I, J = gauge_line(case)
-    series = g.run_gauges(every, T, I, J)
-    hmax, _ = g.download_max()
-    cm = g.celmin
+    return g
+
+
+def inundation_map(hmax, depc, celmin):
+    """Max water-surface elevation (MaxE = hmax^2/g - dep, output_maxwave.f)
+    where the cell was ever wet, 0 elsewhere; block maximum over 10 x 2 child
+    cells of the coastal window -> 64 x 64."""
+    e = np.where(hmax > celmin, hmax * hmax / synth.GRAV - depc, 0.0)
+    e = np.maximum(_window(e), 0.0)
+    return e.reshape(MAP, MAP_H // MAP, MAP, MAP_W // MAP).max(axis=(1, 3))
+
+
+def runup_from_map(maps):
+    """Run-up read from max-surface maps: per pixel, the highest land cell
+    below the pixel's max surface; the maximum over pixels (0: none)."""
+    S = site()
+    m = np.asarray(maps, dtype=np.float64).reshape(-1, NOUT)
+    g = S.pix_ground[None]
+    reach = np.where(g < m[:, :, None], g, -np.inf).max(axis=(1, 2))
+    return np.maximum(reach, 0.0)
+
+
+def nested_solve(case, device=0):
+    """The full nested solve of one scenario: parent and child on the device
+    (cliffs_nest_couple/record/child_step), EVERY * T parent steps.
+    -> (child max-surface map (64, 64), exact max run-up over the map
+    window (m))."""
+    S = site()
+    g = _parent(case, device)
+    c = cliffs_gpu.Grid(S.depc, *S.cmetrics, dt=case.dt / REFINE, ground=case.ground, device=device)
+    src = case.meta["source"]
+    z = np.zeros_like(S.depc)
+    c.upload(synth.depth_to_celerity(S.depc + synth.dipole(S.Xc, S.Yc, S.depc, src), case.ground), z, z)
+    n = cliffs_gpu.Nest(g, S.maps)
+    n.couple(c, capacity=4)
+    dtc, per = case.dt / REFINE, REFINE * EVERY      # child steps per record
+    trec, iarv, kc = [], False, 0
+    for k in range(T):
+        g.run(EVERY)
+        tp = (k + 1) * EVERY * case.dt
+        iarv, nrec = n.record(CUKE, case.ground, tp, iarv)
+        if nrec > len(trec):
+            trec.append(tp)
+        while kc + 1 < per * (len(trec) - 1):
+            kc += 1
+            j = kc // per
+            n.child_step(trec[j] + (kc - j * per) * dtc)
+    hmax, _ = c.download_max()
+    cm = c.celmin
+    n.close()
+    c.close()
     g.close()
-    return series, inundation_map(hmax, case.dep, cm), synth.max_runup(hmax, case.dep, cm)
+    return inundation_map(hmax, S.depc, cm), synth.max_runup(_window(hmax), _window(S.depc), cm)
 
 
-def dataset(seeds, device=0):
-    """Network inputs (N, 256, 64), quadtree targets (N, 4096), maps and run-ups."""
+def parent_gauges(cases, device=0):
+    """Parents of all `cases` stepped together (cliffs_ensemble_*), the
+    gauge series recorded on the device: -> (n, T, G) in m."""
+    S = site()
+    grids = [_parent(c, device) for c in cases]
+    ens = cliffs_gpu.Ensemble(grids)
+    out = ens.run_gauges(EVERY, T, S.gI, S.gJ)
+    ens.close()
+    for g in grids:
+        g.close()
+    return out
+
+
+def dataset(seeds, batch=64, device=0):
+    """Network inputs (N, T, G), quadtree targets (N, 4096), maps and exact
+    run-ups, for the scenarios `seeds`."""
     xs, maps, runups = [], [], []
-    for s in seeds:
-        series, m, r = solve(scenario(int(s)), device=device)
-        xs.append(series * X_SCALE)
-        maps.append(m)
-        runups.append(r)
+    seeds = list(seeds)
+    for b0 in range(0, len(seeds), batch):
+        cases = [scenario(int(s)) for s in seeds[b0:b0 + batch]]
+        xs.append(parent_gauges(cases, device) * X_SCALE)
+        for c in cases:
+            m, r = nested_solve(c, device)
+            maps.append(m)
+            runups.append(r)
     maps = np.array(maps, dtype=np.float32)
-    return (np.array(xs, dtype=np.float32), to_quadtree(maps * Y_SCALE), maps,
+    return (np.concatenate(xs).astype(np.float32), to_quadtree(maps * Y_SCALE), maps,
             np.array(runups))
 
 
@@ -232,7 +350,10 @@ def predict_maps(net, x):
 
 
 def flooded_area(maps, thr=0.05):
-    return (np.asarray(maps) > thr).sum(axis=(-2, -1))
+    """Pixels whose max surface stands above their lowest land cell."""
+    S = site()
+    gmin = S.pix_ground[:, 0].reshape(MAP, MAP)
+    return ((np.asarray(maps) > thr) & (np.asarray(maps) > gmin)).sum(axis=(-2, -1))
 
 
 def gate(maps_a, maps_b, tol):

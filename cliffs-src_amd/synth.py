@@ -219,12 +219,16 @@ def ssl_condition(dep, depmin=1.0, alpha=2.0, passes=50):
     return d
 
 
-def shelf_coast(nx, ny, seed, dx, source_amp, source_len, steps=4000, cfl=0.7):
+def shelf_coast(nx, ny, seed, dx, source_amp, source_len, steps=4000, cfl=0.7, source_seed=None,
+                source_x=(0.35, 0.55), dt=None):
     """Cartesian shelf and coastline for ensemble scenarios: fractal abyssal
     plain 4-6 km, a continental slope and a shelf whose sqrt-depth varies
     linearly (200 m shelf, 2 m at the shore), land rising to +300 m behind a
     seeded coastline, SSL conditioning (ssl_condition); dipole source
-    (+A, -A/2) of the given length with seeded strike and location."""
+    (+A, -A/2) of the given length with seeded strike and location (x in
+    source_x of the domain width).  The terrain follows `seed`; the source
+    follows it too, or `source_seed` when given (one site, many sources).
+    `dt` overrides the CFL time step."""
     rng = np.random.default_rng(seed)
     x = np.arange(nx, dtype=np.float64) * dx
     y = np.arange(ny, dtype=np.float64) * dx
@@ -243,22 +247,30 @@ def shelf_coast(nx, ny, seed, dx, source_amp, source_len, steps=4000, cfl=0.7):
     land = dist <= 0
     dep = np.where(land, dist / (0.25 * Lx) * 300.0 - 0.5, dep)
     dep = ssl_condition(dep)
-    sx = rng.uniform(0.35, 0.55) * Lx
-    sy = rng.uniform(0.3, 0.7) * Ly
-    th = rng.uniform(0, np.pi)
-    a = min(0.5 * source_len, 0.2 * Lx)
-    b = 0.5 * a
-    xr = (X - sx) * np.cos(th) + (Y - sy) * np.sin(th)
-    yr = -(X - sx) * np.sin(th) + (Y - sy) * np.cos(th)
-    src = source_amp * np.exp(-((xr - 0.5 * a) / a) ** 2 - (yr / b) ** 2) \
-        - 0.5 * source_amp * np.exp(-((xr + 0.5 * a) / a) ** 2 - (yr / b) ** 2)
-    src = np.where(dep > 0, src, 0.0)
+    rs = rng if source_seed is None else np.random.default_rng(source_seed)
+    src = dict(amplitude=source_amp, sx=rs.uniform(*source_x) * Lx, sy=rs.uniform(0.3, 0.7) * Ly,
+               strike=rs.uniform(0, np.pi), a=min(0.5 * source_len, 0.2 * Lx))
     ground = 0.01
-    cel = depth_to_celerity(dep + src, ground)
-    dt = cfl * dx / math.sqrt(GRAV * (float(dep.max()) + 2 * source_amp))
+    cel = depth_to_celerity(dep + dipole(X, Y, dep, src), ground)
+    if dt is None:
+        dt = cfl * dx / math.sqrt(GRAV * (float(dep.max()) + 2 * source_amp))
     z = np.zeros_like(dep)
     return Case("shelf_coast", dep, cel, z.copy(), z.copy(), x, y, True, dt, ground, 0.0, steps,
-                dict(seed=seed, amplitude=source_amp, dx=dx, source_len=source_len))
+                dict(seed=seed, amplitude=source_amp, dx=dx, source_len=source_len, source=src))
+
+
+def dipole(X, Y, dep, src):
+    """Initial surface of shelf_coast's source at points (X, Y) over depth
+    dep: +A Gaussian ahead of the strike line, -A/2 behind, semi-axes a and
+    a/2, 0 on land."""
+    A, a = src["amplitude"], src["a"]
+    b = 0.5 * a
+    th = src["strike"]
+    xr = (X - src["sx"]) * np.cos(th) + (Y - src["sy"]) * np.sin(th)
+    yr = -(X - src["sx"]) * np.sin(th) + (Y - src["sy"]) * np.cos(th)
+    eta = A * np.exp(-((xr - 0.5 * a) / a) ** 2 - (yr / b) ** 2) \
+        - 0.5 * A * np.exp(-((xr + 0.5 * a) / a) ** 2 - (yr / b) ** 2)
+    return np.where(dep > 0, eta, 0.0)
 
 
 def ensemble_scenario(seed, steps=4000):

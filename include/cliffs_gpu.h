@@ -147,6 +147,13 @@ int cliffs_ensemble_run_timed(cliffs_ensemble *e, int nsteps, double *ms_total);
  * maxima (their max-wave update is switched off). */
 int cliffs_ensemble_tune(cliffs_ensemble *e, int trial_steps, int *rows_out);
 int cliffs_ensemble_sync(cliffs_ensemble *e);
+/* nsamples x `every` steps of every member (`every` even); after each
+ * `every` steps the surface elevation at each member's n gauges (1-based
+ * I[m*n + q], J[m*n + q]; eta on wet cells, 0 on dry, as output_gages.f:18-29
+ * writes it) goes to eta[(m*nsamples + k)*n + q].  One gauge launch per
+ * sample for all members; the series stay on the device until the end. */
+int cliffs_ensemble_run_gauges(cliffs_ensemble *e, int every, int nsamples, int n, const int *I,
+                               const int *J, float *eta);
 
 /* Gauge sampling (output_gages.f:18-29): eta = c^2/g - d, or -1e32 (the
  * reference's real*4 NAN) where c <= celmin; u = xvel, v = yvel.  I, J are

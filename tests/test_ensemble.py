@@ -102,3 +102,28 @@ def test_ensemble_rejects_bad_members(gpu_lib):
         gpu_lib.Ensemble([g2, g2])
     g1.close()
     g2.close()
+
+
+def test_ensemble_gauges_match_members_alone(gpu_lib):
+    """cliffs_ensemble_run_gauges: each member's series equals the same grid's
+    cliffs_run_gauges series, and the state after it is the same."""
+    cases = _cases()[:4]
+    every, ns = 4, 9
+    rng = np.random.default_rng(7)
+    I = np.stack([rng.integers(1, c.dep.shape[1] + 1, 12) for c in cases]).astype(np.int32)
+    J = np.stack([rng.integers(1, c.dep.shape[0] + 1, 12) for c in cases]).astype(np.int32)
+    want_s, want = [], []
+    for k, case in enumerate(cases):
+        g = _grid(gpu_lib, case)
+        want_s.append(g.run_gauges(every, ns, I[k], J[k]))
+        want.append(_state(g))
+        g.close()
+    grids = [_grid(gpu_lib, c) for c in cases]
+    ens = gpu_lib.Ensemble(grids)
+    got = ens.run_gauges(every, ns, I, J)
+    ens.close()
+    for k, g in enumerate(grids):
+        assert np.array_equal(got[k].view(np.int32), want_s[k].view(np.int32)), f"member {k} series"
+        _same(_state(g), want[k], k)
+        g.close()
+    assert any(np.abs(s).max() > 0 for s in want_s)
