@@ -68,6 +68,7 @@ def load_library():
         "cliffs_run": ([H, C.c_int], C.c_int),
         "cliffs_run_timed": ([H, C.c_int, _dp, _dp, _dp], C.c_int),
         "cliffs_set_fused": ([H, C.c_int], C.c_int),
+        "cliffs_set_tile_rows": ([H, C.c_int], C.c_int),
         "cliffs_run_gauges": ([H, C.c_int, C.c_int, C.c_int, _ip, _ip, C.POINTER(C.c_float)],
                               C.c_int),
         "cliffs_gauges": ([H, C.c_int, _ip, _ip, _dp, _dp, _dp], C.c_int),
@@ -229,6 +230,10 @@ class Grid:
     def set_fused(self, on: bool):
         """2-D steps as one fused kernel (default) or as two sweep kernels."""
         _chk(load_library().cliffs_set_fused(self._h, 1 if on else 0))
+
+    def set_tile_rows(self, rows: int):
+        """Tile height of the fused step (8..1024, 0 = automatic)."""
+        _chk(load_library().cliffs_set_tile_rows(self._h, int(rows)))
 
     def run_timed(self, nsteps: int):
         """-> (ms_total, ms_a, ms_b), HIP events on the handle's stream: mean

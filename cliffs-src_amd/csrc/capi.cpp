@@ -888,6 +888,14 @@ int cliffs_set_fused(cliffs_handle* h, int on) {
     return CLIFFS_OK;
 }
 
+int cliffs_set_tile_rows(cliffs_handle* h, int rows) {
+    if (!h) return fail(CLIFFS_ERR_ARG, "NULL handle");
+    if (rows != 0 && (rows < 8 || rows > kFMaxRows)) return fail(CLIFFS_ERR_ARG, "tile rows must be 0 or 8..1024");
+    if (!h->fused_ok) return fail(CLIFFS_ERR_ARG, "grid has no fused step (1-D or too large)");
+    HIPCHK(hipSetDevice(h->dev));
+    return set_fused_rows(h, rows);
+}
+
 // ---- ensembles --------------------------------------------------------------
 }  // extern "C"
 
@@ -1105,7 +1113,7 @@ int cliffs_ensemble_run_timed(cliffs_ensemble* e, int nsteps, double* ms_total) 
 int cliffs_ensemble_tune(cliffs_ensemble* e, int trial_steps, int* rows_out) {
     if (!e || trial_steps < 1) return fail(CLIFFS_ERR_ARG, "bad tune arguments");
     HIPCHK(hipSetDevice(e->dev));
-    static const int kCand[] = {32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024};
+    static const int kCand[] = {16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024};
     EventSet es;
     es.ev.assign(2, nullptr);
     for (auto& x : es.ev) HIPCHK(hipEventCreate(&x));

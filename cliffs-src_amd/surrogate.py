@@ -212,6 +212,7 @@ class Site:
         self.cmetrics = synth.metrics(self.xc, self.yc, True)
         self.maps = cliffs_gpu.nest_maps(self.x, self.y, self.xc, self.yc)
         self.Xc, self.Yc = np.meshgrid(self.xc, self.yc)
+        self.rows = None                            # tuned tile heights (parent, child)
         # gauges: parent cells on the child's west edge (1-based)
         self.gI = np.full(G, CI0 + 1, dtype=np.int32)
         self.gJ = np.round(np.linspace(CJ0 + 1, CJ1 + 1, G)).astype(np.int32)
@@ -282,17 +283,29 @@ def runup_from_map(maps):
     return np.maximum(reach, 0.0)
 
 
-def nested_solve(case, device=0):
+def _tuned_rows(grid):
+    """Fastest fused-step tile height for this grid (cliffs_ensemble_tune on
+    an ensemble of one; the state is not advanced)."""
+    e = cliffs_gpu.Ensemble([grid])
+    rows = e.tune(3)
+    e.close()
+    return rows
+
+
+def nested_hmax(case, device=0):
     """The full nested solve of one scenario: parent and child on the device
-    (cliffs_nest_couple/record/child_step), EVERY * T parent steps.
-    -> (child max-surface map (64, 64), exact max run-up over the map
-    window (m))."""
+    (cliffs_nest_couple/record/child_step), EVERY * T parent steps, tile
+    heights tuned once per site.  -> (child hmax, child celmin)."""
     S = site()
     g = _parent(case, device)
     c = cliffs_gpu.Grid(S.depc, *S.cmetrics, dt=case.dt / REFINE, ground=case.ground, device=device)
     src = case.meta["source"]
     z = np.zeros_like(S.depc)
     c.upload(synth.depth_to_celerity(S.depc + synth.dipole(S.Xc, S.Yc, S.depc, src), case.ground), z, z)
+    if S.rows is None:
+        S.rows = (_tuned_rows(g), _tuned_rows(c))
+    g.set_tile_rows(S.rows[0])
+    c.set_tile_rows(S.rows[1])
     n = cliffs_gpu.Nest(g, S.maps)
     n.couple(c, capacity=4)
     dtc, per = case.dt / REFINE, REFINE * EVERY      # child steps per record
@@ -312,6 +325,14 @@ def nested_solve(case, device=0):
     n.close()
     c.close()
     g.close()
+    return hmax, cm
+
+
+def nested_solve(case, device=0):
+    """-> (child inundation map (64, 64), exact max run-up over the map
+    window (m)) of the full nested solve (nested_hmax)."""
+    S = site()
+    hmax, cm = nested_hmax(case, device)
     return inundation_map(hmax, S.depc, cm), synth.max_runup(_window(hmax), _window(S.depc), cm)
 
 
@@ -321,6 +342,7 @@ def parent_gauges(cases, device=0):
     S = site()
     grids = [_parent(c, device) for c in cases]
     ens = cliffs_gpu.Ensemble(grids)
+    ens.tune(3)
     out = ens.run_gauges(EVERY, T, S.gI, S.gJ)
     ens.close()
     for g in grids:

@@ -122,6 +122,12 @@ int cliffs_run_gauges(cliffs_handle *h, int every, int nsamples, int n, const in
  * bytes of HBM traffic per cell.  No counterpart in the reference, whose
  * step is always two sweeps (cliffs_main.f:317-345). */
 int cliffs_set_fused(cliffs_handle *h, int on);
+/* Tile height of the fused step: rows = 8..1024, or 0 for the automatic
+ * choice (enough tiles to fill the device once, at least 32 rows).  Results
+ * do not depend on it; small grids step faster with short tiles
+ * (cliffs_ensemble_tune picks one by timing).  No counterpart in the
+ * reference. */
+int cliffs_set_tile_rows(cliffs_handle *h, int rows);
 
 /* ---- ensembles: many grids stepped together ---------------------------- */
 
@@ -142,7 +148,7 @@ int cliffs_ensemble_run(cliffs_ensemble *e, int nsteps);
 /* As cliffs_ensemble_run; ms_total = device time of the nsteps. */
 int cliffs_ensemble_run_timed(cliffs_ensemble *e, int nsteps, double *ms_total);
 /* Autotuner: times trial_steps repetitions of one ensemble step for each
- * candidate tile height (32 .. 1024 rows of 248 columns) and keeps the
+ * candidate tile height (16 .. 1024 rows of 248 columns) and keeps the
  * fastest (rows_out).  The trials change neither the state nor the running
  * maxima (their max-wave update is switched off). */
 int cliffs_ensemble_tune(cliffs_ensemble *e, int trial_steps, int *rows_out);

@@ -58,7 +58,7 @@ def test_ensemble_matches_members_alone(gpu_lib, tune):
     ens = gpu_lib.Ensemble(grids)
     if tune:
         rows = ens.tune(2)
-        assert 32 <= rows <= 1024
+        assert 16 <= rows <= 1024
     ens.run(20)
     ens.run(steps - 20)              # odd remainder: one step outside the graph
     for g in grids:                  # hand-back: single-grid steps after the ensemble

@@ -163,3 +163,30 @@ def test_switching_paths(gpu_lib):
         out.append(_result(g))
         g.close()
     _same(*out)
+
+
+@pytest.mark.parametrize("rows", [8, 16, 24, 40])
+def test_short_tiles(gpu_lib, rows):
+    """cliffs_set_tile_rows: short tiles (many chunks, halos larger than the
+    tile) give the two-kernel step's results, on a coastal case and on a
+    plane beach whose columns flood during the x-sweep."""
+    for case, n in ((synth.coastal(nx=300, ny=260, seed=11), 40), (_plane_beach(nx=400, ny=100, seed=12), 60)):
+        want = _run(gpu_lib, case, False, n)
+        g = _grid(gpu_lib, case, True)
+        g.set_tile_rows(rows)
+        g.run(n // 2)
+        g.set_tile_rows(0)            # back to the automatic tiles mid-run
+        g.run(n - n // 2 - 5)
+        g.set_tile_rows(rows)
+        g.run(5)
+        got = _result(g)
+        g.close()
+        _same(got, want)
+
+
+def test_tile_rows_rejects_bad_values(gpu_lib):
+    g = _grid(gpu_lib, synth.coastal(nx=120, ny=100, seed=1), True)
+    for bad in (-1, 4, 7, 1025):
+        with pytest.raises(gpu_lib.CliffsError):
+            g.set_tile_rows(bad)
+    g.close()
