@@ -371,6 +371,20 @@ def predict_maps(net, x):
     return np.maximum(from_quadtree(net.forward(x)) / Y_SCALE, 0.0)
 
 
+def answer(cases, nets, tol, device=0):
+    """Max-inundation maps of `cases` at the site: the parents stepped as one
+    ensemble give the gauge series; where the two networks' maps agree
+    within tol (gate) their mean is the answer, elsewhere the full nested
+    solve runs.  -> (maps (n, 64, 64), answered by the surrogate (n,))."""
+    x = parent_gauges(cases, device) * X_SCALE
+    pa, pb = predict_maps(nets[0], x), predict_maps(nets[1], x)
+    ok = gate(pa, pb, tol)
+    maps = (0.5 * (pa + pb)).astype(np.float64)
+    for k in np.flatnonzero(~ok):
+        maps[k] = nested_solve(cases[k], device)[0]
+    return maps, ok
+
+
 def flooded_area(maps, thr=0.05):
     """Pixels whose max surface stands above their lowest land cell."""
     S = site()

@@ -139,3 +139,63 @@ def test_solver_data(gpu_lib):
     assert np.isfinite(x).all() and (maps >= 0).all() and (runups >= 0).all()
     assert np.abs(x).max() > 0.0
     assert np.array_equal(S.from_quadtree(yq) / S.Y_SCALE, maps)
+    # the ensemble's gauge series are the parent's own (cliffs_run_gauges)
+    st = S.site()
+    case = S.scenario(1)
+    s1, s2, zeta = case.metrics()
+    g = gpu_lib.Grid(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground)
+    g.upload(case.cel, case.xvel, case.yvel)
+    alone = g.run_gauges(S.EVERY, S.T, st.gI, st.gJ)
+    g.close()
+    assert np.array_equal(alone * np.float32(S.X_SCALE), x[1])
+
+
+@pytest.mark.gpu
+def test_trained_network_gain(gpu_lib):
+    """What the trained network gains over the method without it, on
+    scenarios it has not seen: its maps are closer to the full nested solve
+    than the training mean map (the best answer without a network), and
+    ensemble parents plus inference cost a small fraction of the nested
+    solve."""
+    import time
+    tr = S.dataset(range(64))
+    cases = [S.scenario(s) for s in range(300_000, 300_032)]
+    t0 = time.perf_counter()
+    x = S.parent_gauges(cases) * S.X_SCALE
+    t_par = (time.perf_counter() - t0) / len(cases)
+    t0 = time.perf_counter()
+    truth = [S.nested_solve(c) for c in cases]
+    t_nest = (time.perf_counter() - t0) / len(cases)
+    maps = np.array([m for m, _ in truth])
+    net = S.Surrogate(batch_max=64, seed=1)
+    losses = net.train(tr[0], tr[1], batch=32, steps=600, lr=5e-4, seed=2)
+    assert np.isfinite(losses).all()
+    t0 = time.perf_counter()
+    pred = S.predict_maps(net, x)
+    t_inf = (time.perf_counter() - t0) / len(cases)
+    net.close()
+    r_net = float(np.sqrt(((pred - maps) ** 2).mean()))
+    r_mean = float(np.sqrt(((tr[2].mean(axis=0)[None] - maps) ** 2).mean()))
+    print(f"map RMSE {r_net:.3f} m vs mean map {r_mean:.3f} m; per scenario: parents {t_par * 1e3:.1f} ms, "
+          f"inference {t_inf * 1e3:.2f} ms, nested solve {t_nest * 1e3:.0f} ms")
+    assert r_net < 0.95 * r_mean, (r_net, r_mean)
+    assert t_par + t_inf < 0.2 * t_nest, (t_par, t_inf, t_nest)
+
+
+@pytest.mark.gpu
+def test_gate_falls_back_to_nested_solve(gpu_lib):
+    """surrogate.answer: a scenario the gate rejects is answered by the full
+    nested solve, bit for bit; an accepted one by the networks' mean."""
+    cases = [S.scenario(s) for s in (400_000, 400_001)]
+    nets = [S.Surrogate(batch_max=8, seed=k) for k in (1, 2)]
+    maps, ok = S.answer(cases, nets, tol=0.0)          # untrained networks disagree
+    assert not ok.any()
+    for c, m in zip(cases, maps):
+        assert np.array_equal(m, S.nested_solve(c)[0])
+    maps, ok = S.answer(cases, nets, tol=np.inf)
+    assert ok.all()
+    x = S.parent_gauges(cases) * S.X_SCALE
+    want = 0.5 * (S.predict_maps(nets[0], x) + S.predict_maps(nets[1], x))
+    assert np.array_equal(maps, want)
+    for n in nets:
+        n.close()
