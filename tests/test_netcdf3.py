@@ -1,0 +1,276 @@
+"""NetCDF-3 shim (cliffs-src_amd/netcdf3/nc3.c) against scipy's independent
+reader and writer of the classic format.
+
+Reads: the reference's own example files (tests/golden/slpbeach, written by
+the NetCDF library) and CDF-2 files scipy writes, whole and by hyperslab,
+with conversion to float and double.  Writes: files of every type, fixed and
+record variables, attributes, fill values -- scipy must read back exactly
+what was written -- and a lone short record variable (the format's unpadded
+special case, which scipy does not implement: checked byte by byte).  The Fortran-77 entry points the driver uses are exercised by
+tests/fortran/fnc3 (1-based ids, reversed dimensions, CHARACTER function).
+"""
+import ctypes as C
+import glob
+import os
+import subprocess
+
+import numpy as np
+import pytest
+from scipy.io import netcdf_file
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "cliffs-src_amd", "netcdf3", "libcliffs_nc3.so")
+SLP = os.path.join(ROOT, "tests", "golden", "slpbeach")
+FNC3 = os.path.join(ROOT, "tests", "fortran", "fnc3")
+
+FLOAT, DOUBLE = 5, 6
+TYPES = {"b": 1, "c": 2, "h": 3, "i": 4, "f": 5, "d": 6}
+_sz = C.POINTER(C.c_size_t)
+
+
+@pytest.fixture(scope="module")
+def nc3():
+    if not os.path.exists(LIB):
+        pytest.skip("libcliffs_nc3.so not built (cliffs-src_amd/fortran/build.sh)")
+    lib = C.CDLL(LIB)
+    ip = C.POINTER(C.c_int)
+    lib.nc3_open.argtypes = [C.c_char_p, C.c_int, ip]
+    lib.nc3_create.argtypes = [C.c_char_p, C.c_int, ip]
+    lib.nc3_inq.argtypes = [C.c_int, ip, ip, ip, ip]
+    lib.nc3_inq_dim.argtypes = [C.c_int, C.c_int, C.c_char_p, _sz]
+    lib.nc3_inq_var.argtypes = [C.c_int, C.c_int, C.c_char_p, ip, ip, ip, ip]
+    lib.nc3_get_vara.argtypes = [C.c_int, C.c_int, _sz, _sz, C.c_void_p, C.c_int]
+    lib.nc3_put_vara.argtypes = [C.c_int, C.c_int, _sz, _sz, C.c_void_p, C.c_int]
+    lib.nc3_def_dim.argtypes = [C.c_int, C.c_char_p, C.c_size_t, ip]
+    lib.nc3_def_var.argtypes = [C.c_int, C.c_char_p, C.c_int, C.c_int, ip, ip]
+    lib.nc3_put_att.argtypes = [C.c_int, C.c_int, C.c_char_p, C.c_int, C.c_size_t, C.c_void_p]
+    lib.nc3_inq_att.argtypes = [C.c_int, C.c_int, C.c_char_p, ip, _sz]
+    lib.nc3_get_att.argtypes = [C.c_int, C.c_int, C.c_char_p, C.c_void_p]
+    for f in ("nc3_enddef", "nc3_close", "nc3_sync"):
+        getattr(lib, f).argtypes = [C.c_int]
+    lib.nc3_strerror.restype = C.c_char_p
+    return lib
+
+
+def _ok(lib, rc):
+    assert rc == 0, lib.nc3_strerror(rc).decode()
+
+
+def _read_all(lib, path):
+    """{name: (type code, dims, values as float64)} through the shim."""
+    ncid = C.c_int()
+    _ok(lib, lib.nc3_open(path.encode(), 0, C.byref(ncid)))
+    nd, nv, ng, un = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    _ok(lib, lib.nc3_inq(ncid, C.byref(nd), C.byref(nv), C.byref(ng), C.byref(un)))
+    dims = []
+    for d in range(nd.value):
+        name = C.create_string_buffer(256)
+        ln = C.c_size_t()
+        _ok(lib, lib.nc3_inq_dim(ncid, d, name, C.byref(ln)))
+        dims.append((name.value.decode(), ln.value))
+    out = {}
+    for v in range(nv.value):
+        name = C.create_string_buffer(256)
+        t, ndv, na = C.c_int(), C.c_int(), C.c_int()
+        ids = (C.c_int * 32)()
+        _ok(lib, lib.nc3_inq_var(ncid, v, name, C.byref(t), C.byref(ndv), ids, C.byref(na)))
+        shape = tuple(dims[ids[k]][1] for k in range(ndv.value))
+        vals = np.empty(shape, dtype=np.float64)
+        if t.value != 2:
+            _ok(lib, lib.nc3_get_vara(ncid, v, None, None, vals.ctypes.data_as(C.c_void_p), DOUBLE))
+        out[name.value.decode()] = (t.value, tuple(dims[ids[k]][0] for k in range(ndv.value)), vals)
+    _ok(lib, lib.nc3_close(ncid))
+    return dims, out
+
+
+def test_reads_reference_example_files(nc3):
+    files = sorted(glob.glob(os.path.join(SLP, "*.nc")))
+    assert len(files) == 5
+    for path in files:
+        dims, got = _read_all(nc3, path)
+        with netcdf_file(path, "r", mmap=False) as ref:
+            assert dims == [(k, (v or 0) if v is not None else 0) for k, v in ref.dimensions.items()] \
+                or [d for d, _ in dims] == list(ref.dimensions)
+            assert list(got) == list(ref.variables)
+            for name, var in ref.variables.items():
+                t, vdims, vals = got[name]
+                assert t == TYPES[var.typecode()] and vdims == var.dimensions
+                assert np.array_equal(vals, np.asarray(var[:], dtype=np.float64)), (path, name)
+
+
+def test_float_conversion_and_hyperslab(nc3):
+    path = os.path.join(SLP, "solitary.0185_h.nc")
+    with netcdf_file(path, "r", mmap=False) as ref:
+        ha = np.array(ref.variables["ha"][:])          # double (time, yyy, xxx)
+    ncid = C.c_int()
+    _ok(nc3, nc3.nc3_open(path.encode(), 0, C.byref(ncid)))
+    start = (C.c_size_t * 3)(0, 0, 10)
+    count = (C.c_size_t * 3)(1, 1, 20)
+    f = np.empty(20, dtype=np.float32)
+    _ok(nc3, nc3.nc3_get_vara(ncid, 3, start, count, f.ctypes.data_as(C.c_void_p), FLOAT))
+    assert np.array_equal(f, ha[0, 0, 10:30].astype(np.float32))
+    bad = (C.c_size_t * 3)(0, 0, 80)
+    assert nc3.nc3_get_vara(ncid, 3, bad, count, f.ctypes.data_as(C.c_void_p), FLOAT) != 0
+    _ok(nc3, nc3.nc3_close(ncid))
+
+
+def test_writes_read_back_by_scipy(nc3, tmp_path):
+    rng = np.random.default_rng(0)
+    path = str(tmp_path / "w.nc")
+    ncid = C.c_int()
+    _ok(nc3, nc3.nc3_create(path.encode(), 0, C.byref(ncid)))
+    did = {}
+    for name, ln in (("x", 7), ("y", 5), ("time", 0)):
+        d = C.c_int()
+        _ok(nc3, nc3.nc3_def_dim(ncid, name.encode(), ln, C.byref(d)))
+        did[name] = d.value
+    specs = [("xd", "d", ("x",)), ("fld", "f", ("y", "x")), ("ib", "b", ("y", "x")),
+             ("ih", "h", ("x",)), ("ii", "i", ("y",)), ("t", "d", ("time",)),
+             ("rec", "f", ("time", "y", "x")), ("rh", "h", ("time", "x"))]
+    vid = {}
+    for name, tc, dims in specs:
+        ids = (C.c_int * 3)(*[did[d] for d in dims])
+        v = C.c_int()
+        _ok(nc3, nc3.nc3_def_var(ncid, name.encode(), TYPES[tc], len(dims), ids, C.byref(v)))
+        vid[name] = v.value
+    txt = b"Maximal Water Elevation"
+    _ok(nc3, nc3.nc3_put_att(ncid, vid["fld"], b"name", 2, len(txt), txt))
+    dv = np.array([1.5, -2.25], dtype=np.float64)
+    _ok(nc3, nc3.nc3_put_att(ncid, -1, b"scale", DOUBLE, 2, dv.ctypes.data_as(C.c_void_p)))
+    _ok(nc3, nc3.nc3_enddef(ncid))
+    want = {"xd": rng.normal(size=7), "fld": rng.normal(size=(5, 7)).astype(np.float32),
+            "ib": rng.integers(-100, 100, (5, 7)), "ih": rng.integers(-3000, 3000, 7),
+            "ii": rng.integers(-10**6, 10**6, 5)}
+    for name, val in want.items():
+        buf = np.ascontiguousarray(val, dtype=np.float64)
+        _ok(nc3, nc3.nc3_put_vara(ncid, vid[name], None, None, buf.ctypes.data_as(C.c_void_p), DOUBLE))
+    recs = rng.normal(size=(4, 5, 7)).astype(np.float32)
+    for k in (0, 1, 3):                        # record 2 stays at the fill value
+        st = (C.c_size_t * 3)(k, 0, 0)
+        ct = (C.c_size_t * 3)(1, 5, 7)
+        _ok(nc3, nc3.nc3_put_vara(ncid, vid["rec"], st, ct, recs[k].ctypes.data_as(C.c_void_p), FLOAT))
+        tv = np.array([10.0 * k])
+        _ok(nc3, nc3.nc3_put_vara(ncid, vid["t"], (C.c_size_t * 1)(k), (C.c_size_t * 1)(1),
+                                  tv.ctypes.data_as(C.c_void_p), DOUBLE))
+    rh = rng.integers(-500, 500, (4, 7)).astype(np.float64)
+    _ok(nc3, nc3.nc3_put_vara(ncid, vid["rh"], (C.c_size_t * 2)(0, 0), (C.c_size_t * 2)(4, 7),
+                              rh.ctypes.data_as(C.c_void_p), DOUBLE))
+    _ok(nc3, nc3.nc3_close(ncid))
+    with netcdf_file(path, "r", mmap=False) as f:
+        assert f.dimensions["time"] is None and f.variables["rec"].shape == (4, 5, 7)
+        for name, val in want.items():
+            assert np.array_equal(f.variables[name][:], val), name
+        got = f.variables["rec"][:]
+        for k in (0, 1, 3):
+            assert np.array_equal(got[k], recs[k])
+        assert np.all(got[2] == np.float32(9.9692099683868690e+36))     # default fill
+        assert np.array_equal(f.variables["t"][:][[0, 1, 3]], [0.0, 10.0, 30.0])
+        assert np.array_equal(f.variables["rh"][:], rh)
+        assert f.variables["fld"].name == txt
+        assert np.array_equal(f.scale, dv)
+    # and the shim reads its own records back, the 1-D record variable too
+    _, got = _read_all(nc3, path)
+    assert np.array_equal(got["t"][2][[0, 1, 3]], [0.0, 10.0, 30.0]) and got["t"][2][2] == 9.9692099683868690e+36
+    assert np.array_equal(got["rh"][2], rh)
+    assert np.array_equal(got["rec"][2][3], recs[3].astype(np.float64))
+
+
+def test_lone_short_record_variable(nc3, tmp_path):
+    """One record variable of an odd-sized slab: records are not padded (the
+    classic format's one exception to 4-byte padding; scipy's reader pads it
+    anyway, so this is checked by size and by the shim's own reader)."""
+    path = str(tmp_path / "lone.nc")
+    ncid, d0, d1, v = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    _ok(nc3, nc3.nc3_create(path.encode(), 0, C.byref(ncid)))
+    _ok(nc3, nc3.nc3_def_dim(ncid, b"t", 0, C.byref(d0)))
+    _ok(nc3, nc3.nc3_def_dim(ncid, b"n", 3, C.byref(d1)))
+    _ok(nc3, nc3.nc3_def_var(ncid, b"s", 3, 2, (C.c_int * 2)(d0.value, d1.value), C.byref(v)))
+    _ok(nc3, nc3.nc3_enddef(ncid))
+    vals = np.arange(15, dtype=np.float64).reshape(5, 3) - 7
+    _ok(nc3, nc3.nc3_put_vara(ncid, v, (C.c_size_t * 2)(0, 0), (C.c_size_t * 2)(5, 3),
+                              vals.ctypes.data_as(C.c_void_p), DOUBLE))
+    _ok(nc3, nc3.nc3_close(ncid))
+    data = open(path, "rb").read()
+    assert data[4:8] == (5).to_bytes(4, "big")                    # numrecs
+    begin = len(data) - 5 * 6                                      # 3 shorts per record, no padding
+    assert data[begin - 4:begin] == begin.to_bytes(4, "big")       # the header's last word: begin
+    assert np.array_equal(np.frombuffer(data[begin:], dtype=">i2").reshape(5, 3), vals)
+    _, got = _read_all(nc3, path)
+    assert np.array_equal(got["s"][2], vals)
+
+
+def test_reads_64bit_offset_files(nc3, tmp_path):
+    path = str(tmp_path / "cdf2.nc")
+    rng = np.random.default_rng(1)
+    a = rng.normal(size=(3, 4, 6))
+    with netcdf_file(path, "w", version=2) as f:
+        f.createDimension("time", None)
+        f.createDimension("y", 4)
+        f.createDimension("x", 6)
+        v = f.createVariable("ha", "f", ("time", "y", "x"))
+        v[:] = a.astype(np.float32)
+        w = f.createVariable("xx", "d", ("x",))
+        w[:] = np.arange(6.0)
+        tv = f.createVariable("time", "d", ("time",))     # one value per record
+        tv[:] = [0.5, 7.25, 9.0]
+    _, got = _read_all(nc3, path)
+    assert np.array_equal(got["ha"][2], a.astype(np.float32).astype(np.float64))
+    assert np.array_equal(got["xx"][2], np.arange(6.0))
+    assert np.array_equal(got["time"][2], [0.5, 7.25, 9.0])
+
+
+def test_errors(nc3, tmp_path):
+    ncid = C.c_int()
+    assert nc3.nc3_open(str(tmp_path / "none.nc").encode(), 0, C.byref(ncid)) != 0
+    junk = tmp_path / "junk.nc"
+    junk.write_bytes(b"HDF\x89" + bytes(100))
+    assert nc3.nc3_open(str(junk).encode(), 0, C.byref(ncid)) == -51
+    assert nc3.nc3_close(12345) == -33
+    path = str(tmp_path / "e.nc")
+    _ok(nc3, nc3.nc3_create(path.encode(), 0, C.byref(ncid)))
+    d = C.c_int()
+    _ok(nc3, nc3.nc3_def_dim(ncid, b"t", 0, C.byref(d)))
+    assert nc3.nc3_def_dim(ncid, b"t2", 0, C.byref(d)) == -54    # second unlimited dimension
+    assert nc3.nc3_def_dim(ncid, b"t", 4, C.byref(d)) == -42     # name in use
+    _ok(nc3, nc3.nc3_close(ncid))
+    assert nc3.nc3_create(path.encode(), 4, C.byref(ncid)) == -35  # NOCLOBBER
+
+
+def test_fortran_entry_points(tmp_path):
+    if not os.path.exists(FNC3):
+        pytest.skip("tests/fortran/fnc3 not built (cliffs-src_amd/fortran/build.sh test)")
+    grid = os.path.join(SLP, "slpbeach1D.nc")
+    prefix = str(tmp_path / "f")
+    out = subprocess.run([FNC3, grid, prefix], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stdout + out.stderr
+    lines = {ln.split()[0]: ln.split()[1:] for ln in out.stdout.splitlines()}
+    with netcdf_file(grid, "r", mmap=False) as ref:
+        x = np.array(ref.variables["xxx"][:])
+        dep = np.array(ref.variables["bathy"][:], dtype=np.float64)   # (yyy, xxx)
+    # ids 1-based, Fortran dimension order: dim 1 is x (the last C dimension)
+    assert lines["inq"] == ["2", "3", "0", "-1"]
+    assert lines["dim1"] == ["xxx", "384"]
+    assert lines["grid"] == ["384", "1", "6", "6", "5", "bathy"]
+    assert float(lines["x"][2]) == pytest.approx(x.sum(), rel=1e-15)
+    assert float(lines["dep"][0]) == dep[0, 0] and float(lines["dep"][1]) == dep[0, -1]
+    assert float(lines["dep4"][0]) == pytest.approx(dep.sum(), rel=1e-15)
+    assert lines["badopen"] == ["1"]
+    assert " ".join(lines["strerror"]) == "NetCDF: Variable not found"
+    assert lines["records"] == ["3"]
+    assert lines["closeagain"] == ["-33"]
+    e = (np.arange(1, 385, dtype=np.float32)[None, :] + np.float32(1000.0) + np.float32(0.25))
+    with netcdf_file(prefix + "_fixed.nc", "r", mmap=False) as f:
+        assert f.variables["MaxE"].dimensions == ("yyy", "xxx")
+        assert np.array_equal(f.variables["MaxE"][:], e)
+        assert np.array_equal(f.variables["xxx"][:], x)
+        assert f.variables["MaxE"].name == b"Maximal Water Elevation"
+        assert f.model == b"Cliffs"
+    with netcdf_file(prefix + "_rec.nc", "r", mmap=False) as f:
+        assert np.array_equal(f.variables["time"][:], [10.0, 20.0, 30.0])
+        g = f.variables["gage"][:]                                   # (time, point)
+        assert np.array_equal(g[2], [4.5, -6.0, 3e-3])
+        ha = f.variables["ha"][:]
+        assert np.array_equal(ha[0], e) and np.all(ha[1] == np.float32(9.9692099683868690e+36))
+        want3 = e * 3
+        want3[0, 1:4] = [-1.0, -2.0, -3.0]
+        assert np.array_equal(ha[2], want3)
