@@ -238,7 +238,8 @@ class Grid:
     def run_timed(self, nsteps: int):
         """-> (ms_total, ms_a, ms_b), HIP events on the handle's stream: mean
         durations of the x- and y-sweep kernels, or of the fused step kernel
-        and its flag-check kernel when the step runs fused."""
+        and its flag-check kernel when the step runs fused, over every 8th
+        step of the run (the first included)."""
         t, x, y = C.c_double(), C.c_double(), C.c_double()
         _chk(load_library().cliffs_run_timed(self._h, int(nsteps), C.byref(t), C.byref(x),
                                              C.byref(y)))

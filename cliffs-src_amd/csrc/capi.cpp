@@ -787,10 +787,12 @@ int cliffs_run(cliffs_handle* h, int nsteps) {
     return CLIFFS_OK;
 }
 
-// Events bracket every sweep.  They live in a ring of kTimedSlots steps: before
-// a slot is reused its (long finished) events are read and accumulated, so the
-// host never waits on recent work and the event count stays bounded.
-constexpr int kTimedSlots = 256;
+// Events bracket the sweeps of every kTimedStride-th step.  They live in a
+// ring of kTimedSlots bracketed steps: before a slot is reused its (long
+// finished) events are read and accumulated, so the host never waits on
+// recent work and the event count stays bounded.
+constexpr int kTimedSlots = 256;   // event ring: slots of four events (x/y launch brackets)
+constexpr int kTimedStride = 8;   // one bracketed step in kTimedStride
 
 namespace {
 struct EventSet {
@@ -808,7 +810,7 @@ int cliffs_run_timed(cliffs_handle* h, int nsteps, double* ms_total, double* ms_
     if (nsteps < 1) return fail(CLIFFS_ERR_ARG, "nsteps < 1");
     HIPCHK(hipSetDevice(h->dev));
     int rc;
-    const int slots = std::min(nsteps, kTimedSlots);
+    const int slots = std::min((nsteps + kTimedStride - 1) / kTimedStride, kTimedSlots);
     EventSet es;
     es.ev.assign(4 * (size_t)slots + 2, nullptr);
     for (auto& e : es.ev) HIPCHK(hipEventCreate(&e));
@@ -829,20 +831,29 @@ int cliffs_run_timed(cliffs_handle* h, int nsteps, double* ms_total, double* ms_
         }
         return CLIFFS_OK;
     };
+    // every kTimedStride-th step is bracketed (event records between launches
+    // cost the stream a few microseconds each); its launches are a sample of
+    // the region's, in the same sequence
+    const int nsampled = (nsteps + kTimedStride - 1) / kTimedStride;
     HIPCHK(hipEventRecord(ev[0], h->st));
-    for (int s = 0; s < nsteps; s++) {
-        const int q = s % slots;
-        if (s >= slots && (rc = harvest(q))) return rc;
+    for (int s = 0, k = 0; s < nsteps; s++) {
+        if (s % kTimedStride) {
+            if ((rc = do_step(h, nullptr, nullptr))) return rc;
+            continue;
+        }
+        const int q = k % slots;
+        if (k >= slots && (rc = harvest(q))) return rc;
         if ((rc = do_step(h, nullptr, nullptr, ev + 2 + 4 * (size_t)q))) return rc;
+        k++;
     }
     HIPCHK(hipEventRecord(ev[1], h->st));
     HIPCHK(hipEventSynchronize(ev[1]));
     HIPCHK(hipEventElapsedTime(&t, ev[0], ev[1]));
     *ms_total = t;
-    for (int s = std::max(nsteps - slots, 0); s < nsteps; s++)
-        if ((rc = harvest(s % slots))) return rc;
-    *ms_x = h->xrun ? sx / nsteps : 0.0;
-    *ms_y = h->yrun ? sy / nsteps : 0.0;
+    for (int k = std::max(nsampled - slots, 0); k < nsampled; k++)
+        if ((rc = harvest(k % slots))) return rc;
+    *ms_x = h->xrun ? sx / nsampled : 0.0;
+    *ms_y = h->yrun ? sy / nsampled : 0.0;
     return CLIFFS_OK;
 }
 

@@ -99,13 +99,15 @@ int cliffs_step(cliffs_handle *h, const double *xedge1, const double *xedge2,
  * synchronisation. */
 int cliffs_run(cliffs_handle *h, int nsteps);
 
-/* Like cliffs_run, bracketing every kernel launch with HIP events on the
- * handle's stream.  Returns the device time from the first to the last
- * launch (ms_total) and mean kernel durations: of the x- and y-sweep
- * kernels, or, when the step runs fused (2-D grids, the default), of the
- * fused step kernel (ms_xsweep) and of its flag-check kernel (ms_ysweep).
- * Events are reused in a ring of 256 steps, so any nsteps works; the call
- * returns when the work has finished. */
+/* Like cliffs_run, bracketing the kernel launches of every 8th step (the
+ * first included) with HIP events on the handle's stream -- an event record
+ * between launches costs the stream a few microseconds, so the other steps
+ * run without.  Returns the device time from the first to the last launch
+ * (ms_total) and mean kernel durations over the bracketed steps: of the x-
+ * and y-sweep kernels, or, when the step runs fused (2-D grids, the
+ * default), of the fused step kernel (ms_xsweep) and of its flag-check
+ * kernel (ms_ysweep).  Events are reused in a ring, so any nsteps works;
+ * the call returns when the work has finished. */
 int cliffs_run_timed(cliffs_handle *h, int nsteps, double *ms_total, double *ms_xsweep,
                      double *ms_ysweep);
 

@@ -190,3 +190,21 @@ def test_tile_rows_rejects_bad_values(gpu_lib):
         with pytest.raises(gpu_lib.CliffsError):
             g.set_tile_rows(bad)
     g.close()
+
+
+def test_run_timed_matches_run(gpu_lib):
+    """cliffs_run_timed (events around every 8th step, in a ring of 256
+    bracketed steps) advances the grid exactly as cliffs_run does, for step
+    counts below, at and past the sampling stride and the ring."""
+    case = synth.coastal(nx=300, ny=260, seed=7)
+    for n in (1, 8, 9, 2100):
+        a = _grid(gpu_lib, case, True)
+        a.run(n)
+        want = _result(a)
+        a.close()
+        b = _grid(gpu_lib, case, True)
+        total, k1, k2 = b.run_timed(n)
+        got = _result(b)
+        b.close()
+        _same(got, want)
+        assert total > 0 and k1 > 0 and k2 > 0 and k1 + k2 <= total / n * 8 + 1e-3
