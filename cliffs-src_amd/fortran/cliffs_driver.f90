@@ -112,7 +112,7 @@ contains
     integer, intent(out) :: err
     integer :: num, i
     err = 0
-    write(LOGU, *) 'Read Computational parameters: ', trim(indir)//trim(pfile)
+    write(LOGU, '(1x,2a)') 'Read Computational parameters: ', trim(indir)//trim(pfile)
     open(unit=PARU, file=trim(indir)//trim(pfile), status='old', form='formatted')
     read(PARU, *) num
     cartesian = num == 1
@@ -174,7 +174,7 @@ contains
     k1 = index(fname, '/', .true.) + 1
     k2 = index(fname, '.nc') - 1
     gridname = fname(k1:k2)
-    write(LOGU, *) 'Master bathy/topo file:  ', trim(indir)//trim(fname)
+    write(LOGU, '(1x,2a)') 'Master bathy/topo file:  ', trim(indir)//trim(fname)
     call nc(nf_open(trim(indir)//trim(fname), NF_NOWRITE, ncid))
     call nc(nf_inq_dimlen(ncid, 1, nx))
     call nc(nf_inq_dimlen(ncid, 2, ny))
@@ -258,7 +258,7 @@ contains
     integer :: k
     err = 0
     first = 0
-    write(LOGU, *) ' Boundary Input: ', trim(srcdir)//trim(bname)
+    write(LOGU, '(1x,2a)') 'Boundary Input: ', trim(srcdir)//trim(bname)
     if (xrun) then
       call read_side('west', ny, west, .true., err)
       if (err == 0) call read_side('east', ny, east, .false., err)
@@ -299,7 +299,7 @@ contains
       err = 0
       if (nf_open(trim(srcdir)//trim(bname)//'_'//side//'.nc', NF_NOWRITE, ncid) /= NF_NOERR) then
         err = -1
-        write(LOGU, *) trim(srcdir)//trim(bname)//'_'//side//'.nc not found'
+        write(LOGU, '(1x,a)') trim(srcdir)//trim(bname)//'_'//side//'.nc not found'
         return
       end if
       call nc(nf_inq_dimlen(ncid, 1, np))
@@ -342,7 +342,7 @@ contains
     err = 0
     do i = 1, 3
       have(i) = nf_open(trim(srcdir)//trim(aprfx)//'_'//suf(i)//'.nc', NF_NOWRITE, ids(i)) == NF_NOERR
-      write(LOGU, *) 'Initial ', suf(i), ': ', trim(srcdir)//trim(aprfx)//'_'//suf(i)//'.nc', have(i)
+      write(LOGU, '(1x,4a,1x,l1)') 'Initial ', suf(i), ': ', trim(srcdir)//trim(aprfx)//'_'//suf(i)//'.nc', have(i)
     end do
     uin = have(1)
     vin = have(2)
@@ -505,7 +505,7 @@ contains
     call nc(nf_put_var_double(f%ncid, xid, xcrd))
     call nc(nf_put_var_double(f%ncid, yid, ycrd))
     call nc(nf_sync(f%ncid))
-    write(LOGU, *) 'Maxwave output to: ', trim(fname)
+    write(LOGU, '(1x,2a)') 'Maxwave output to: ', trim(fname)
   end subroutine open_max
 
   ! MaxE = hmax^2/g - dep and MaxV = sqrt(velmax) where hmax > celmin, else
@@ -563,7 +563,7 @@ contains
     call nc(nf_enddef(f%ncid))
     call nc(nf_put_var_double(f%ncid, xid, gx))
     call nc(nf_put_var_double(f%ncid, yid, gy))
-    write(LOGU, *) 'Screenshot output to: ', trim(fname)
+    write(LOGU, '(1x,2a)') 'Screenshot output to: ', trim(fname)
   end subroutine open_sea
 
   ! one snapshot record of a field (output_sea_state.f:14-58)
@@ -620,7 +620,7 @@ contains
     end do
     call nc(nf_put_var_double(f%ncid, xid, gx))
     call nc(nf_put_var_double(f%ncid, yid, gy))
-    write(LOGU, *) 'Time-series output to  ', trim(fname)
+    write(LOGU, '(1x,2a)') 'Time-series output to  ', trim(fname)
   end subroutine open_gauges
 
   subroutine write_gauges(f, t, eta, u, v)
