@@ -374,6 +374,7 @@ FArgs fargs(cliffs_handle* h, const float* xe, const float* ye) {
 
 // Line flags of the current state for the next fused step.
 int ensure_fused(cliffs_handle* h) {
+    if (!h->fflags) return fail(CLIFFS_ERR_STATE, "no fused-step flag buffers (a tile-height change failed)");
     if (h->fused_valid) return CLIFFS_OK;
     FArgs a = fargs(h, nullptr, nullptr);
     a.rfl_out = h->rfl(h->fn & 1);
@@ -407,9 +408,12 @@ int set_fused_rows(cliffs_handle* h, int rows) {
     h->f_cstride = (nc + 15) / 16 * 16;
     const size_t fb = 2 * (h->rfl_bytes() + h->cpr_bytes() + (size_t)h->nx);
     // cpr bytes beyond nchunks are read as zeros and never written
-    if (hipMalloc(&h->fflags, fb) != hipSuccess) return fail(CLIFFS_ERR_NOMEM, "flag allocation");
-    HIPCHK(hipMemset(h->fflags, 0, fb));
     h->fused_valid = false;
+    if (hipMalloc(&h->fflags, fb) != hipSuccess) {
+        h->fflags = nullptr;  // ensure_fused refuses to step until a geometry is set again
+        return fail(CLIFFS_ERR_NOMEM, "flag allocation");
+    }
+    HIPCHK(hipMemset(h->fflags, 0, fb));
     return CLIFFS_OK;
 }
 
@@ -979,6 +983,7 @@ int ensemble_rebuild(cliffs_ensemble* e) {
 // Members' work so far happens before the ensemble's next work; flags of the
 // fused step are made current on each member's stream first.
 int ensemble_take(cliffs_ensemble* e) {
+    if (!e->gexec) return fail(CLIFFS_ERR_STATE, "ensemble has no step graph (a rebuild failed)");
     for (cliffs_handle* h : e->m) {
         int rc;
         if ((rc = ensure_fused(h))) return rc;
@@ -1029,7 +1034,15 @@ int ensemble_set_rows(cliffs_ensemble* e, int rows) {
     HIPCHK(hipStreamSynchronize(e->st));
     for (cliffs_handle* h : e->m) {
         const int rc = set_fused_rows(h, rows);
-        if (rc) return rc;
+        if (rc) {
+            // the captured graph and block map no longer match the members'
+            // tiles: refuse to run until a rebuild succeeds
+            if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+            if (e->graph) (void)hipGraphDestroy(e->graph);
+            e->gexec = nullptr;
+            e->graph = nullptr;
+            return rc;
+        }
     }
     e->rows = rows;
     return ensemble_rebuild(e);

@@ -18,6 +18,7 @@
 #define MAXDIMS 1024
 #define MAXVARDIMS 32
 #define MAXNAME 256
+#define BADMAGIC (-1000) /* internal: not a classic file at all (no retry) */
 
 typedef struct {
     char name[MAXNAME];
@@ -239,25 +240,43 @@ static void layout_records(file_t *f) {
     }
 }
 
+static int parse_header(file_t *f, const unsigned char *buf, size_t hn);
+
+/* The header is read in a buffer that doubles until it holds the whole
+ * header (a few KB for the files Cliffs writes; the data is not read). */
 static int read_header(file_t *f) {
     if (fseek(f->fp, 0, SEEK_END)) return NC3_EIO;
     const long fsz = ftell(f->fp);
     if (fsz < 32) return NC3_ENOTNC;
-    // the header is at most a few MB for any grid file; read up to 64 MB
-    const size_t hn = (size_t)(fsz < (64L << 20) ? fsz : (64L << 20));
-    unsigned char *buf = (unsigned char *)malloc(hn);
-    if (!buf) return NC3_ENOMEM;
-    rewind(f->fp);
-    if (fread(buf, 1, hn, f->fp) != hn) {
+    for (size_t hn = 64 * 1024;; hn *= 2) {
+        if (hn > (size_t)fsz) hn = (size_t)fsz;
+        unsigned char *buf = (unsigned char *)malloc(hn);
+        if (!buf) return NC3_ENOMEM;
+        rewind(f->fp);
+        if (fread(buf, 1, hn, f->fp) != hn) {
+            free(buf);
+            return NC3_EIO;
+        }
+        const int rc = parse_header(f, buf, hn);
         free(buf);
-        return NC3_EIO;
+        if (rc == BADMAGIC) return NC3_ENOTNC;
+        if (rc != NC3_ENOTNC || hn == (size_t)fsz) return rc;
+        // possibly a header longer than the buffer: start over with a larger one
+        for (int v = 0; v < f->nvars; v++) free_atts(&f->vars[v].atts);
+        free(f->vars);
+        free(f->dims);
+        free_atts(&f->gatts);
+        f->vars = NULL;
+        f->dims = NULL;
+        f->nvars = f->ndims = 0;
+        f->unlim = -1;
     }
+}
+
+static int parse_header(file_t *f, const unsigned char *buf, size_t hn) {
     rd_t r = {buf, buf + hn, 0};
     int rc = NC3_NOERR;
-    if (memcmp(buf, "CDF", 3) != 0 || (buf[3] != 1 && buf[3] != 2)) {
-        free(buf);
-        return NC3_ENOTNC;
-    }
+    if (memcmp(buf, "CDF", 3) != 0 || (buf[3] != 1 && buf[3] != 2)) return BADMAGIC;
     f->version = buf[3];
     r.p += 4;
     const uint32_t nrecs = rd_u32(&r);
@@ -310,7 +329,6 @@ static int read_header(file_t *f) {
         v->vsize = pad4(s);
     }
     if (!rc && r.bad) rc = NC3_ENOTNC;
-    free(buf);
     if (rc) return rc;
     layout_records(f);
     f->recbegin = 0;

@@ -274,3 +274,15 @@ def test_fortran_entry_points(tmp_path):
         want3 = e * 3
         want3[0, 1:4] = [-1.0, -2.0, -3.0]
         assert np.array_equal(ha[2], want3)
+
+
+def test_header_longer_than_first_read(nc3, tmp_path):
+    """A 150 KB text attribute puts the variables past the reader's first
+    64 KB header buffer: it grows the buffer and parses again."""
+    path = str(tmp_path / "longhdr.nc")
+    with netcdf_file(path, "w") as f:
+        f.history = "x" * 150_000
+        f.createDimension("x", 5)
+        f.createVariable("v", "d", ("x",))[:] = np.arange(5.0) * 1.5
+    _, got = _read_all(nc3, path)
+    assert np.array_equal(got["v"][2], np.arange(5.0) * 1.5)
