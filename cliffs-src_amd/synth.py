@@ -138,9 +138,11 @@ def beach_1d(seed=0, n=2000, dx=0.25, steps=20000, amplitude=None):
 
 
 def coastal(nx=256, ny=192, seed=0, cartesian=True, dx=500.0, steps=200, source_amp=3.0,
-            crough=0.0, dlon=None):
+            crough=0.0, dlon=None, lat0=38.0):
     """Shelf + seeded coastline with land up to +300 m; elliptical uplift/
-    subsidence dipole (+A, -A/2) offshore.  Land is on the high-x side."""
+    subsidence dipole (+A, -A/2) offshore.  Land is on the high-x side.
+    Spherical grids have spacing dlon (30 arc-seconds by default) and start
+    at longitude 140, latitude lat0."""
     rng = np.random.default_rng(seed)
     if cartesian:
         x = np.arange(nx, dtype=np.float64) * dx
@@ -149,7 +151,7 @@ def coastal(nx=256, ny=192, seed=0, cartesian=True, dx=500.0, steps=200, source_
         Lx, Ly = x[-1], y[-1]
     else:
         dl = (30.0 / 3600.0) if dlon is None else dlon
-        lon0, lat0 = 140.0, 38.0
+        lon0 = 140.0
         x = lon0 + np.arange(nx) * dl
         y = lat0 + np.arange(ny) * dl
         Xd, Yd = np.meshgrid(x - x[0], y - y[0])

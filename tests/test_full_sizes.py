@@ -1,0 +1,150 @@
+"""The workloads at their full sizes, through properties that do not need
+the oracle to run the whole thing where it cannot:
+
+* 1-D sloping beach, 2000 cells, 20,000 steps (the full run): surface
+  elevation within 1e-5 of the amplitude of the FP64 oracle on cells wet in
+  both runs, at most one wet/dry flip, run-up limit within one cell;
+* 4096 x 4096 open ocean, 100 steps of the full grid against the oracle at
+  the bar of test_gpu_parity (the oracle runs on all host threads);
+* spherical 8192 x 6144 parent with run-up and a 2047 x 2047 child at 3x
+  refinement on its coast: the child stepped from device feed records
+  (cliffs_nest_couple/record/child_step) ends bit-identical to the child
+  stepped from feed records brought to the host and blended there as
+  cliffs_main.f:305-339 does.
+"""
+import numpy as np
+import pytest
+from scipy.interpolate import RegularGridInterpolator
+
+import oracle as O
+import synth
+from test_gpu_parity import _compare
+
+pytestmark = pytest.mark.gpu
+G = synth.GRAV
+
+
+def _wet_eta(c, dep, celmin):
+    return c > celmin, c * c / G - dep
+
+
+def test_c1_beach_full_run(gpu_lib):
+    case = synth.beach_1d(seed=1)                 # 2000 cells, 20,000 steps
+    s1, s2, zeta = case.metrics()
+    orc = O.Oracle(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground, crough=case.crough)
+    orc.set_state(case.cel, case.xvel, case.yvel)
+    g = gpu_lib.Grid(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground, crough=case.crough)
+    g.upload(case.cel, case.xvel, case.yvel)
+    amp, worst, flips = case.meta["amplitude"], 0.0, 0
+    for _ in range(40):
+        orc.steps(500)
+        g.run(500)
+        wo, eo = _wet_eta(orc.get_state()[0], case.dep, orc.celmin)
+        wg, eg = _wet_eta(g.download()[0], case.dep, orc.celmin)
+        both = wo & wg
+        worst = max(worst, float(np.abs(eg - eo)[both].max()))
+        flips = max(flips, int((wo != wg).sum()))
+    hm_o, hm_g = orc.get_state()[3][0], g.download_max()[0][0]
+    lim = abs(int(np.argmax(hm_o > orc.celmin)) - int(np.argmax(hm_g > orc.celmin)))
+    g.close()
+    print(f"C1: eta err {worst:.3e} (bar {1e-5 * amp:.3e}), flips {flips}, run-up limit cells {lim}")
+    assert worst <= 1e-5 * amp and flips <= 1 and lim <= 1
+
+
+def test_c2_open_ocean_full_grid(gpu_lib):
+    case = synth.open_ocean(n=4096, seed=0)
+    s1, s2, zeta = case.metrics()
+    orc = O.Oracle(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground, crough=case.crough)
+    orc.set_state(case.cel, case.xvel, case.yvel)
+    g = gpu_lib.Grid(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground, crough=case.crough)
+    g.upload(case.cel, case.xvel, case.yvel)
+    orc.steps(100)
+    g.run(100)
+    err = _compare(case, orc, g, amp=1.0)          # the 1 m hump
+    g.close()
+    print(f"C2: eta err {err:.3e} after 100 steps of the full grid (bar 1e-5 m)")
+
+
+def _host_edges(recs, state, t, mx):
+    """cliffs_main.f:305-339 in FP64: records (t, west, east, south, north)."""
+    while state["bndr"] and t >= recs[state["irec"] - 1][0]:
+        assert state["irec"] < len(recs), "child ahead of the feed"
+        state["irec"] += 1
+    k = state["irec"]
+    t0, t1 = recs[k - 2][0], recs[k - 1][0]
+    cc = (t1 - t) / (t1 - t0)
+    out = []
+    for side in range(1, 5):
+        e = recs[k - 2][side] * cc + recs[k - 1][side] * (1 - cc)
+        pad = np.zeros((mx, 3))
+        pad[:e.shape[0]] = e
+        out.append(pad)
+    return out
+
+
+def test_c3_nest_full_size(gpu_lib):
+    case = synth.coastal(nx=8192, ny=6144, seed=3, cartesian=False, lat0=-25.0)
+    x, y = case.xcrd, case.ycrd
+    i0, j0, n = 5350, 3000, 682                   # child: parent cells i0..i0+n, j0..j0+n (coast)
+    xc = np.linspace(x[i0], x[i0 + n], 3 * n + 1)
+    yc = np.linspace(y[j0], y[j0 + n], 3 * n + 1)
+    interp = RegularGridInterpolator((y[j0:j0 + n + 1], x[i0:i0 + n + 1]),
+                                     case.dep[j0:j0 + n + 1, i0:i0 + n + 1])
+    Yc, Xc = np.meshgrid(yc, xc, indexing="ij")
+    depc = interp(np.stack([Yc, Xc], axis=-1))
+    del Yc, Xc
+    maps = gpu_lib.nest_maps(x, y, xc, yc)
+    s1, s2, zeta = case.metrics()
+    cs1, cs2, czeta = synth.metrics(xc, yc, False)
+
+    def parent():
+        p = gpu_lib.Grid(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground)
+        p.upload(case.cel, case.xvel, case.yvel)
+        return p
+
+    def child():
+        c = gpu_lib.Grid(depc, cs1, cs2, czeta, dt=case.dt / 3, ground=case.ground)
+        z = np.zeros_like(depc)
+        c.upload(synth.depth_to_celerity(depc, case.ground), z, z)
+        return c
+
+    gA, cA = parent(), child()
+    gB, cB = parent(), child()
+    nA = gpu_lib.Nest(gA, maps)
+    nB = gpu_lib.Nest(gB, maps)
+    nB.couple(cB, capacity=4)
+    dtp, dtc, bndout, cuke = case.dt, case.dt / 3, 2, 0.01
+    recs, state = [], {"irec": 1, "bndr": True}
+    iarvA = iarvB = False
+    tp, tc, child_steps = 0.0, None, 0
+    for _ in range(8):
+        gA.run(bndout)
+        gB.run(bndout)
+        tp += bndout * dtp
+        iarvA, fed, w, e, s, nn = nA.feed(cuke, case.ground, iarvA)
+        if fed:
+            recs.append((tp, w, e, s, nn))
+        iarvB, nrec = nB.record(cuke, case.ground, tp, iarvB)
+        assert iarvA == iarvB and nrec == len(recs)
+        if len(recs) < 2:
+            continue
+        if tc is None:
+            tc = recs[0][0]
+        while tc + dtc < recs[-1][0]:
+            tc += dtc
+            cA.step(*_host_edges(recs, state, tc, cA.mx))
+            nB.child_step(tc)
+            child_steps += 1
+    assert len(recs) >= 6 and child_steps >= 25
+    assert max(float(np.abs(r[1][:, 2]).max()) for r in recs) > 1e-3     # a wave on the west edge
+    a = cA.download() + cA.download_max()
+    b = cB.download() + cB.download_max()
+    for arr_a, arr_b, name in zip(a, b, ("cel", "xvel", "yvel", "hmax", "velmax")):
+        bad = int((arr_a.view(np.int64) != arr_b.view(np.int64)).sum())
+        assert bad == 0, f"child {name}: {bad} cells differ"
+    print(f"C3: parent {case.dep.shape}, child {depc.shape}, {len(recs)} records, {child_steps} child steps, "
+          f"child max |u| {float(np.abs(b[1]).max()):.3e}")
+    for h in (nA, nB):
+        h.close()
+    for h in (gA, gB, cA, cB):
+        h.close()
