@@ -13,7 +13,9 @@ default step is one fused kernel (x-sweep, y-sweep and max-wave of a step in
 one pass): 44 B per cell-update, i.e. read s, cp, u, v, hmax, velmax and
 write cp, u, v, hmax, velmax in FP32.  With --unfused the step is two sweep
 kernels: 28 B per cell for the x-sweep (read s, cp, u, v; write cp, u, v) and
-44 B for the y-sweep, which also carries the max-wave update.
+44 B for the y-sweep, which also carries the max-wave update.  --coastal
+times a spherical 8192 x 6144 coastal grid with run-up instead (land, dry
+lines and wet/dry fronts in the tiles); the default is the line above.
 """
 import argparse
 import json
@@ -37,11 +39,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def pmc_traffic(kernel, n):
-    """HBM bytes per launch of `kernel` on the n x n workload, from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/), already
-    scaled by the plain-copy calibration; None when no summary covers it."""
-    path = os.path.join(ROOT, "profiles", f"pmc_traffic_open_ocean_{n}.txt")
+def pmc_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on the workload ("open_ocean_4096",
+    "coastal_8192x6144"), from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (profiles/), already scaled by the plain-copy
+    calibration; None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.txt")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -98,15 +101,12 @@ def ensemble_bench(nscen, steps, seed0):
                       f"(within one cell)"}
 
 
-def cpu_baseline(n, seed, steps):
+def cpu_baseline(case, steps):
     """FP64 C restatement (oracle/) on the host cores, on a bounded sample of
-    the same workload: `steps` steps of the full n x n grid."""
-    import numpy as np  # noqa: F401
+    the same workload: `steps` steps of the full grid."""
     import oracle
-    import synth
     if not os.path.exists(oracle.ORACLE_SO):
         oracle.build(with_ref=False)
-    case = synth.open_ocean(n=n, seed=seed)
     s1, s2, zeta = case.metrics()
     o = oracle.Oracle(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground)
     o.set_state(case.cel, case.xvel, case.yvel)
@@ -115,9 +115,11 @@ def cpu_baseline(n, seed, steps):
     o.steps(steps)
     dt = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": n * n * steps / dt / 1e9, "unit": "Gcell-updates/s", "cores": cores,
+    ny, nx = case.dep.shape
+    name = case.name.replace("_", "-")
+    return {"value": nx * ny * steps / dt / 1e9, "unit": "Gcell-updates/s", "cores": cores,
             "kind": "port",
-            "sample": f"{steps} steps of the {n}x{n} open-ocean grid after 1 warm-up step, "
+            "sample": f"{steps} steps of the {nx}x{ny} {name} grid after 1 warm-up step, "
                       f"FP64 C restatement of cliffs.f with OpenMP over lines, "
                       f"{dt:.1f} s wall"}
 
@@ -134,6 +136,10 @@ def main():
     ap.add_argument("--ensemble-scenarios", type=int, default=256,
                     help="scenarios of the batched-ensemble measurement (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=4000)
+    ap.add_argument("--coastal", action="store_true",
+                    help="instead of the open ocean: the spherical 8192 x 6144 coastal grid "
+                         "(30 arc-seconds, shelf, coastline, run-up), whose tiles take the "
+                         "wet/dry paths")
     ap.add_argument("--unfused", action="store_true",
                     help="run the step as two sweep kernels instead of one fused kernel")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -171,8 +177,20 @@ def main():
     import synth
 
     n = args.n
-    log(f"[bench] generating {n}x{n} open-ocean case, seed {args.seed}")
-    case = synth.open_ocean(n=n, seed=args.seed, steps=args.steps)
+    if args.coastal:
+        log(f"[bench] generating the 8192x6144 spherical coastal case, seed {args.seed}")
+        case = synth.coastal(nx=8192, ny=6144, seed=args.seed, cartesian=False, lat0=-25.0,
+                             steps=args.steps)
+        workload = (f"2D spherical coastal grid 8192x6144 at 30 arc-seconds from 25 S, shelf "
+                    f"and seeded coastline with land to +300 m, uplift/subsidence dipole 3 m, "
+                    f"run-up, seed {args.seed}")
+    else:
+        log(f"[bench] generating {n}x{n} open-ocean case, seed {args.seed}")
+        case = synth.open_ocean(n=n, seed=args.seed, steps=args.steps)
+        workload = (f"2D Cartesian open ocean {n}x{n}, dx=dy=500 m, depth 4000 m minus fractal "
+                    f"seamounts <=2500 m, Gaussian hump 1 m sigma 20 km, open boundaries, "
+                    f"non-dispersive, seed {args.seed}")
+    ny, nx = case.dep.shape
     s1, s2, zeta = case.metrics()
     g = cliffs_gpu.Grid(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground,
                         crough=case.crough, device=0)
@@ -188,7 +206,7 @@ def main():
     ms_dev, ms_x, ms_y = g.run_timed(args.steps)
     g.sync()
     wall = time.perf_counter() - t0
-    cells = n * n
+    cells = nx * ny
     value = cells * args.steps / wall / 1e9
     log(f"[bench] {wall:.3f} s wall, {ms_dev:.1f} ms device, {value:.2f} Gcell-updates/s")
     c, u, v = g.download()
@@ -212,7 +230,7 @@ def main():
         step_bytes = F_BYTES
     roof = {
         "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(dom, n),
+        "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(dom, "coastal_8192x6144" if args.coastal else f"open_ocean_{n}"),
         "kernel": dom, "bytes_per_cell": byt, "ms_per_launch": round(ms, 4),
         "other_kernel": other,
         "step_bytes_per_cell": step_bytes,
@@ -227,17 +245,15 @@ def main():
     cpu = None
     if not args.no_cpu:
         log(f"[bench] CPU baseline: {args.cpu_steps} steps of the FP64 oracle")
-        cpu = cpu_baseline(n, args.seed, args.cpu_steps)
+        cpu = cpu_baseline(case, args.cpu_steps)
         log(f"[bench] CPU {cpu['value']:.4f} Gcell-updates/s on {cpu['cores']} cores")
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(wall / args.steps * 1e3, 5), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"2D Cartesian open ocean {n}x{n}, dx=dy=500 m, depth 4000 m "
-                               f"minus fractal seamounts <=2500 m, Gaussian hump 1 m sigma 20 km, "
-                               f"open boundaries, non-dispersive, seed {args.seed}",
-                   "nx": n, "ny": n, "dt_s": round(case.dt, 6), "state_finite": finite},
+        "config": {"workload": workload, "nx": nx, "ny": ny, "dt_s": round(case.dt, 6),
+                   "state_finite": finite},
         "roofline": roof,
         "ensemble": ensemble,
         "cpu_baseline": cpu,

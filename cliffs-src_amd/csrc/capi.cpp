@@ -113,7 +113,7 @@ struct cliffs_handle {
     int f_nstrips = 0, f_nchunks = 0, f_rows = 0;
     int f_cus = 0, f_bpc = 0;  // CUs of the device, fused-kernel blocks per CU
     int f_cstride = 0;         // bytes per column of cpr: roundup(nchunks, 16)
-    unsigned char* fflags = nullptr;  // rfl[2][nstrips][ny], cpr[2][nx][cstride], ctr[2][nx]
+    unsigned char* fflags = nullptr;  // rfl[2][nstrips][ny], cpr[2][nx][cstride], ctr[2][nx], mpend[ntiles]
     int fn = 0;                // fused step parity
     bool fused_valid = false;
     bool state = false;
@@ -145,6 +145,7 @@ struct cliffs_handle {
     unsigned char* ctr(int b) {
         return fflags + 2 * rfl_bytes() + 2 * cpr_bytes() + (size_t)b * nx;
     }
+    unsigned char* mpend() { return ctr(2); }
 };
 
 struct cliffs_nest {
@@ -364,6 +365,7 @@ FArgs fargs(cliffs_handle* h, const float* xe, const float* ye) {
     a.cpr_out = h->cpr(b ^ 1);
     a.ctr = h->ctr(b);
     a.ctr_next = h->ctr(b ^ 1);
+    a.mpend = h->mpend();
     a.row_lo = h->jj1;
     a.row_hi = h->jj2;
     a.col_lo = h->ii1;
@@ -406,7 +408,7 @@ int set_fused_rows(cliffs_handle* h, int rows) {
     h->f_nchunks = nc;
     h->f_rows = r;
     h->f_cstride = (nc + 15) / 16 * 16;
-    const size_t fb = 2 * (h->rfl_bytes() + h->cpr_bytes() + (size_t)h->nx);
+    const size_t fb = 2 * (h->rfl_bytes() + h->cpr_bytes() + (size_t)h->nx) + (size_t)ns * nc;
     // cpr bytes beyond nchunks are read as zeros and never written
     h->fused_valid = false;
     if (hipMalloc(&h->fflags, fb) != hipSuccess) {

@@ -103,6 +103,7 @@ struct FArgs {
     unsigned char* cpr_out;       // the same, leaving the step
     unsigned char* ctr;           // [nx]: column i wet after the x-sweep (only 1s stored)
     unsigned char* ctr_next;      // [nx]: the next step's ctr, cleared by the fixup kernel
+    unsigned char* mpend;         // [ntiles]: a deferred max-wave update of the tile changes a maximum
     int row_lo, row_hi;           // rows the x-sweep covers (0-based, inclusive)
     int col_lo, col_hi;           // columns the y-sweep covers
     int maxwave;                  // update hmax/velmax (0 only for timing trials)

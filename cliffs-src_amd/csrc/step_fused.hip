@@ -27,7 +27,9 @@
 // max-wave update is deferred to the fixup kernel, so a wrong prediction
 // never leaves a wrong maximum behind.  (Sweeping an all-dry column only
 // resets its nodes to c = u = v = 0, which cannot raise a maximum, so the
-// opposite mistake needs no deferral.)
+// opposite mistake needs no deferral.)  The step kernel reads those
+// columns' maxima and flags the tile (mpend) only if the update would change
+// one, so the fixup reads dry land again only where it must.
 #include <hip/hip_runtime.h>
 
 #include <vector>
@@ -121,13 +123,16 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
     } else if (in_range) {
         yflag = column_any(a.cpr_in, a.cstride, col) ? 1 : 0;
     }
-    const int keepmax = (a.maxwave && col_ok && (kExact || !in_range || yflag)) ? 1 : 0;  // else deferred
+    // max-wave of the column updated here (keepmax) or deferred (loadmax
+    // only: the tile is flagged if the update would change a maximum)
+    const int keepmax = (a.maxwave && col_ok && (kExact || !in_range || yflag)) ? 1 : 0;
+    const int loadmax = (a.maxwave && col_ok) ? 1 : 0;
     YCol cc;
     cc.swept = in_range && yflag;
     cc.load_edges(a.yedges, nx, col, col_ok);
     YPipe pipe;
     pipe.init();
-    int colwet_f = 0;
+    int colwet_f = 0;  // bit 0: a node of the column is wet after the step; bit 1: deferred change
     for (int k = t; k < j1 - j0; k += 256) sh.rowany[k] = 0u;
     __syncthreads();  // xmt and rowany are complete
 
@@ -167,7 +172,7 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int r = j0 - 6 + 4 * g + k;
-            const bool ok = keepmax && r >= j0 && r < j1;
+            const bool ok = loadmax && r >= j0 && r < j1;
             const unsigned o = (unsigned)max(r, 0) * pitch + (unsigned)col;
             h[k] = ok ? F.ld(oHM + o) : 0.f;
             v[k] = ok ? F.ld(oVM + o) : 0.f;
@@ -221,10 +226,12 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
                         F.st(oCPn + o, f.cp);
                         F.st(oUn + o, f.x);
                         F.st(oVn + o, f.y);
-                        if (keepmax) {  // max-wave (cliffs_main.f:347-355): store on change only
+                        {  // max-wave (cliffs_main.f:347-355): store on change only
                             const float sp2 = f.x * f.x + f.y * f.y;
-                            if (f.cp > hc[k]) F.st(oHM + o, f.cp);
-                            if (sp2 > vc[k]) F.st(oVM + o, sp2);
+                            const bool ch = f.cp > hc[k], cv = sp2 > vc[k];
+                            if (keepmax && ch) F.st(oHM + o, f.cp);
+                            if (keepmax && cv) F.st(oVM + o, sp2);
+                            colwet_f |= (ch || cv) ? 2 : 0;  // read only for deferred columns
                         }
                         wet = !is_land(c0_of(f.s) + f.cp, p);
                         colwet_f |= wet ? 1 : 0;
@@ -236,9 +243,10 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
         // no second barrier: the next X phase fills the other buffer, and the
         // one after it waits at the next barrier for this Y phase to finish
     }
-    __syncthreads();
+    const int tile_mchange = __syncthreads_or((colwet_f & 2) && loadmax && !keepmax);
+    if (!kExact && t == 0) a.mpend[tile] = tile_mchange ? 1 : 0;
     for (int k = t; k < j1 - j0; k += 256) a.rfl_out[(size_t)strip * ny + j0 + k] = sh.rowany[k] ? 1 : 0;
-    if (col_ok) a.cpr_out[(size_t)col * a.cstride + chunk] = colwet_f ? 1 : 0;
+    if (col_ok) a.cpr_out[(size_t)col * a.cstride + chunk] = colwet_f & 1;
     if (!kExact && outmask && ld_lane && colwet_x) {
 #pragma unroll
         for (int k = 0; k < 4; k++)
@@ -272,13 +280,17 @@ __device__ __forceinline__ void fixup_tile(const FArgs& a, int tile, FShared& sh
         fused_tile<true>(a, tile, sh);
         return;
     }
-    if (a.maxwave && in_range && !pred) {  // column left unswept: max-wave of its final state
+    // columns left unswept: max-wave of their final state, in tiles where the
+    // step kernel found it changes a maximum (after the first steps, dry land
+    // changes none); store on change only, as the step kernel does
+    if (a.maxwave && a.mpend[tile] && in_range && !pred) {
         const int j0 = chunk * a.rows, j1 = min(j0 + a.rows, ny);
         for (int r = j0; r < j1; r++) {
             const size_t o = (size_t)r * a.g.pitch + col;
-            const float u = a.g.Un[o], v = a.g.Vn[o];
-            a.g.HM[o] = fmaxf(a.g.HM[o], a.g.CPn[o]);
-            a.g.VM[o] = fmaxf(a.g.VM[o], u * u + v * v);
+            const float cp = a.g.CPn[o], u = a.g.Un[o], v = a.g.Vn[o];
+            const float sp2 = u * u + v * v;
+            if (cp > a.g.HM[o]) a.g.HM[o] = cp;
+            if (sp2 > a.g.VM[o]) a.g.VM[o] = sp2;
         }
     }
 }
