@@ -7,7 +7,10 @@ ARCH=${CLIFFS_ARCH:-gfx950}
 FLAGS="-O3 -std=c++17 -fPIC --offload-arch=$ARCH -Xclang -target-feature -Xclang -packed-fp32-ops -Wall"
 cd "$HERE/csrc"
 $HIPCC $FLAGS -c sweeps.hip -o "$HERE/csrc/sweeps.o"
-$HIPCC $FLAGS -c step_fused.hip -o "$HERE/csrc/step_fused.o"
+# the fused step kernel is bound by instruction issue at 168 VGPRs; the
+# scheduler's AMDGPU register-pressure trackers give it a ~1 % faster schedule
+# (same arithmetic, bit-identical results)
+$HIPCC $FLAGS -mllvm -amdgpu-use-amdgpu-trackers -c step_fused.hip -o "$HERE/csrc/step_fused.o"
 $HIPCC $FLAGS -c nest.hip -o "$HERE/csrc/nest.o"
 $HIPCC $FLAGS -c surrogate.hip -o "$HERE/csrc/surrogate.o"
 $HIPCC $FLAGS -c capi.cpp -o "$HERE/csrc/capi.o"
