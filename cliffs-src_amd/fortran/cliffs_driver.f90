@@ -49,6 +49,7 @@ module drv_data
   logical :: cartesian, freeze
   integer :: nests, itopo, steps_total, quake, seaout, lonsub, latsub, bndout, maxout
   integer :: ngauges = 0, gout = 1
+  integer :: pline = 0   ! lines of the parameter file read so far
   integer, allocatable :: ig(:), jg(:)
   real(dp) :: cuke, ground, crough, dwall, dt
   ! master grid
@@ -106,15 +107,65 @@ contains
     call nc(nf_get_var_double(ncid, varid, out))
   end subroutine get_real8
 
+  ! One line of the parameter file: its leading value (list-directed, the
+  ! rest of the line is comment, as allabout.f reads it).  A missing or
+  ! malformed value sets err = -1 and names the line, where the reference's
+  ! reads would stop the program.
+  subroutine param_int(v, err)
+    integer, intent(out) :: v
+    integer, intent(inout) :: err
+    integer :: ios
+    v = 0
+    if (err /= 0) return
+    pline = pline + 1
+    read(PARU, *, iostat=ios) v
+    if (ios /= 0) call param_bad(err)
+  end subroutine param_int
+
+  subroutine param_real(v, err)
+    real(dp), intent(out) :: v
+    integer, intent(inout) :: err
+    integer :: ios
+    v = 0.0_dp
+    if (err /= 0) return
+    pline = pline + 1
+    read(PARU, *, iostat=ios) v
+    if (ios /= 0) call param_bad(err)
+  end subroutine param_real
+
+  subroutine param_name(v, err)
+    character(len=*), intent(out) :: v
+    integer, intent(inout) :: err
+    integer :: ios
+    v = ' '
+    if (err /= 0) return
+    pline = pline + 1
+    read(PARU, '(a)', iostat=ios) v
+    if (ios /= 0 .or. len_trim(v) == 0) call param_bad(err)
+  end subroutine param_name
+
+  subroutine param_bad(err)
+    integer, intent(inout) :: err
+    err = -1
+    write(LOGU, '(1x,a,i0)') 'parameter file: missing or malformed value on line ', pline
+  end subroutine param_bad
+
   ! parameter file and the grids it names (allabout.f:6-97)
   subroutine read_params(indir, pfile, err)
     character(len=*), intent(in) :: indir, pfile
     integer, intent(out) :: err
-    integer :: num, i
+    integer :: num, i, ios
     err = 0
+    pline = 0
     write(LOGU, '(1x,2a)') 'Read Computational parameters: ', trim(indir)//trim(pfile)
-    open(unit=PARU, file=trim(indir)//trim(pfile), status='old', form='formatted')
-    read(PARU, *) num
+    open(unit=PARU, file=trim(indir)//trim(pfile), status='old', form='formatted', iostat=ios)
+    if (ios /= 0) then
+      err = -1
+      write(LOGU, '(1x,2a)') 'parameter file not found: ', trim(indir)//trim(pfile)
+      return
+    end if
+    call param_int(num, err)
+    if (err /= 0) return
     cartesian = num == 1
     if (cartesian) then
       write(LOGU, *) 'CARTESIAN COORDINATE SYSTEM'
@@ -123,8 +174,14 @@ contains
     end if
     call read_master(indir, err)
     if (err /= 0) return
-    read(PARU, *) nests
+    call param_int(nests, err)
+    if (err /= 0) return
     write(LOGU, *) 'Number of grids enclosed in Master Grid: ', nests
+    if (nests < 0 .or. nests > 99) then
+      err = -1
+      write(LOGU, *) 'number of nested grids must be 0..99'
+      return
+    end if
     if (nests > 0 .and. (nx < 3 .or. ny < 3)) then
       err = -1
       write(LOGU, *) 'nesting in 1D is not permitted'
@@ -132,30 +189,34 @@ contains
     end if
     if (nests > 0) call read_nests(indir, err)
     if (err /= 0) return
-    read(PARU, *) cuke
-    read(PARU, *) ground
-    read(PARU, *) crough
-    read(PARU, *) itopo
-    read(PARU, *) dwall
-    read(PARU, *) dt
-    read(PARU, *) steps_total
-    read(PARU, *) quake
-    read(PARU, *) num
+    call param_real(cuke, err)
+    call param_real(ground, err)
+    call param_real(crough, err)
+    call param_int(itopo, err)
+    call param_real(dwall, err)
+    call param_real(dt, err)
+    call param_int(steps_total, err)
+    call param_int(quake, err)
+    call param_int(num, err)
     freeze = num == 0
-    read(PARU, *) seaout
-    read(PARU, *) lonsub
-    read(PARU, *) latsub
-    read(PARU, *) bndout
-    read(PARU, *) maxout
-    read(PARU, *) ngauges
-    if (ngauges >= 1) then
-      read(PARU, *) gout
+    call param_int(seaout, err)
+    call param_int(lonsub, err)
+    call param_int(latsub, err)
+    call param_int(bndout, err)
+    call param_int(maxout, err)
+    call param_int(ngauges, err)
+    if (err == 0 .and. ngauges >= 1) then
+      call param_int(gout, err)
       allocate(ig(ngauges), jg(ngauges))
       do i = 1, ngauges
-        read(PARU, *) ig(i), jg(i)
+        if (err /= 0) exit
+        pline = pline + 1
+        read(PARU, *, iostat=ios) ig(i), jg(i)
+        if (ios /= 0) call param_bad(err)
       end do
     end if
     close(PARU)
+    if (err /= 0) return
     write(LOGU, *) 'cuke, ground, crough:', cuke, ground, crough
     write(LOGU, *) 'dt, steps:', dt, steps_total, ' topo', itopo, ' quake', quake, ' freeze ', freeze
     write(LOGU, *) 'outputs every (steps): sea', seaout, ' feed', bndout, ' max', maxout, ' gauges', gout
@@ -170,7 +231,8 @@ contains
     integer :: ncid, i, j, ndims, nvar, ngatts, unlim, k1, k2
     real(dp) :: beta, alpha
     err = 0
-    read(PARU, '(a)') fname
+    call param_name(fname, err)
+    if (err /= 0) return
     k1 = index(fname, '/', .true.) + 1
     k2 = index(fname, '.nc') - 1
     gridname = fname(k1:k2)
@@ -226,7 +288,8 @@ contains
     integer :: ncid(99), ns, mx, my
     err = 0
     do ns = 1, nests
-      read(PARU, '(a)') fname
+      call param_name(fname, err)
+      if (err /= 0) return
       nestname(ns) = fname(1:index(fname, '.nc') - 1)
       call nc(nf_open(trim(indir)//trim(fname), NF_NOWRITE, ncid(ns)))
       call nc(nf_inq_dimlen(ncid(ns), 1, nxs(ns)))

@@ -47,7 +47,8 @@ typedef struct orc_grid {
  * iu=2: column irw along y.  1-based irw.  Returns the number of wet
  * segments found (kseg), or -1 if kseg exceeded the reference's fixed
  * 99-segment ghost storage (cliffs.f:26), where the reference's behaviour
- * is undefined; the line is then left unmodified. */
+ * is undefined.  The line is updated in either case: this restatement's
+ * ghost storage grows with the line, as the GPU path has no segment limit. */
 int orc_cliffs_line(orc_grid *g, int iu, int irw);
 
 /* Sweep lines first..last (1-based, inclusive), OpenMP over lines

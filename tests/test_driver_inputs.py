@@ -103,3 +103,45 @@ def test_missing_and_bad_boundary_feed(tmp_path):
         f.write("\n".join(lines) + "\n")
     rc, msg, log = _run([d + "c", d, "bnd", "0", d + "p2.txt"], d + "c_log.txt")
     assert rc == 1 and "bndr signal below threshold" in log
+
+
+def _params(d, edit):
+    """slpbeach_params.txt with its lines passed through edit(lines) -> lines."""
+    with open(os.path.join(SLP, "slpbeach_params.txt")) as f:
+        lines = f.read().splitlines()
+    with open(d + "p.txt", "w") as f:
+        f.write("\n".join(edit(lines)) + "\n")
+    return d + "p.txt"
+
+
+def test_parameter_file_errors(tmp_path):
+    """Missing, short and malformed parameter files end with a message that
+    names the line and status 1 (the reference's list-directed reads would
+    stop the program at the first of these)."""
+    d = str(tmp_path) + "/"
+    shutil.copy(os.path.join(SLP, "slpbeach1D.nc"), d)
+    args = lambda p: [d + "x", SLP + "/", "0", "solitary.0185", p]   # noqa: E731
+    rc, msg, log = _run(args(d + "nosuch.txt"), d + "x_log.txt")
+    assert rc == 1 and "parameter file not found" in log
+    rc, msg, log = _run(args(_params(d, lambda l: l[:9])), d + "x_log.txt")     # ends after dt
+    assert rc == 1 and "missing or malformed value on line 10" in log
+    rc, msg, log = _run(args(_params(d, lambda l: l[:8] + ["abc\ttime step"] + l[9:])), d + "x_log.txt")
+    assert rc == 1 and "missing or malformed value on line 9" in log
+    rc, msg, log = _run(args(_params(d, lambda l: l[:1] + [""] + l[2:])), d + "x_log.txt")  # no grid name
+    assert rc == 1 and "missing or malformed value on line 2" in log
+    # two gauges announced, one listed
+    rc, msg, log = _run(args(_params(d, lambda l: l[:17] + ["2\tN gages", "1\tevery", "10 1"])),
+                        d + "x_log.txt")
+    assert rc == 1 and "missing or malformed value on line 21" in log
+
+
+def test_nesting_limits(tmp_path):
+    """Nested grids inside a 1-D master grid (allabout.f:35-39) and more
+    nested grids than the nest tables hold are refused."""
+    d = str(tmp_path) + "/"
+    shutil.copy(os.path.join(SLP, "slpbeach1D.nc"), d)
+    args = lambda p: [d + "x", SLP + "/", "0", "solitary.0185", p]   # noqa: E731
+    rc, msg, log = _run(args(_params(d, lambda l: l[:2] + ["1\tnested grids"] + l[3:])), d + "x_log.txt")
+    assert rc == 1 and "nesting in 1D is not permitted" in log
+    rc, msg, log = _run(args(_params(d, lambda l: l[:2] + ["100\tnested grids"] + l[3:])), d + "x_log.txt")
+    assert rc == 1 and "must be 0..99" in log
