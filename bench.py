@@ -124,7 +124,49 @@ def cpu_baseline(case, steps):
                       f"{dt:.1f} s wall"}
 
 
-def roofline(cells, value, ms_x, ms_y, unfused, traffic_key):
+def copy_bandwidth(mib=1024, reps=5):
+    """GB/s of a plain device-to-device copy of `mib` MiB (bytes read plus
+    bytes written, per second; HIP events around `reps` hipMemcpyAsync on
+    the null stream after one untimed copy): the bandwidth a streaming
+    kernel can reach on this device, beside the 8 TB/s spec."""
+    import ctypes as C
+    try:
+        hip = C.CDLL("libamdhip64.so")
+    except OSError:
+        hip = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+    vp, n = C.c_void_p, C.c_size_t(mib << 20)
+    a, b, e0, e1, ms = vp(), vp(), vp(), vp(), C.c_float()
+
+    def chk(rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc})")
+    hip.hipMemcpyAsync.argtypes = [vp, vp, C.c_size_t, C.c_int, vp]
+    hip.hipMemset.argtypes = [vp, C.c_int, C.c_size_t]
+    hip.hipEventRecord.argtypes = [vp, vp]
+    chk(hip.hipMalloc(C.byref(a), n), "hipMalloc")
+    try:
+        chk(hip.hipMalloc(C.byref(b), n), "hipMalloc")
+        chk(hip.hipMemset(a, 1, n), "hipMemset")
+        chk(hip.hipEventCreate(C.byref(e0)), "hipEventCreate")
+        chk(hip.hipEventCreate(C.byref(e1)), "hipEventCreate")
+        chk(hip.hipMemcpyAsync(b, a, n, 3, None), "hipMemcpyAsync")      # 3: device to device
+        chk(hip.hipEventRecord(e0, None), "hipEventRecord")
+        for _ in range(reps):
+            chk(hip.hipMemcpyAsync(b, a, n, 3, None), "hipMemcpyAsync")
+        chk(hip.hipEventRecord(e1, None), "hipEventRecord")
+        chk(hip.hipEventSynchronize(e1), "hipEventSynchronize")
+        chk(hip.hipEventElapsedTime(C.byref(ms), e0, e1), "hipEventElapsedTime")
+    finally:
+        for e in (e0, e1):
+            if e.value:
+                hip.hipEventDestroy(e)
+        for q in (a, b):
+            if q.value:
+                hip.hipFree(q)
+    return 2 * (mib << 20) * reps / (ms.value * 1e-3) / 1e9
+
+
+def roofline(cells, value, ms_x, ms_y, unfused, traffic_key, copy_gbs=None):
     """The roofline object of the JSON line for a step timed at `value`
     Gcell-updates/s whose two kernels took ms_x and ms_y per launch (the
     x- and y-sweep, or the fused step and its flag-check kernel).
@@ -159,6 +201,8 @@ def roofline(cells, value, ms_x, ms_y, unfused, traffic_key):
         "step_bytes_per_cell": step_bytes,
         "step_frac": round(step_bytes * value / HBM_PEAK_GBS, 4),
         "step_frac_with_maxwave": round((step_bytes + MW_BYTES) * value / HBM_PEAK_GBS, 4),
+        "copy_gbs": round(copy_gbs, 1) if copy_gbs else None,
+        "frac_of_copy": round(ach / copy_gbs, 4) if copy_gbs else None,
     }
 
 
@@ -267,7 +311,9 @@ def main():
     g.close()
 
     traffic_key = "coastal_8192x6144" if args.coastal else f"open_ocean_{n}"
-    roof = roofline(nx * ny, value, ms_x, ms_y, args.unfused, traffic_key)
+    copy_gbs = copy_bandwidth()
+    log(f"[bench] device copy: {copy_gbs:.0f} GB/s")
+    roof = roofline(nx * ny, value, ms_x, ms_y, args.unfused, traffic_key, copy_gbs)
     ensemble = None
     if args.ensemble_scenarios > 0:
         ensemble = ensemble_bench(args.ensemble_scenarios, args.ensemble_steps, args.seed)

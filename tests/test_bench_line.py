@@ -30,6 +30,9 @@ def test_fused_roofline_counts_28_bytes():
     # committed rocprofv3 summary: HBM bytes per launch, max-wave included
     assert r["traffic"] == 715327488
     assert r["peak"] == 8000.0 and r["unit"] == "GB/s" and r["bound"] == "hbm"
+    assert r["copy_gbs"] is None and r["frac_of_copy"] is None
+    r = bench.roofline(CELLS, 70.0, 0.2345, 0.0076, False, "open_ocean_4096", copy_gbs=5000.0)
+    assert r["frac_of_copy"] == pytest.approx(r["achieved"] / 5000.0, abs=1e-4)
 
 
 def test_unfused_roofline_picks_the_slower_sweep():
