@@ -72,6 +72,7 @@ def load_library():
         "cliffs_run_gauges": ([H, C.c_int, C.c_int, C.c_int, _ip, _ip, C.POINTER(C.c_float)],
                               C.c_int),
         "cliffs_gauges": ([H, C.c_int, _ip, _ip, _dp, _dp, _dp], C.c_int),
+        "cliffs_health": ([H, _dp, C.POINTER(C.c_longlong)], C.c_int),
         "cliffs_ensemble_create": ([C.POINTER(H), C.c_int, C.POINTER(H)], C.c_int),
         "cliffs_ensemble_destroy": ([H], C.c_int),
         "cliffs_ensemble_run": ([H, C.c_int], C.c_int),
@@ -264,6 +265,13 @@ class Grid:
         _chk(load_library().cliffs_gauges(self._h, n, I.ctypes.data_as(_ip), J.ctypes.data_as(_ip),
                                           _p(eta), _p(u), _p(v)))
         return eta, u, v
+
+    def health(self):
+        """-> (largest Courant number over wet cells, count of cells with a
+        non-finite c, u or v) of the current state (cliffs_health)."""
+        cour, bad = C.c_double(), C.c_longlong()
+        _chk(load_library().cliffs_health(self._h, C.byref(cour), C.byref(bad)))
+        return cour.value, bad.value
 
     def sync(self):
         _chk(load_library().cliffs_sync(self._h))

@@ -23,6 +23,8 @@ hipError_t launch_maxwave(const DevGrid& g, hipStream_t st);
 hipError_t launch_copy_lines(const DevGrid& g, int dir, int first, int last, hipStream_t st);
 hipError_t launch_gauge_sample(const DevGrid& g, const Params& p, const int* I, const int* J, int n,
                                float* series, int slot, hipStream_t st);
+hipError_t launch_health(const DevGrid& g, const Params& p, const float4* xm, const float2* xr,
+                         const float4* ym, int xrun, int yrun, HealthOut* out, hipStream_t st);
 hipError_t launch_nest_arrival(const NestArgs& a, hipStream_t st);
 hipError_t launch_nest_feed(const NestArgs& a, hipStream_t st);
 hipError_t launch_child_edges(const double* r0, const double* r1, double cc, double omc, double grav,
@@ -98,6 +100,7 @@ struct cliffs_handle {
     float2* xr = nullptr;
     float4* ym = nullptr;
     float* zeta = nullptr;
+    HealthOut* health = nullptr;  // cliffs_health result (allocated on first use)
     float* edges = nullptr;    // kEdgeSlots x (6*nx + 6*ny) floats
     float* hedges = nullptr;   // pinned staging, same size
     hipEvent_t edge_ev[kEdgeSlots] = {};
@@ -192,6 +195,7 @@ void free_handle(cliffs_handle* h) {
     (void)hipFree(h->edges);
     (void)hipFree(h->flags);
     (void)hipFree(h->fflags);
+    (void)hipFree(h->health);
     if (h->hedges) (void)hipHostFree(h->hedges);
     if (h->st) (void)hipStreamDestroy(h->st);
     delete h;
@@ -896,6 +900,28 @@ int cliffs_run_gauges(cliffs_handle* h, int every, int nsamples, int n, const in
     cleanup();
     if (rc) return rc;
     if (e != hipSuccess) return hipfail(e, "gauge series");
+    return CLIFFS_OK;
+}
+
+int cliffs_health(cliffs_handle* h, double* courant, long long* nonfinite) {
+    if (!h || !courant || !nonfinite) return fail(CLIFFS_ERR_ARG, "NULL argument");
+    if (!h->state) return fail(CLIFFS_ERR_STATE, "no state uploaded");
+    HIPCHK(hipSetDevice(h->dev));
+    if (!h->health && hipMalloc(&h->health, sizeof(HealthOut)) != hipSuccess) {
+        h->health = nullptr;
+        return fail(CLIFFS_ERR_NOMEM, "health result allocation");
+    }
+    HIPCHK(hipMemsetAsync(h->health, 0, sizeof(HealthOut), h->st));
+    const hipError_t e = launch_health(h->grid(), h->p, h->xm, h->xr, h->ym, h->xrun ? 1 : 0,
+                                       h->yrun ? 1 : 0, h->health, h->st);
+    if (e != hipSuccess) return hipfail(e, "health");
+    HealthOut r;
+    HIPCHK(hipMemcpyAsync(&r, h->health, sizeof r, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    float c;
+    std::memcpy(&c, &r.cmax_bits, sizeof c);
+    *courant = c;
+    *nonfinite = (long long)r.nonfinite;
     return CLIFFS_OK;
 }
 

@@ -41,6 +41,13 @@ struct DevGrid {
     float* VM;
 };
 
+// Result of health_kernel (cliffs_health).
+struct HealthOut {
+    unsigned long long nonfinite;   // cells with a non-finite cp, u or v
+    unsigned cmax_bits;             // largest Courant number (float bits)
+    unsigned pad;
+};
+
 struct XArgs {
     DevGrid g;
     const float4* xm;     // per node i: (1/b(i-1), 1/b(i), 1/(b(i-1)+b(i)), b(i)-b(i-1))

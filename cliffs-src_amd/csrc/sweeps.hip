@@ -265,8 +265,57 @@ __global__ void gauge_sample_kernel(DevGrid g, Params p, const int* I, const int
     series[(size_t)slot * n + q] = is_land(c0_of(s) + cp, p) ? 0.0f : geta_of(s, cp) / p.grav;
 }
 
+// Health of the state the next step reads: the largest Courant number
+// (|u| + c) dt / dx over wet cells, in x where rows are swept and in y where
+// columns are (dx the smaller spacing next to the cell), and the number of
+// cells whose cp, u or v is not finite.  The reference checks neither.
+// Thread = column i, rows strided by gridDim.y; wave reduction by
+// butterfly shuffles, then one atomic per wave (Courant >= 0, so its bits
+// order as unsigned integers).
+__global__ void health_kernel(DevGrid g, Params p, const float4* __restrict__ xm,
+                              const float2* __restrict__ xr, const float4* __restrict__ ym, int xrun,
+                              int yrun, HealthOut* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float cmax = 0.0f;
+    unsigned nbad = 0u;
+    if (i < g.nx) {
+        const float4 mi = xm[i];
+        const float rdx = fmaxf(mi.x, mi.y);
+        for (int j = blockIdx.y; j < g.ny; j += gridDim.y) {
+            const size_t o = (size_t)j * g.pitch + i;
+            const float cp = g.CP[o], u = g.U[o], v = g.V[o];
+            if (!(isfinite(cp) && isfinite(u) && isfinite(v))) {
+                nbad++;
+                continue;
+            }
+            const float c = c0_of(g.S[o]) + cp;
+            if (is_land(c, p)) continue;
+            if (xrun) cmax = fmaxf(cmax, (fabsf(u) + c) * p.dt * (rdx * xr[j].x));
+            if (yrun) {
+                const float4 mj = ym[j];
+                cmax = fmaxf(cmax, (fabsf(v) + c) * p.dt * fmaxf(mj.x, mj.y));
+            }
+        }
+    }
+    for (int k = 32; k > 0; k >>= 1) {
+        cmax = fmaxf(cmax, __shfl_xor(cmax, k));
+        nbad += __shfl_xor(nbad, k);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&out->cmax_bits, __float_as_uint(cmax));
+        if (nbad) atomicAdd(&out->nonfinite, (unsigned long long)nbad);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
+hipError_t launch_health(const DevGrid& g, const Params& p, const float4* xm, const float2* xr,
+                         const float4* ym, int xrun, int yrun, HealthOut* out, hipStream_t st) {
+    const dim3 grid((g.nx + 255) / 256, g.ny < 256 ? g.ny : 256);
+    hipLaunchKernelGGL(health_kernel, grid, dim3(256), 0, st, g, p, xm, xr, ym, xrun, yrun, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_gauge_sample(const DevGrid& g, const Params& p, const int* I, const int* J, int n,
                                float* series, int slot, hipStream_t st) {
     hipLaunchKernelGGL(gauge_sample_kernel, dim3((n + 255) / 256), dim3(256), 0, st, g, p, I, J, n, series,

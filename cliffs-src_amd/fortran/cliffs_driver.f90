@@ -771,6 +771,7 @@ program cliffs_gpu_driver
   character(len=1000) :: notes
   integer :: nargs, k1, i, j, irec, istep, err, ns, mx, nxsub, nysub, nshot
   logical :: hin, uin, vin, bndr, ended
+  logical :: warned = .false.   ! non-finite state reported
   real(dp) :: t, cc, celmin, t0, t1
   real(dp), allocatable, target :: cel(:, :), xvel(:, :), yvel(:, :), hmax(:, :), velmax(:, :)
   real(dp), allocatable, target :: xe1(:, :), xe2(:, :), ye1(:, :), ye2(:, :)
@@ -902,6 +903,7 @@ program cliffs_gpu_driver
   ! outputs and their first records (cliffs_main.f:229-285)
   call open_max(trim(casename)//'_maxwave.nc', fmax)
   nshot = 0
+  istep = 0                         ! the outputs at the start time
   if (seaout <= steps_total) then
     nxsub = (nx - 1) / lonsub + 1
     nysub = (ny - 1) / latsub + 1
@@ -1023,7 +1025,21 @@ contains
     if (xrun) call write_sea(fsea(2), t, snap(:, :, 2))
     if (yrun) call write_sea(fsea(3), t, snap(:, :, 3))
     write(LOGU, *) 'screenshot', nshot, 'at', t, 'sec'
+    call health()
   end subroutine snapshot
+
+  ! largest Courant number and non-finite cells of the state (cliffs_health);
+  ! the reference checks neither, so a blow-up is reported, not acted on
+  subroutine health()
+    real(c_double) :: courant
+    integer(c_long_long) :: nbad
+    call gpu(cliffs_health(h, courant, nbad), 'cliffs_health')
+    write(LOGU, '(1x,a,f8.4)') 'max Courant number ', courant
+    if (nbad > 0 .and. .not. warned) then
+      write(LOGU, '(1x,a,i0,a,i0)') 'WARNING: state not finite in ', nbad, ' cells at step ', istep
+      warned = .true.
+    end if
+  end subroutine health
 
   subroutine gauges()
     call gpu(cliffs_gauges(h, int(ngauges, c_int), ig, jg, geta, gu, gv), 'cliffs_gauges')

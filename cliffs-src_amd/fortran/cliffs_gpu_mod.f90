@@ -29,7 +29,7 @@ module cliffs_gpu_mod
   public :: cliffs_create, cliffs_destroy, cliffs_upload_state, cliffs_download_state
   public :: cliffs_download_max, cliffs_reset_max, cliffs_sweep, cliffs_maxwave
   public :: cliffs_step, cliffs_run, cliffs_run_timed, cliffs_gauges, cliffs_sync
-  public :: cliffs_stream, cliffs_info, cliffs_last_error
+  public :: cliffs_stream, cliffs_info, cliffs_last_error, cliffs_health
   public :: cliffs_nest_maps, cliffs_nest_create, cliffs_nest_destroy, cliffs_nest_feed
   public :: cliffs_gpu_open, cliffs_error_message
 
@@ -94,6 +94,12 @@ module cliffs_gpu_mod
       integer(c_int), value :: n
       integer(c_int), intent(in) :: ig(*), jg(*)
       real(c_double), intent(out) :: eta(*), u(*), v(*)
+    end function
+    integer(c_int) function cliffs_health(h, courant, nonfinite) bind(C, name="cliffs_health")
+      import :: c_int, c_ptr, c_double, c_long_long
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: courant
+      integer(c_long_long), intent(out) :: nonfinite
     end function
     integer(c_int) function cliffs_sync(h) bind(C, name="cliffs_sync")
       import :: c_int, c_ptr

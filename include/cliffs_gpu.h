@@ -229,6 +229,15 @@ int cliffs_nest_end_feed(cliffs_nest *n);
  * the feed has not ended. */
 int cliffs_nest_child_step(cliffs_nest *n, double t);
 
+/* Health of the current state: *courant = the largest Courant number
+ * (|u| + c) dt / dx over wet cells (x along rows where the x-sweep runs, y
+ * along columns where the y-sweep runs, dx the smaller spacing next to the
+ * cell), *nonfinite = the number of cells whose c, u or v is NaN or
+ * infinite.  One reduction over the state on the device; waits for it.  No
+ * counterpart in the reference, which checks neither (cliffs_main.f
+ * integrates with a fixed dt); for drivers that want to detect a blow-up. */
+int cliffs_health(cliffs_handle *h, double *courant, long long *nonfinite);
+
 int cliffs_sync(cliffs_handle *h);
 /* The handle's hipStream_t, for callers that enqueue their own work. */
 int cliffs_stream(cliffs_handle *h, void **stream);

@@ -17,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def test_library_exports_every_header_symbol():
     lib = cliffs_gpu.load_library()
     names = cliffs_gpu.header_symbols()
-    assert len(names) == 34
+    assert len(names) == 35
     for n in names:
         assert hasattr(lib, n), n
     out = subprocess.run(["nm", "-D", "--defined-only", cliffs_gpu.LIB_PATH], check=True,
@@ -47,6 +47,8 @@ def test_argument_errors_without_compute():
     assert b"NULL" in lib.cliffs_last_error()
     assert lib.cliffs_sync(None) == -1
     assert lib.cliffs_run(None, 1) == -1
+    cour, bad = C.c_double(), C.c_longlong()
+    assert lib.cliffs_health(None, C.byref(cour), C.byref(bad)) == -1
     assert lib.cliffs_destroy(None) == 0
 
 

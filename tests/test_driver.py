@@ -64,7 +64,12 @@ def _same(a, b, what):
 
 def test_slpbeach_against_oracle(tmp_path, gpu_lib):
     out = str(tmp_path) + "/"
-    _drive([out + "slp", SLP + "/", "0", "solitary.0185", SLP + "/slpbeach_params.txt"], out + "slp_log.txt")
+    log = _drive([out + "slp", SLP + "/", "0", "solitary.0185", SLP + "/slpbeach_params.txt"],
+                 out + "slp_log.txt")
+    # each screenshot logs the state's largest Courant number (cliffs_health)
+    cour = [float(l.split()[-1]) for l in log.splitlines() if "max Courant number" in l]
+    assert len(cour) == 2000 // 3 + 1 and 0.0 < max(cour) < 1.0
+    assert "not finite" not in log
     case = slpbeach_case("0185")
     amp = 0.0185 * case["dep"][0, -1]
     o = O.Oracle(case["dep"], case["s1"], case["s2"], case["zeta"], dt=case["dt"], ground=case["ground"])
