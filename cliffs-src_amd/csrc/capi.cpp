@@ -3,11 +3,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/cliffs_gpu.h"
@@ -23,6 +26,7 @@ hipError_t launch_maxwave(const DevGrid& g, hipStream_t st);
 hipError_t launch_copy_lines(const DevGrid& g, int dir, int first, int last, hipStream_t st);
 hipError_t launch_gauge_sample(const DevGrid& g, const Params& p, const int* I, const int* J, int n,
                                float* series, int slot, hipStream_t st);
+hipError_t launch_reset_max(const DevGrid& g, hipStream_t st);
 hipError_t launch_health(const DevGrid& g, const Params& p, const float4* xm, const float2* xr,
                          const float4* ym, int xrun, int yrun, HealthOut* out, hipStream_t st);
 hipError_t launch_nest_arrival(const NestArgs& a, hipStream_t st);
@@ -201,24 +205,63 @@ void free_handle(cliffs_handle* h) {
     delete h;
 }
 
-// pitched FP32 host buffer <-> device field
+// pitched FP32 host buffer <-> device field: host and device rows have the
+// same pitch, so the field is one contiguous block (a 2-D copy from pageable
+// memory would be staged row by row)
 int put_field(cliffs_handle* h, float* dst, const std::vector<float>& src) {
-    HIPCHK(hipMemcpy2DAsync(dst, (size_t)h->pitch * sizeof(float), src.data(),
-                            (size_t)h->pitch * sizeof(float), (size_t)h->pitch * sizeof(float),
-                            (size_t)h->ny, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(dst, src.data(), (size_t)h->pitch * h->ny * sizeof(float),
+                          hipMemcpyHostToDevice, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     return CLIFFS_OK;
 }
 int get_field(cliffs_handle* h, const float* src, std::vector<float>& dst) {
     dst.resize((size_t)h->pitch * h->ny);
-    HIPCHK(hipMemcpy2DAsync(dst.data(), (size_t)h->pitch * sizeof(float), src,
-                            (size_t)h->pitch * sizeof(float), (size_t)h->pitch * sizeof(float),
-                            (size_t)h->ny, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipMemcpyAsync(dst.data(), src, (size_t)h->pitch * h->ny * sizeof(float),
+                          hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     return CLIFFS_OK;
 }
 inline size_t hidx(const cliffs_handle* h, int i, int j) {
     return (size_t)j * h->pitch + i;
+}
+
+// Host threads for the per-cell conversions of uploads and downloads: up to
+// 16 (fewer if OMP_NUM_THREADS says so).
+int host_threads() {
+    static const int n = [] {
+        int t = (int)std::thread::hardware_concurrency();
+        if (t < 1) t = 1;
+        if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+            const int v = std::atoi(e);
+            if (v >= 1) t = std::min(t, v);
+        }
+        return std::min(t, 16);
+    }();
+    return n;
+}
+
+// f(j0, j1) over the rows [0, ny) of a grid of `cells` cells.  Cells convert
+// independently of each other, so the rows are split over host threads;
+// grids under 2^20 cells stay on the calling thread.
+template <class F>
+void for_rows(int ny, size_t cells, const F& f) {
+    const int nt = std::min(host_threads(), ny);
+    if (cells < ((size_t)1 << 20) || nt <= 1) {
+        f(0, ny);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve((size_t)nt - 1);
+    for (int t = 1; t < nt; t++) {
+        const int j0 = (int)((long)ny * t / nt), j1 = (int)((long)ny * (t + 1) / nt);
+        try {
+            th.emplace_back([&f, j0, j1] { f(j0, j1); });
+        } catch (...) {
+            f(j0, j1);  // no thread available: convert these rows here
+        }
+    }
+    f(0, (int)((long)ny / nt));
+    for (auto& x : th) x.join();
 }
 
 // Edge arrays (mx,3) FP64 -> device [6][n] (ub, vb, g*eta for edge1, then
@@ -531,9 +574,14 @@ int cliffs_create(const cliffs_grid_desc* d, cliffs_handle** out) {
         return bail(fail(CLIFFS_ERR_HIP, "hipStreamCreate"));
 
     const size_t m = (size_t)nx * ny;
-    h->dep.assign(d->dep, d->dep + m);
+    h->dep.resize(m);
     h->s.resize(m);
-    for (size_t c = 0; c < m; c++) h->s[c] = to_s(h->dep[c], h->k);
+    for_rows(ny, m, [&](int j0, int j1) {
+        for (size_t c = (size_t)j0 * nx; c < (size_t)j1 * nx; c++) {
+            h->dep[c] = d->dep[c];
+            h->s[c] = to_s(d->dep[c], h->k);
+        }
+    });
 
     // ---- metrics.  x: s1(i,j) = a(j) b(i)  (MasterGrid.f:86-99)
     std::vector<float4> xm(nx, make_float4(0.f, 0.f, 0.f, 0.f));
@@ -544,12 +592,16 @@ int cliffs_create(const cliffs_grid_desc* d, cliffs_handle** out) {
         std::vector<double> b(l), aj(ny);
         for (int i = 0; i < l; i++) b[i] = s1[i];
         for (int j = 0; j < ny; j++) aj[j] = s1[(size_t)l * j] / b[0];
-        for (int j = 0; j < ny; j++)
-            for (int i = 0; i < l; i++) {
-                const double v = s1[i + (size_t)l * j];
-                if (!(v > 0.0) || std::fabs(aj[j] * b[i] - v) > 1e-9 * v)
-                    return bail(fail(CLIFFS_ERR_METRIC, "s1(i,j) must be positive and separable a(j)*b(i)"));
-            }
+        std::atomic<bool> separable{true};
+        for_rows(ny, m, [&](int j0, int j1) {
+            for (int j = j0; j < j1; j++)
+                for (int i = 0; i < l; i++) {
+                    const double v = s1[i + (size_t)l * j];
+                    if (!(v > 0.0) || std::fabs(aj[j] * b[i] - v) > 1e-9 * v) separable = false;
+                }
+        });
+        if (!separable)
+            return bail(fail(CLIFFS_ERR_METRIC, "s1(i,j) must be positive and separable a(j)*b(i)"));
         for (int i = 0; i < nx; i++) {
             const double bm = i >= 1 ? b[i - 1] : b[0];
             const double bp = i < l ? b[i] : b[l - 1];
@@ -615,8 +667,10 @@ int cliffs_create(const cliffs_grid_desc* d, cliffs_handle** out) {
 
     // background celerity
     h->hbuf.assign((size_t)h->pitch * ny, 0.0f);
-    for (int j = 0; j < ny; j++)
-        for (int i = 0; i < nx; i++) h->hbuf[hidx(h, i, j)] = h->s[(size_t)i + (size_t)nx * j];
+    for_rows(ny, m, [&](int j0, int j1) {
+        for (int j = j0; j < j1; j++)
+            for (int i = 0; i < nx; i++) h->hbuf[hidx(h, i, j)] = h->s[(size_t)i + (size_t)nx * j];
+    });
     int rc = put_field(h, h->grid().S, h->hbuf);
     if (rc) return bail(rc);
     rc = cliffs_reset_max(h);
@@ -636,20 +690,24 @@ int cliffs_upload_state(cliffs_handle* h, const double* cel, const double* xvel,
     HIPCHK(hipSetDevice(h->dev));
     const int nx = h->nx, ny = h->ny;
     std::vector<float>& b = h->hbuf;
+    const size_t m = (size_t)nx * ny;
     b.assign((size_t)h->pitch * ny, 0.0f);
-    for (int j = 0; j < ny; j++)
-        for (int i = 0; i < nx; i++) {
-            const size_t c = (size_t)i + (size_t)nx * j;
-            b[hidx(h, i, j)] = to_cp(cel[c], h->dep[c], h->s[c], h->k);
-        }
+    for_rows(ny, m, [&](int j0, int j1) {
+        for (int j = j0; j < j1; j++)
+            for (int i = 0; i < nx; i++) {
+                const size_t c = (size_t)i + (size_t)nx * j;
+                b[hidx(h, i, j)] = to_cp(cel[c], h->dep[c], h->s[c], h->k);
+            }
+    });
     int rc = put_field(h, h->grid().CP, b);
     if (rc) return rc;
-    for (int j = 0; j < ny; j++)
-        for (int i = 0; i < nx; i++) b[hidx(h, i, j)] = (float)xvel[(size_t)i + (size_t)nx * j];
-    if ((rc = put_field(h, h->grid().U, b))) return rc;
-    for (int j = 0; j < ny; j++)
-        for (int i = 0; i < nx; i++) b[hidx(h, i, j)] = (float)yvel[(size_t)i + (size_t)nx * j];
-    if ((rc = put_field(h, h->grid().V, b))) return rc;
+    for (const double* f : {xvel, yvel}) {
+        for_rows(ny, m, [&](int j0, int j1) {
+            for (int j = j0; j < j1; j++)
+                for (int i = 0; i < nx; i++) b[hidx(h, i, j)] = (float)f[(size_t)i + (size_t)nx * j];
+        });
+        if ((rc = put_field(h, f == xvel ? h->grid().U : h->grid().V, b))) return rc;
+    }
     h->state = true;
     h->flags_valid = false;
     h->fused_valid = false;
@@ -661,19 +719,23 @@ int cliffs_download_state(cliffs_handle* h, double* cel, double* xvel, double* y
     HIPCHK(hipSetDevice(h->dev));
     const int nx = h->nx, ny = h->ny;
     std::vector<float> b;
+    const size_t m = (size_t)nx * ny;
     int rc = get_field(h, h->grid().CP, b);
     if (rc) return rc;
-    for (int j = 0; j < ny; j++)
-        for (int i = 0; i < nx; i++) {
-            const size_t c = (size_t)i + (size_t)nx * j;
-            cel[c] = to_cel(h->s[c], b[hidx(h, i, j)], h->dep[c], h->k);
-        }
-    if ((rc = get_field(h, h->grid().U, b))) return rc;
-    for (int j = 0; j < ny; j++)
-        for (int i = 0; i < nx; i++) xvel[(size_t)i + (size_t)nx * j] = b[hidx(h, i, j)];
-    if ((rc = get_field(h, h->grid().V, b))) return rc;
-    for (int j = 0; j < ny; j++)
-        for (int i = 0; i < nx; i++) yvel[(size_t)i + (size_t)nx * j] = b[hidx(h, i, j)];
+    for_rows(ny, m, [&](int j0, int j1) {
+        for (int j = j0; j < j1; j++)
+            for (int i = 0; i < nx; i++) {
+                const size_t c = (size_t)i + (size_t)nx * j;
+                cel[c] = to_cel(h->s[c], b[hidx(h, i, j)], h->dep[c], h->k);
+            }
+    });
+    for (double* f : {xvel, yvel}) {
+        if ((rc = get_field(h, f == xvel ? h->grid().U : h->grid().V, b))) return rc;
+        for_rows(ny, m, [&](int j0, int j1) {
+            for (int j = j0; j < j1; j++)
+                for (int i = 0; i < nx; i++) f[(size_t)i + (size_t)nx * j] = b[hidx(h, i, j)];
+        });
+    }
     return CLIFFS_OK;
 }
 
@@ -682,30 +744,30 @@ int cliffs_download_max(cliffs_handle* h, double* hmax, double* velmax) {
     HIPCHK(hipSetDevice(h->dev));
     const int nx = h->nx, ny = h->ny;
     std::vector<float> b;
+    const size_t m = (size_t)nx * ny;
     int rc = get_field(h, h->grid().HM, b);
     if (rc) return rc;
-    for (int j = 0; j < ny; j++)
-        for (int i = 0; i < nx; i++) {
-            const size_t c = (size_t)i + (size_t)nx * j;
-            hmax[c] = to_cel(h->s[c], b[hidx(h, i, j)], h->dep[c], h->k);
-        }
+    for_rows(ny, m, [&](int j0, int j1) {
+        for (int j = j0; j < j1; j++)
+            for (int i = 0; i < nx; i++) {
+                const size_t c = (size_t)i + (size_t)nx * j;
+                hmax[c] = to_cel(h->s[c], b[hidx(h, i, j)], h->dep[c], h->k);
+            }
+    });
     if ((rc = get_field(h, h->grid().VM, b))) return rc;
-    for (int j = 0; j < ny; j++)
-        for (int i = 0; i < nx; i++) velmax[(size_t)i + (size_t)nx * j] = b[hidx(h, i, j)];
+    for_rows(ny, m, [&](int j0, int j1) {
+        for (int j = j0; j < j1; j++)
+            for (int i = 0; i < nx; i++) velmax[(size_t)i + (size_t)nx * j] = b[hidx(h, i, j)];
+    });
     return CLIFFS_OK;
 }
 
 int cliffs_reset_max(cliffs_handle* h) {
     if (!h) return fail(CLIFFS_ERR_ARG, "NULL handle");
     HIPCHK(hipSetDevice(h->dev));
-    // hmax = 0 means c = 0, i.e. cp = -max(s,0)
-    std::vector<float>& b = h->hbuf;
-    b.assign((size_t)h->pitch * h->ny, 0.0f);
-    for (int j = 0; j < h->ny; j++)
-        for (int i = 0; i < h->nx; i++) b[hidx(h, i, j)] = -c0f(h->s[(size_t)i + (size_t)h->nx * j]);
-    int rc = put_field(h, h->grid().HM, b);
-    if (rc) return rc;
-    HIPCHK(hipMemsetAsync(h->grid().VM, 0, h->fstride * sizeof(float), h->st));
+    // hmax = 0 means c = 0, i.e. cp = -max(s,0), from the device's S
+    const hipError_t e = launch_reset_max(h->grid(), h->st);
+    if (e != hipSuccess) return hipfail(e, "reset_max");
     HIPCHK(hipStreamSynchronize(h->st));
     return CLIFFS_OK;
 }

@@ -307,8 +307,26 @@ __global__ void health_kernel(DevGrid g, Params p, const float4* __restrict__ xm
     }
 }
 
+// Running maxima reset (cliffs_reset_max): hmax to c = 0, i.e. cp = -max(s, 0)
+// (the expression of the host conversion, exact in FP32), velmax to 0; the
+// row padding beyond nx to 0.
+__global__ void reset_max_kernel(DevGrid g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= g.pitch) return;
+    const size_t o = (size_t)j * g.pitch + i;
+    const float s = g.S[o];
+    g.HM[o] = i < g.nx ? -(s > 0.0f ? s : 0.0f) : 0.0f;
+    g.VM[o] = 0.0f;
+}
+
 // ---------------------------------------------------------------------------
 // launchers
+hipError_t launch_reset_max(const DevGrid& g, hipStream_t st) {
+    hipLaunchKernelGGL(reset_max_kernel, dim3((g.pitch + 255) / 256, g.ny), dim3(256), 0, st, g);
+    return hipGetLastError();
+}
+
 hipError_t launch_health(const DevGrid& g, const Params& p, const float4* xm, const float2* xr,
                          const float4* ym, int xrun, int yrun, HealthOut* out, hipStream_t st) {
     const dim3 grid((g.nx + 255) / 256, g.ny < 256 ? g.ny : 256);
