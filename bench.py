@@ -72,21 +72,31 @@ def ensemble_bench(nscen, steps, seed0):
         cases = pool.map(functools.partial(synth.ensemble_scenario, steps=steps),
                          range(seed0, seed0 + nscen), chunksize=4)
     cells = sum(c.dep.size for c in cases)
-    t0 = time.perf_counter()
-    grids = []
-    for c in cases:
+    cliffs_gpu.load_library()
+
+    def make(c):
         s1, s2, zeta = c.metrics()
         g = cliffs_gpu.Grid(c.dep, s1, s2, zeta, dt=c.dt, ground=c.ground, crough=c.crough)
         g.upload(c.cel, c.xvel, c.yvel)
-        grids.append(g)
+        return g
+
+    def runup(gc):
+        g, c = gc
+        return round(synth.max_runup(g.download_max()[0], c.dep, g.celmin), 2)
+
+    # grids are created, uploaded and read back from host threads, one grid
+    # (handle) per task: the library's calls release the interpreter lock, so
+    # one grid's conversions overlap another's copies
+    from concurrent.futures import ThreadPoolExecutor
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        grids = list(ex.map(make, cases))
     ens = cliffs_gpu.Ensemble(grids)
     rows = ens.tune(2)
     log(f"[bench] ensemble: {cells / 1e6:.1f} M cells, tile height {rows}, {steps} steps")
     ms = ens.run_timed(steps)
-    runups = []
-    for c, g in zip(cases, grids):
-        hmax, _ = g.download_max()
-        runups.append(round(synth.max_runup(hmax, c.dep, g.celmin), 2))
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        runups = list(ex.map(runup, zip(grids, cases)))
     wall = time.perf_counter() - t0
     ens.close()
     for g in grids:
