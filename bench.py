@@ -111,6 +111,67 @@ def ensemble_bench(nscen, steps, seed0):
                       f"(within one cell)"}
 
 
+def nested_bench(steps, warmup):
+    """Full-size nested workload: the spherical 8192 x 6144 coastal parent
+    with run-up and a 2047^2 child at 3x on its coast, coupled on the device
+    (a feed record after every parent step, cliffs_nest_record; the child
+    stepped at dt/3 behind the feed, cliffs_nest_child_step, on its own
+    stream).  Times `steps` parent steps with their records and child steps
+    after `warmup` such cycles; value counts parent and child cell-updates."""
+    import numpy as np
+    import cliffs_gpu
+    import synth
+    log("[bench] nested: generating the 8192x6144 parent and its 2047^2 child")
+    case, xc, yc, depc = synth.coastal_nest()
+    maps = cliffs_gpu.nest_maps(case.xcrd, case.ycrd, xc, yc)
+    s1, s2, zeta = case.metrics()
+    g = cliffs_gpu.Grid(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground, device=0)
+    g.upload(case.cel, case.xvel, case.yvel)
+    cs1, cs2, czeta = synth.metrics(xc, yc, False)
+    c = cliffs_gpu.Grid(depc, cs1, cs2, czeta, dt=case.dt / 3, ground=case.ground, device=0)
+    z = np.zeros_like(depc)
+    c.upload(synth.depth_to_celerity(depc, case.ground), z, z)
+    nest = cliffs_gpu.Nest(g, maps)
+    nest.couple(c, capacity=8)
+    st = {"tp": 0.0, "iarv": False, "trec": [], "kc": 0}
+
+    def cycles(n):
+        """n parent steps, each followed by its feed record and the child
+        steps that stay behind the last record; -> child steps taken."""
+        taken = 0
+        for _ in range(n):
+            g.run(1)
+            st["tp"] += case.dt
+            st["iarv"], nrec = nest.record(0.01, case.ground, st["tp"], st["iarv"])
+            if nrec > len(st["trec"]):
+                st["trec"].append(st["tp"])
+            while st["kc"] + 1 < 3 * (len(st["trec"]) - 1):
+                st["kc"] += 1
+                j = st["kc"] // 3
+                nest.child_step(st["trec"][j] + (st["kc"] - 3 * j) * case.dt / 3)
+                taken += 1
+        return taken
+
+    cycles(warmup)
+    g.sync()
+    c.sync()
+    t0 = time.perf_counter()
+    nchild = cycles(steps)
+    g.sync()
+    c.sync()
+    wall = time.perf_counter() - t0
+    finite = bool(np.isfinite(c.download()[0]).all() and np.isfinite(g.download()[0]).all())
+    _, ms_p, ms_fix = g.run_timed(200)    # the parent's step kernels alone, for their roofline
+    cells_p, cells_c = case.dep.size, depc.size
+    value = (cells_p * steps + cells_c * nchild) / wall / 1e9
+    log(f"[bench] nested: {steps} parent steps and {nchild} child steps in {wall:.3f} s, "
+        f"{value:.2f} Gcell-updates/s")
+    nest.close()
+    c.close()
+    g.close()
+    return value, wall, nchild, (ms_p, ms_fix), cells_p, cells_c, case.dt, finite
+
+
 def cpu_baseline(case, steps):
     """FP64 C restatement (oracle/) on the host cores, on a bounded sample of
     the same workload: `steps` steps of the full grid."""
@@ -247,6 +308,9 @@ def main():
                     help="instead of the open ocean: the spherical 8192 x 6144 coastal grid "
                          "(30 arc-seconds, shelf, coastline, run-up), whose tiles take the "
                          "wet/dry paths")
+    ap.add_argument("--nested", action="store_true",
+                    help="the full-size nested workload instead: the 8192 x 6144 coastal "
+                         "parent with a 2047^2 child at 3x coupled on the device")
     ap.add_argument("--unfused", action="store_true",
                     help="run the step as two sweep kernels instead of one fused kernel")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -282,6 +346,19 @@ def main():
     import numpy as np
     import cliffs_gpu
     import synth
+
+    if args.nested:
+        value, wall, nchild, ms_p, cells_p, cells_c, dt, finite = nested_bench(args.steps, args.warmup)
+        roof = roofline(cells_p, cells_p / (ms_p[0] * 1e-3) / 1e9, ms_p[0], ms_p[1], False,
+                        "coastal_8192x6144", copy_bandwidth())
+        roof["kernel"] = "step_fused (parent alone)"
+        out = bench_line(value, args.steps, args.warmup, wall,
+                         f"nested: 2D spherical coastal parent 8192x6144 (30 arc-seconds, run-up, "
+                         f"seed 3) and a 2047x2047 child at 3x on its coast, coupled on the device; "
+                         f"{nchild} child steps", 8192, 6144, dt, finite, roof, None, None)
+        out["config"]["child"] = {"nx": 2047, "ny": 2047, "steps": nchild}
+        print(json.dumps(out), flush=True)
+        return
 
     n = args.n
     if args.coastal:

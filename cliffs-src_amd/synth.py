@@ -191,6 +191,23 @@ def coastal(nx=256, ny=192, seed=0, cartesian=True, dx=500.0, steps=200, source_
                 steps, dict(seed=seed, amplitude=source_amp))
 
 
+def coastal_nest(seed=3, i0=5350, j0=3000, n=682, refine=3):
+    """The spherical 8192 x 6144 coastal parent (30 arc-seconds from 25 S,
+    run-up) and a child grid refine x finer over parent cells i0..i0+n,
+    j0..j0+n on its coast (a (refine*n+1)^2 lattice, 2047^2 by default),
+    its depth bilinear in the parent's.  -> (parent case, xc, yc, depc)."""
+    from scipy.interpolate import RegularGridInterpolator
+    case = coastal(nx=8192, ny=6144, seed=seed, cartesian=False, lat0=-25.0)
+    x, y = case.xcrd, case.ycrd
+    xc = np.linspace(x[i0], x[i0 + n], refine * n + 1)
+    yc = np.linspace(y[j0], y[j0 + n], refine * n + 1)
+    interp = RegularGridInterpolator((y[j0:j0 + n + 1], x[i0:i0 + n + 1]),
+                                     case.dep[j0:j0 + n + 1, i0:i0 + n + 1])
+    Yc, Xc = np.meshgrid(yc, xc, indexing="ij")
+    depc = interp(np.stack([Yc, Xc], axis=-1))
+    return case, xc, yc, depc
+
+
 def ssl_condition(dep, depmin=1.0, alpha=2.0, passes=50):
     """Depth conditioning in the manner of depth_ssl.f:236-290: in open water
     (depth >= depmin, both neighbours too) a node whose sqrt-depth differs

@@ -14,7 +14,6 @@ the oracle to run the whole thing where it cannot:
 """
 import numpy as np
 import pytest
-from scipy.interpolate import RegularGridInterpolator
 
 import oracle as O
 import synth
@@ -83,16 +82,9 @@ def _host_edges(recs, state, t, mx):
 
 
 def test_c3_nest_full_size(gpu_lib):
-    case = synth.coastal(nx=8192, ny=6144, seed=3, cartesian=False, lat0=-25.0)
+    # child: parent cells 5350..6032 x 3000..3682 (coast) at 3x, 2047^2
+    case, xc, yc, depc = synth.coastal_nest()
     x, y = case.xcrd, case.ycrd
-    i0, j0, n = 5350, 3000, 682                   # child: parent cells i0..i0+n, j0..j0+n (coast)
-    xc = np.linspace(x[i0], x[i0 + n], 3 * n + 1)
-    yc = np.linspace(y[j0], y[j0 + n], 3 * n + 1)
-    interp = RegularGridInterpolator((y[j0:j0 + n + 1], x[i0:i0 + n + 1]),
-                                     case.dep[j0:j0 + n + 1, i0:i0 + n + 1])
-    Yc, Xc = np.meshgrid(yc, xc, indexing="ij")
-    depc = interp(np.stack([Yc, Xc], axis=-1))
-    del Yc, Xc
     maps = gpu_lib.nest_maps(x, y, xc, yc)
     s1, s2, zeta = case.metrics()
     cs1, cs2, czeta = synth.metrics(xc, yc, False)
