@@ -8,7 +8,7 @@ an ensemble with device gauge recording for the inputs, the coupled
 parent/child nested solve for the targets), permutes maps to and from the
 network's quadtree order, trains, and gates predictions: two independently
 trained networks must agree, otherwise the scenario is solved with the full
-nested solve (the fallback and the judge of the surrogate).
+nested solve (the fallback, and the answer its error is measured against).
 """
 from __future__ import annotations
 

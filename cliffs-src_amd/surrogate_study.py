@@ -1,4 +1,4 @@
-"""Train and judge the learned nest surrogate on solver-generated scenarios.
+"""Train and evaluate the learned nest surrogate on solver-generated scenarios.
 
     python3 cliffs-src_amd/surrogate_study.py [--train 896] [--calib 128] [--test 128]
 
