@@ -4,8 +4,9 @@
  * A small network that maps the surface-elevation series of 64 gauges at a
  * nested grid's boundary (256 samples each) to the grid's 64 x 64 map of
  * maximum inundation, trained on runs the solver itself generates.  The full
- * nested solve is its fallback and the answer its error is measured against.  No counterpart
- * in the reference (Cliffs has no learned component).
+ * nested solve is its fallback and the answer its error is measured
+ * against.  No counterpart in the reference (Cliffs has no learned
+ * component).
  *
  * Every layer is a GEMM on the MFMA units: BF16 operands, FP32 accumulation,
  * hand-written kernels for the forward pass, the backward pass and Adam.
