@@ -64,7 +64,10 @@ __device__ __forceinline__ int tile_of_block(int b, int ntiles) {
 struct FShared {
     float row[2][4][4][256];      // [buffer][row of the group][s, cp, u, v][window node]
     float4 xmt[256];              // x metric of the tile's window nodes
-    unsigned rowany[kFMaxRows];   // row of the tile has a wet node after the step
+    // [wave][row of the tile]: the wave's columns of the row have a wet node
+    // after the step.  Each wave writes each row of the tile exactly once
+    // (all lanes, the same value): no clearing, no atomics.
+    unsigned char rowany[4][kFMaxRows];
 };
 
 // Field access by 32-bit element offset from the common base: global
@@ -133,8 +136,7 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
     YPipe pipe;
     pipe.init();
     int colwet_f = 0;  // bit 0: a node of the column is wet after the step; bit 1: deferred change
-    for (int k = t; k < j1 - j0; k += 256) sh.rowany[k] = 0u;
-    __syncthreads();  // xmt and rowany are complete
+    __syncthreads();  // xmt is complete
 
     // rows j0-3 .. j1+2 stream through; group g holds stream rows 4g .. 4g+3
     const int nstream = j1 - j0 + 6;
@@ -236,7 +238,7 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
                         wet = !is_land(c0_of(f.s) + f.cp, p);
                         colwet_f |= wet ? 1 : 0;
                     }
-                    if (__ballot(wet) != 0 && lane == 0) atomicOr(&sh.rowany[r - j0], 1u);
+                    sh.rowany[w][r - j0] = __ballot(wet) != 0 ? 1 : 0;
                 }
             }
         }
@@ -245,7 +247,8 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
     }
     const int tile_mchange = __syncthreads_or((colwet_f & 2) && loadmax && !keepmax);
     if (!kExact && t == 0) a.mpend[tile] = tile_mchange ? 1 : 0;
-    for (int k = t; k < j1 - j0; k += 256) a.rfl_out[(size_t)strip * ny + j0 + k] = sh.rowany[k] ? 1 : 0;
+    for (int k = t; k < j1 - j0; k += 256) a.rfl_out[(size_t)strip * ny + j0 + k] =
+            (sh.rowany[0][k] | sh.rowany[1][k] | sh.rowany[2][k] | sh.rowany[3][k]) ? 1 : 0;
     if (col_ok) a.cpr_out[(size_t)col * a.cstride + chunk] = colwet_f & 1;
     if (!kExact && outmask && ld_lane && colwet_x) {
 #pragma unroll
