@@ -231,8 +231,10 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
                         {  // max-wave (cliffs_main.f:347-355): store on change only
                             const float sp2 = f.x * f.x + f.y * f.y;
                             const bool ch = f.cp > hc[k], cv = sp2 > vc[k];
-                            if (keepmax && ch) F.st(oHM + o, f.cp);
-                            if (keepmax && cv) F.st(oVM + o, sp2);
+                            if (keepmax && (ch || cv)) {
+                                if (ch) F.st(oHM + o, f.cp);
+                                if (cv) F.st(oVM + o, sp2);
+                            }
                             colwet_f |= (ch || cv) ? 2 : 0;  // read only for deferred columns
                         }
                         wet = !is_land(c0_of(f.s) + f.cp, p);
