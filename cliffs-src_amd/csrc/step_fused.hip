@@ -184,7 +184,10 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
     x_load(0);
     for (int g = 0; g < ngroups; g++) {
         float(*buf)[4][256] = sh.row[g & 1];
-        // ---- X: wave w sweeps grid row q into the ring
+        // ---- X: wave w sweeps grid row q into the ring.  The block's next
+        // barrier waits for these rows, so the X phase runs at raised wave
+        // priority against the Y phases of other blocks on the CU.
+        __builtin_amdgcn_s_setprio(1);
         {
             const int q = j0 - 3 + 4 * g + w;
             const bool valid = x_valid(g);
@@ -204,6 +207,7 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
             }
         }
         __syncthreads();
+        __builtin_amdgcn_s_setprio(0);
         hv_load(g, hc, vc);
         // ---- Y: four rows through the column pipelines
 #pragma unroll
