@@ -444,6 +444,9 @@ int set_fused_rows(cliffs_handle* h, int rows) {
         ns = (h->nx + kFStrip - 1) / kFStrip;
         r = std::min(std::min(rows, kFMaxRows), h->ny);
         nc = (h->ny + r - 1) / r;
+        // the same number of chunks, of equal height: no short last chunk
+        const int rb = (h->ny + nc - 1) / nc;
+        if (rb >= 8) r = rb;
     }
     if (h->fflags && ns == h->f_nstrips && nc == h->f_nchunks && r == h->f_rows) return CLIFFS_OK;
     if (h->fflags) {

@@ -124,11 +124,12 @@ int cliffs_run_gauges(cliffs_handle *h, int every, int nsamples, int n, const in
  * bytes of HBM traffic per cell.  No counterpart in the reference, whose
  * step is always two sweeps (cliffs_main.f:317-345). */
 int cliffs_set_fused(cliffs_handle *h, int on);
-/* Tile height of the fused step: rows = 8..1024, or 0 for the automatic
- * choice (enough tiles to fill the device once, at least 32 rows).  Results
- * do not depend on it; small grids step faster with short tiles
- * (cliffs_ensemble_tune picks one by timing).  No counterpart in the
- * reference. */
+/* Tile height of the fused step: at most rows = 8..1024 (the grid's rows are
+ * split into as many chunks as that height needs, of equal height), or 0
+ * for the automatic choice (enough tiles to fill the device once, at least
+ * 32 rows).  Results do not depend on it; small grids step faster with
+ * short tiles (cliffs_ensemble_tune picks one by timing).  No counterpart
+ * in the reference. */
 int cliffs_set_tile_rows(cliffs_handle *h, int rows);
 
 /* ---- ensembles: many grids stepped together ---------------------------- */
