@@ -43,3 +43,26 @@ def test_threaded_conversions_match_serial(gpu_lib):
         for a, b, name in zip(whole, band, ("cel", "xvel", "yvel", "hmax", "velmax")):
             bad = int((a[j0:j1].view(np.int64) != b.view(np.int64)).sum())
             assert bad == 0, f"{name}, rows {j0}..{j1}: {bad} cells differ"
+
+
+def test_reset_max_after_steps(gpu_lib):
+    """cliffs_reset_max on a stepped grid (a device kernel from the device's
+    S): hmax back to c = 0 and velmax to 0, bit for bit; one step later the
+    maxima are that step's c (where positive) and u^2 + v^2."""
+    case = synth.coastal(nx=300, ny=260, seed=5)
+    s1, s2, zeta = case.metrics()
+    g = gpu_lib.Grid(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground)
+    g.upload(case.cel, case.xvel, case.yvel)
+    g.run(40)
+    hm, vm = g.download_max()
+    assert (hm > 0).any() and (vm > 0).any()
+    g.reset_max()
+    hm, vm = g.download_max()
+    assert not np.any(hm.view(np.int64)) and not np.any(vm.view(np.int64))
+    g.run(1)
+    c, u, v = g.download()
+    hm, vm = g.download_max()
+    assert np.array_equal(hm, np.where(c > 0, c, 0.0))
+    sp2 = (u.astype(np.float32) ** 2 + v.astype(np.float32) ** 2).astype(np.float64)
+    assert np.allclose(vm, sp2, rtol=1e-6, atol=0)
+    g.close()
