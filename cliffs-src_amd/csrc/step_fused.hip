@@ -331,32 +331,40 @@ __global__ __launch_bounds__(256) void step_fixup_batch_kernel(const FArgs* __re
 
 // Line flags of the current state (written to rfl_out, cpr_out) and cleared
 // ctr buffers, before the first fused step after any other operation.
-__global__ void fused_flags_kernel(const FArgs a, unsigned char* ctr0, unsigned char* ctr1) {
+// Row flags: one wave per (strip, row), its lanes across the strip's
+// columns (coalesced), the wave's vote the flag.
+__global__ void fused_rowflags_kernel(const FArgs a) {
+    const long wid = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;  // uniform in a wave
+    const int lane = threadIdx.x & 63;
+    if (wid >= (long)a.nstrips * a.g.ny) return;
+    const int s = (int)(wid / a.g.ny), j = (int)(wid % a.g.ny);
+    const int i1 = min((s + 1) * kFStrip, a.g.nx);
+    bool any = false;
+    for (int i = s * kFStrip + lane; i < i1; i += 64) {
+        const size_t o = (size_t)j * a.g.pitch + i;
+        any = any | !is_land(c0_of(a.g.S[o]) + a.g.CP[o], a.p);
+    }
+    const bool w = __ballot(any) != 0;
+    if (lane == 0) a.rfl_out[wid] = w ? 1 : 0;
+}
+
+// Column flags per chunk (consecutive threads: consecutive columns) and the
+// cleared ctr buffers.
+__global__ void fused_colflags_kernel(const FArgs a, unsigned char* ctr0, unsigned char* ctr1) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int nx = a.g.nx, ny = a.g.ny;
-    const long nr = (long)a.nstrips * ny;
-    const long nc = (long)a.nchunks * nx;
     if (t < nx) {
         ctr0[t] = 0;
         ctr1[t] = 0;
     }
-    if (t < nr) {
-        const int s = (int)(t / ny), j = (int)(t % ny);
-        const int i1 = min((s + 1) * kFStrip, nx);
-        bool any = false;
-        for (int i = s * kFStrip; i < i1 && !any; i++) {
-            const size_t o = (size_t)j * a.g.pitch + i;
-            any = !is_land(c0_of(a.g.S[o]) + a.g.CP[o], a.p);
-        }
-        a.rfl_out[t] = any ? 1 : 0;
-    } else if (t < nr + nc) {
-        const long u = t - nr;
-        const int c = (int)(u / nx), i = (int)(u % nx);  // consecutive threads: consecutive columns
+    if (t < (long)a.nchunks * nx) {
+        const int c = (int)(t / nx), i = (int)(t % nx);
         const int j1 = min((c + 1) * a.rows, ny);
-        bool any = false;
-        for (int j = c * a.rows; j < j1 && !any; j++) {
+        bool any = false;  // no early exit: the rows' loads stay independent
+#pragma unroll 8
+        for (int j = c * a.rows; j < j1; j++) {
             const size_t o = (size_t)j * a.g.pitch + i;
-            any = !is_land(c0_of(a.g.S[o]) + a.g.CP[o], a.p);
+            any = any | !is_land(c0_of(a.g.S[o]) + a.g.CP[o], a.p);
         }
         a.cpr_out[(size_t)i * a.cstride + c] = any ? 1 : 0;
     }
@@ -411,9 +419,13 @@ hipError_t launch_fixup_batch(const FArgs* args, const int2* bmap, int nblocks, 
 
 hipError_t launch_fused_flags(const FArgs& a, unsigned char* ctr0, unsigned char* ctr1,
                               hipStream_t st) {
-    const long n = (long)a.nstrips * a.g.ny + (long)a.nchunks * a.g.nx;
+    const long rows = (long)a.nstrips * a.g.ny;  // one wave each
+    hipLaunchKernelGGL(fused_rowflags_kernel, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, st, a);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const long n = (long)a.nchunks * a.g.nx;
     const long m = n > a.g.nx ? n : a.g.nx;
-    hipLaunchKernelGGL(fused_flags_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, a,
+    hipLaunchKernelGGL(fused_colflags_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, a,
                        ctr0, ctr1);
     return hipGetLastError();
 }
