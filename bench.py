@@ -266,8 +266,11 @@ def roofline(cells, value, ms_x, ms_y, unfused, traffic_key, copy_gbs=None):
         "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(dom, traffic_key),
         "kernel": dom, "bytes_per_cell": SWEEP_BYTES, "ms_per_launch": round(ms, 4),
+        "algorithmic_bytes": SWEEP_BYTES * cells,
+        # traffic (measured HBM bytes per launch) includes the max-wave update
         "with_maxwave": {"bytes_per_cell": SWEEP_BYTES + mw, "achieved": round(ach_mw, 1),
-                         "frac": round(ach_mw / HBM_PEAK_GBS, 4)},
+                         "frac": round(ach_mw / HBM_PEAK_GBS, 4),
+                         "algorithmic_bytes": (SWEEP_BYTES + mw) * cells},
         "other_kernel": other,
         "step_bytes_per_cell": step_bytes,
         "step_frac": round(step_bytes * value / HBM_PEAK_GBS, 4),

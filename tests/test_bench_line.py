@@ -27,8 +27,11 @@ def test_fused_roofline_counts_28_bytes():
     assert r["step_frac"] == pytest.approx(28 * 70.0 / 8000.0, abs=1e-4)
     assert r["step_frac_with_maxwave"] == pytest.approx(44 * 70.0 / 8000.0, abs=1e-4)
     assert r["other_kernel"] == {"step_fixup": {"ms_per_launch": 0.0076}}
-    # committed rocprofv3 summary: HBM bytes per launch, max-wave included
+    # committed rocprofv3 summary: HBM bytes per launch, max-wave included,
+    # between the algorithmic bytes without and with the max-wave update
     assert r["traffic"] == 715327488
+    assert r["algorithmic_bytes"] == 28 * CELLS
+    assert r["algorithmic_bytes"] < r["traffic"] <= r["with_maxwave"]["algorithmic_bytes"]
     assert r["peak"] == 8000.0 and r["unit"] == "GB/s" and r["bound"] == "hbm"
     assert r["copy_gbs"] is None and r["frac_of_copy"] is None
     r = bench.roofline(CELLS, 70.0, 0.2345, 0.0076, False, "open_ocean_4096", copy_gbs=5000.0)
