@@ -140,3 +140,47 @@ def test_c3_nest_full_size(gpu_lib):
         h.close()
     for h in (gA, gB, cA, cB):
         h.close()
+
+
+def test_largest_grids_step_as_two_kernels(gpu_lib):
+    """A grid past the fused step's 32-bit offsets (11008^2 = 121 M cells,
+    nine fields of 4.4 GB) steps with the two sweep kernels and 64-bit
+    offsets.  Ten steps of a hump in its middle must equal, bit for bit, the
+    same ten steps of the 600^2 block around the hump as a grid of its own
+    (fused step), away from that block's edges: a step moves information at
+    most three cells."""
+    n, dx, ground = 11008, 500.0, 0.01
+    assert 9 * (n * n + 64) * 4 >= 1 << 32
+    x = np.arange(n) * dx
+    y = np.arange(n) * dx
+    dep = 4000.0 - 1500.0 * np.outer(np.sin(y / 4.0e5) ** 2, np.cos(x / 3.0e5) ** 2)
+    i0 = j0 = 5200
+    m = 600
+    eta = np.zeros_like(dep)
+    xb, yb = x[i0:i0 + m], y[j0:j0 + m]
+    eta[j0:j0 + m, i0:i0 + m] = np.exp(-((xb[None, :] - xb[m // 2]) ** 2 + (yb[:, None] - yb[m // 2]) ** 2)
+                                       / (2 * 20000.0 ** 2))
+    cel = synth.depth_to_celerity(dep + eta, ground)
+    del eta
+    z = np.zeros_like(dep)
+    dt = 0.8 * dx / np.sqrt(synth.GRAV * 4001.0)
+    s1, s2, zeta = synth.metrics(x, y, True)
+    big = gpu_lib.Grid(dep, s1, s2, zeta, dt=dt, ground=ground)
+    big.upload(cel, z, z)
+    big.run(10)
+    want = big.download() + big.download_max()
+    big.close()
+    blk = (slice(j0, j0 + m), slice(i0, i0 + m))
+    bs1, bs2, bzeta = synth.metrics(xb, yb, True)
+    small = gpu_lib.Grid(dep[blk].copy(), bs1, bs2, bzeta, dt=dt, ground=ground)
+    small.upload(cel[blk].copy(), z[blk].copy(), z[blk].copy())
+    small.run(10)
+    got = small.download() + small.download_max()
+    small.close()
+    inner = (slice(40, m - 40), slice(40, m - 40))
+    for a, b, name in zip(want, got, ("cel", "xvel", "yvel", "hmax", "velmax")):
+        wa = np.ascontiguousarray(a[blk][inner])
+        wb = np.ascontiguousarray(b[inner])
+        bad = int((wa.view(np.int64) != wb.view(np.int64)).sum())
+        assert bad == 0, f"{name}: {bad} cells differ"
+    assert float(np.abs(want[1][blk]).max()) > 1e-4      # the hump moved
