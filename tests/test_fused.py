@@ -215,3 +215,39 @@ def test_run_timed_matches_run(gpu_lib):
         b.close()
         _same(got, want)
         assert total > 0 and k1 > 0 and k2 > 0 and k1 + k2 <= total / n * 8 + 1e-3
+
+
+def test_all_dry_grid_is_left_untouched(gpu_lib):
+    """A grid with no wet node: every line is all-dry, which the reference
+    leaves untouched (cliffs.f:113), so steps change no state, even where
+    dry cells hold a film below the threshold and small velocities; the
+    fused and the two-kernel step agree."""
+    rng = np.random.default_rng(4)
+    ny, nx = 90, 130
+    dep = rng.uniform(-5.0, -0.1, (ny, nx))
+    ground = 0.01
+    cm = np.sqrt(synth.GRAV * ground)
+    cel = cm * rng.uniform(0.0, 0.9, (ny, nx))
+    u = rng.uniform(-0.1, 0.1, (ny, nx))
+    v = rng.uniform(-0.1, 0.1, (ny, nx))
+    x, y = np.arange(nx) * 10.0, np.arange(ny) * 10.0
+    case = synth.Case("all_dry", dep, cel, u, v, x, y, True, 0.1, ground, 0.0, 5, {})
+    out = []
+    for fused in (True, False):
+        g = _grid(gpu_lib, case, fused)
+        before = g.download()
+        g.run(5)
+        after = g.download()
+        for a, b in zip(before, after):
+            assert np.array_equal(a.view(np.int64), b.view(np.int64))
+        out.append(_result(g))
+        g.close()
+    _same(*out)
+
+
+def test_smallest_2d_grid(gpu_lib):
+    """3 x 3 cells: one interior node, every line an edge line."""
+    case = synth.coastal(nx=3, ny=3, seed=2)
+    case.cel = np.full((3, 3), 10.0)
+    case.xvel = np.array([[0.0, 0.1, 0.0], [0.2, 0.3, -0.1], [0.0, 0.0, 0.1]])
+    _same(_run(gpu_lib, case, True, 12, chunk=5), _run(gpu_lib, case, False, 12, chunk=5))
