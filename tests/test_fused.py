@@ -7,8 +7,9 @@ all-dry rows (x-sweep skipped), all-dry columns (y-sweep skipped, max-wave
 deferred to the fixup kernel), columns that flood during the x-sweep (the
 predicted column flag is wrong and the fixup kernel recomputes the tile),
 dry cells holding non-canonical values (which an unwanted sweep would
-reset), time-varying edges, one-strip and one-chunk grids, and switching
-between the fused step and single sweeps.
+reset), time-varying edges, one-strip and one-chunk grids, switching
+between the fused step and single sweeps, a grid with no wet node and the
+smallest 2-D grid.
 """
 import numpy as np
 import pytest
