@@ -31,6 +31,11 @@ module cliffs_gpu_mod
   public :: cliffs_step, cliffs_run, cliffs_run_timed, cliffs_gauges, cliffs_sync
   public :: cliffs_stream, cliffs_info, cliffs_last_error, cliffs_health
   public :: cliffs_nest_maps, cliffs_nest_create, cliffs_nest_destroy, cliffs_nest_feed
+  public :: cliffs_nest_couple, cliffs_nest_record, cliffs_nest_end_feed, cliffs_nest_child_step
+  public :: cliffs_run_gauges, cliffs_set_fused, cliffs_set_tile_rows
+  public :: cliffs_ensemble_create, cliffs_ensemble_destroy, cliffs_ensemble_run
+  public :: cliffs_ensemble_run_timed, cliffs_ensemble_tune, cliffs_ensemble_sync
+  public :: cliffs_ensemble_run_gauges
   public :: cliffs_gpu_open, cliffs_error_message
 
   interface
@@ -149,6 +154,87 @@ module cliffs_gpu_mod
       integer(c_int), intent(inout) :: iarv
       integer(c_int), intent(out) :: fed
       real(c_double), intent(out) :: uvq_w(*), uvq_e(*), uvq_s(*), uvq_n(*)
+    end function
+    integer(c_int) function cliffs_run_gauges(h, every, nsamples, n, ig, jg, eta) &
+        bind(C, name="cliffs_run_gauges")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: h
+      integer(c_int), value :: every, nsamples, n
+      integer(c_int), intent(in) :: ig(*), jg(*)
+      real(c_float), intent(out) :: eta(*)
+    end function
+    integer(c_int) function cliffs_set_fused(h, on) bind(C, name="cliffs_set_fused")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: on
+    end function
+    integer(c_int) function cliffs_set_tile_rows(h, rows) bind(C, name="cliffs_set_tile_rows")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: rows
+    end function
+    integer(c_int) function cliffs_nest_couple(nest, child, capacity) bind(C, name="cliffs_nest_couple")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: nest, child
+      integer(c_int), value :: capacity
+    end function
+    integer(c_int) function cliffs_nest_record(nest, cuke, hmin, t, iarv, nrec) &
+        bind(C, name="cliffs_nest_record")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: nest
+      real(c_double), value :: cuke, hmin, t
+      integer(c_int), intent(inout) :: iarv
+      integer(c_int), intent(out) :: nrec
+    end function
+    integer(c_int) function cliffs_nest_end_feed(nest) bind(C, name="cliffs_nest_end_feed")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: nest
+    end function
+    integer(c_int) function cliffs_nest_child_step(nest, t) bind(C, name="cliffs_nest_child_step")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: nest
+      real(c_double), value :: t
+    end function
+    ! members: an array of n grid handles
+    integer(c_int) function cliffs_ensemble_create(members, n, e) bind(C, name="cliffs_ensemble_create")
+      import :: c_int, c_ptr
+      type(c_ptr), intent(in) :: members(*)
+      integer(c_int), value :: n
+      type(c_ptr), intent(out) :: e
+    end function
+    integer(c_int) function cliffs_ensemble_destroy(e) bind(C, name="cliffs_ensemble_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+    end function
+    integer(c_int) function cliffs_ensemble_run(e, nsteps) bind(C, name="cliffs_ensemble_run")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: nsteps
+    end function
+    integer(c_int) function cliffs_ensemble_run_timed(e, nsteps, ms_total) &
+        bind(C, name="cliffs_ensemble_run_timed")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: e
+      integer(c_int), value :: nsteps
+      real(c_double), intent(out) :: ms_total
+    end function
+    integer(c_int) function cliffs_ensemble_tune(e, trial_steps, rows) bind(C, name="cliffs_ensemble_tune")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: trial_steps
+      integer(c_int), intent(out) :: rows
+    end function
+    integer(c_int) function cliffs_ensemble_sync(e) bind(C, name="cliffs_ensemble_sync")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+    end function
+    integer(c_int) function cliffs_ensemble_run_gauges(e, every, nsamples, n, ig, jg, eta) &
+        bind(C, name="cliffs_ensemble_run_gauges")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: e
+      integer(c_int), value :: every, nsamples, n
+      integer(c_int), intent(in) :: ig(*), jg(*)
+      real(c_float), intent(out) :: eta(*)
     end function
   end interface
 

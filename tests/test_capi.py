@@ -101,3 +101,14 @@ def test_conversion_keeps_fp64_predicates():
     eta0 = cel ** 2 / g - dep
     eta1 = c2 ** 2 / g - dep
     assert np.abs(eta0 - eta1)[wet].max() < 1e-6 * (1 + np.abs(eta0[wet]).max())
+
+
+def test_fortran_module_binds_every_header_function():
+    """cliffs_gpu_mod.f90 declares a bind(C) interface for every function of
+    include/cliffs_gpu.h, under the same name."""
+    import re
+    path = os.path.join(os.path.dirname(cliffs_gpu.HEADER), "..", "cliffs-src_amd", "fortran",
+                        "cliffs_gpu_mod.f90")
+    with open(path) as f:
+        bound = set(re.findall(r'bind\(C, name="(cliffs_\w+)"\)', f.read()))
+    assert set(cliffs_gpu.header_symbols()) == bound
