@@ -4,7 +4,9 @@ set -e
 HERE=$(cd "$(dirname "$0")" && pwd)
 HIPCC=${HIPCC:-/opt/rocm/bin/hipcc}
 ARCH=${CLIFFS_ARCH:-gfx950}
-FLAGS="-O3 -std=c++17 -fPIC --offload-arch=$ARCH -Xclang -target-feature -Xclang -packed-fp32-ops -Wall"
+# -ffp-contract=off: every fused multiply-add is an explicit fmaf in the source,
+# so the device, the host side and the test emulator round identically
+FLAGS="-O3 -std=c++17 -fPIC --offload-arch=$ARCH -ffp-contract=off -Xclang -target-feature -Xclang -packed-fp32-ops -Wall"
 cd "$HERE/csrc"
 $HIPCC $FLAGS -c sweeps.hip -o "$HERE/csrc/sweeps.o"
 # the fused step kernel is bound by instruction issue at 168 VGPRs; the

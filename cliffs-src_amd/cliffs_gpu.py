@@ -97,6 +97,7 @@ def load_library():
         "cliffs_nest_record": ([H, C.c_double, C.c_double, C.c_double, _ip, _ip], C.c_int),
         "cliffs_nest_end_feed": ([H], C.c_int),
         "cliffs_nest_child_step": ([H, C.c_double], C.c_int),
+        "cliffs_nest_child_start": ([H, C.c_double, _dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -233,7 +234,7 @@ class Grid:
         _chk(load_library().cliffs_set_fused(self._h, 1 if on else 0))
 
     def set_tile_rows(self, rows: int):
-        """Tile height of the fused step (8..1024, 0 = automatic)."""
+        """Tile height of the fused step (8..256, 0 = automatic)."""
         _chk(load_library().cliffs_set_tile_rows(self._h, int(rows)))
 
     def run_timed(self, nsteps: int):
@@ -412,6 +413,13 @@ class Nest:
 
     def end_feed(self):
         _chk(load_library().cliffs_nest_end_feed(self._h))
+
+    def child_start(self, cuke):
+        """The child's start record under its threshold cuke
+        (get_boundary_feed.f:144-167) -> that record's time t0."""
+        t0 = C.c_double()
+        _chk(load_library().cliffs_nest_child_start(self._h, float(cuke), C.byref(t0)))
+        return t0.value
 
     def child_step(self, t):
         _chk(load_library().cliffs_nest_child_step(self._h, float(t)))

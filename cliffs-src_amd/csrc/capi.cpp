@@ -33,6 +33,7 @@ hipError_t launch_nest_arrival(const NestArgs& a, hipStream_t st);
 hipError_t launch_nest_feed(const NestArgs& a, hipStream_t st);
 hipError_t launch_child_edges(const double* r0, const double* r1, double cc, double omc, double grav,
                               int nxc, int nyc, int zero, float* xe, float* ye, hipStream_t st);
+hipError_t launch_record_wave(const double* rec, int nxc, int nyc, double* wave, hipStream_t st);
 hipError_t launch_step_fused(const FArgs& a, hipStream_t st);
 hipError_t launch_step_fixup(const FArgs& a, hipStream_t st);
 hipError_t launch_fused_flags(const FArgs& a, unsigned char* ctr0, unsigned char* ctr1,
@@ -125,6 +126,10 @@ struct cliffs_handle {
     bool fused_valid = false;
     bool state = false;
     std::vector<float> hbuf;   // pitched host staging
+    // nests (as parent or coupled child) and ensembles that use this grid's
+    // device memory and stream; the grid cannot be destroyed while any is live
+    int users = 0;
+    int ens_users = 0;         // ensembles among them (they fix the tile geometry)
 
     DevGrid grid() const {
         DevGrid g;
@@ -180,6 +185,14 @@ struct cliffs_nest {
     bool bndr = true;                  // bndr_input: feed still running for the child
     bool ended = false;                // no more records will come
     float* cedges = nullptr;           // child edges: x [6][nyc], y [6][nxc]
+    // child start rule (get_boundary_feed.f:144-167): the first record whose
+    // largest edge |eta| reaches the child's threshold
+    double* dwave = nullptr;           // per ring slot: largest |eta| of its record
+    double start_cuke = 0.0;           // child's threshold (0: the first record)
+    bool start_set = false;            // threshold given by cliffs_nest_child_start
+    int start = 0;                     // 1-based start record, 0 while unknown
+    int scanned = 0;                   // records found below the threshold
+    bool counted = false;              // the parent's use count includes this nest
     NestArgs args(double cuke) const;
 };
 
@@ -683,6 +696,9 @@ int cliffs_create(const cliffs_grid_desc* d, cliffs_handle** out) {
 }
 
 int cliffs_destroy(cliffs_handle* h) {
+    if (h && h->users > 0)
+        return fail(CLIFFS_ERR_STATE, "grid still used by " + std::to_string(h->users) +
+                                          " nest(s) or ensemble(s): destroy those first");
     free_handle(h);
     return CLIFFS_OK;
 }
@@ -998,8 +1014,10 @@ int cliffs_set_fused(cliffs_handle* h, int on) {
 
 int cliffs_set_tile_rows(cliffs_handle* h, int rows) {
     if (!h) return fail(CLIFFS_ERR_ARG, "NULL handle");
-    if (rows != 0 && (rows < 8 || rows > kFMaxRows)) return fail(CLIFFS_ERR_ARG, "tile rows must be 0 or 8..1024");
+    if (rows != 0 && (rows < 8 || rows > kFMaxRows)) return fail(CLIFFS_ERR_ARG, "tile rows must be 0 or 8..256");
     if (!h->fused_ok) return fail(CLIFFS_ERR_ARG, "grid has no fused step (1-D or too large)");
+    if (h->ens_users > 0)
+        return fail(CLIFFS_ERR_STATE, "grid belongs to an ensemble, whose tuner sets its tile height");
     HIPCHK(hipSetDevice(h->dev));
     return set_fused_rows(h, rows);
 }
@@ -1020,6 +1038,8 @@ struct cliffs_ensemble {
     hipEvent_t argev = nullptr;   // the argument copy has executed (staging reusable)
     hipGraph_t graph = nullptr;   // two steps (both parities)
     hipGraphExec_t gexec = nullptr;
+    std::vector<int> tiles;       // members' tile counts the block map and graph were built for
+    bool counted = false;         // members' use counts include this ensemble
 };
 
 namespace {
@@ -1036,6 +1056,11 @@ void free_ensemble(cliffs_ensemble* e) {
     (void)hipFree(e->dmap);
     if (e->hargs) (void)hipHostFree(e->hargs);
     if (e->st) (void)hipStreamDestroy(e->st);
+    if (e->counted)
+        for (cliffs_handle* h : e->m) {
+            h->users--;
+            h->ens_users--;
+        }
     delete e;
 }
 
@@ -1044,6 +1069,7 @@ int ensemble_rebuild(cliffs_ensemble* e) {
     const int n = (int)e->m.size();
     std::vector<int> nt(n);
     for (int i = 0; i < n; i++) nt[i] = e->m[i]->f_nstrips * e->m[i]->f_nchunks;
+    e->tiles.clear();
     std::vector<int2> bmap;
     ensemble_block_map(nt.data(), n, bmap);
     HIPCHK(hipStreamSynchronize(e->st));
@@ -1070,6 +1096,7 @@ int ensemble_rebuild(cliffs_ensemble* e) {
     if (ec != hipSuccess) return hipfail(ec, "hipStreamEndCapture");
     e->graph = g;
     HIPCHK(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
+    e->tiles = nt;
     return CLIFFS_OK;
 }
 
@@ -1077,6 +1104,14 @@ int ensemble_rebuild(cliffs_ensemble* e) {
 // fused step are made current on each member's stream first.
 int ensemble_take(cliffs_ensemble* e) {
     if (!e->gexec) return fail(CLIFFS_ERR_STATE, "ensemble has no step graph (a rebuild failed)");
+    // a member shared with another ensemble may have been re-tiled by that
+    // ensemble's tuner: the block map and graph must match its tiles
+    for (size_t i = 0; i < e->m.size(); i++)
+        if (e->m[i]->f_nstrips * e->m[i]->f_nchunks != e->tiles[i]) {
+            const int rc = ensemble_rebuild(e);
+            if (rc) return rc;
+            break;
+        }
     for (cliffs_handle* h : e->m) {
         int rc;
         if ((rc = ensure_fused(h))) return rc;
@@ -1177,6 +1212,11 @@ int cliffs_ensemble_create(cliffs_handle* const* members, int n, cliffs_ensemble
     if (hipEventRecord(e->argev, e->st) != hipSuccess) return bail(fail(CLIFFS_ERR_HIP, "event"));
     const int rc = ensemble_rebuild(e);
     if (rc) return bail(rc);
+    for (cliffs_handle* h : e->m) {
+        h->users++;
+        h->ens_users++;
+    }
+    e->counted = true;
     *out = e;
     return CLIFFS_OK;
 }
@@ -1230,7 +1270,7 @@ int cliffs_ensemble_run_timed(cliffs_ensemble* e, int nsteps, double* ms_total) 
 int cliffs_ensemble_tune(cliffs_ensemble* e, int trial_steps, int* rows_out) {
     if (!e || trial_steps < 1) return fail(CLIFFS_ERR_ARG, "bad tune arguments");
     HIPCHK(hipSetDevice(e->dev));
-    static const int kCand[] = {16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024};
+    static const int kCand[] = {16, 24, 32, 48, 64, 96, 128, 160, 192, 256};
     EventSet es;
     es.ev.assign(2, nullptr);
     for (auto& x : es.ev) HIPCHK(hipEventCreate(&x));
@@ -1410,6 +1450,8 @@ int cliffs_nest_create(cliffs_handle* h, int nxc, int nyc, const int* we_ny, con
         return fail(CLIFFS_ERR_HIP, "nest map upload");
     }
     n->huvq.resize(n->nuvq);
+    h->users++;
+    n->counted = true;
     *out = n;
     return CLIFFS_OK;
 }
@@ -1419,14 +1461,19 @@ int cliffs_nest_destroy(cliffs_nest* n) {
     if (n->parent) {
         (void)hipSetDevice(n->parent->dev);
         if (n->parent->st) (void)hipStreamSynchronize(n->parent->st);
+        if (n->counted) n->parent->users--;
     }
-    if (n->child && n->child->st) (void)hipStreamSynchronize(n->child->st);
+    if (n->child) {
+        if (n->child->st) (void)hipStreamSynchronize(n->child->st);
+        n->child->users--;
+    }
     for (auto e : n->rec_ev)
         if (e) (void)hipEventDestroy(e);
     for (auto e : n->use_ev)
         if (e) (void)hipEventDestroy(e);
     (void)hipFree(n->ring);
     (void)hipFree(n->cedges);
+    (void)hipFree(n->dwave);
     (void)hipFree(n->we_ny);
     (void)hipFree(n->sn_nx);
     (void)hipFree(n->we_cy);
@@ -1528,13 +1575,19 @@ int cliffs_nest_couple(cliffs_nest* n, cliffs_handle* child, int capacity) {
         HIPCHK(hipEventRecord(n->use_ev[k], child->st));
     }
     if (hipMalloc(&n->ring, sizeof(double) * n->nuvq * capacity) != hipSuccess ||
+        hipMalloc(&n->dwave, sizeof(double) * capacity) != hipSuccess ||
         hipMalloc(&n->cedges, sizeof(float) * (6 * (size_t)n->nyc + 6 * (size_t)n->nxc)) != hipSuccess)
         return fail(CLIFFS_ERR_NOMEM, "nest ring allocation");
     n->child = child;
+    child->users++;
     n->tinput.clear();
     n->irec = 1;
     n->bndr = true;
     n->ended = false;
+    n->start_cuke = 0.0;
+    n->start_set = false;
+    n->start = 0;
+    n->scanned = 0;
     return CLIFFS_OK;
 }
 
@@ -1553,14 +1606,18 @@ int cliffs_nest_record(cliffs_nest* n, double cuke, double hmin, double t, int* 
     if (!*iarv) return CLIFFS_OK;
     const int k = (int)n->tinput.size() + 1;  // 1-based record being written
     // the slot's previous record (k - cap) must be past the child's need:
-    // the child reads records irec-1 and irec
-    if (k - n->cap >= n->irec - 1 && k > n->cap)
+    // the child reads records irec-1 and irec from its start record on;
+    // before the start is known, records found below the threshold are free
+    const int need_lo = n->start > 0 ? std::max(n->irec - 1, n->start) : n->scanned + 1;
+    if (k > n->cap && k - n->cap >= need_lo)
         return fail(CLIFFS_ERR_STATE, "parent feed too far ahead of the child (raise the capacity)");
     const int slot = (k - 1) % n->cap;
     a.uvq = n->ring + (size_t)slot * n->nuvq;
     HIPCHK(hipStreamWaitEvent(h->st, n->use_ev[slot], 0));
-    const hipError_t e = launch_nest_feed(a, h->st);
+    hipError_t e = launch_nest_feed(a, h->st);
     if (e != hipSuccess) return hipfail(e, "nest_feed");
+    e = launch_record_wave(a.uvq, n->nxc, n->nyc, n->dwave + slot, h->st);
+    if (e != hipSuccess) return hipfail(e, "record_wave");
     HIPCHK(hipEventRecord(n->rec_ev[slot], h->st));
     n->tinput.push_back(t);
     *nrec = k;
@@ -1573,6 +1630,54 @@ int cliffs_nest_end_feed(cliffs_nest* n) {
     return CLIFFS_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Start rule of get_boundary_feed.f:144-167 over the records written so
+// far: the first record whose largest edge |eta| reaches start_cuke, and an
+// error when the feed ended without one before its last record (it >= nTn).
+int nest_find_start(cliffs_nest* n) {
+    if (n->start > 0) return CLIFFS_OK;
+    const int nrec = (int)n->tinput.size();
+    while (n->scanned < nrec) {
+        const int k = n->scanned + 1, slot = (k - 1) % n->cap;
+        double w = 0.0;
+        HIPCHK(hipEventSynchronize(n->rec_ev[slot]));
+        HIPCHK(hipMemcpy(&w, n->dwave + slot, sizeof(double), hipMemcpyDeviceToHost));
+        if (w >= n->start_cuke) {
+            if (n->ended && k >= nrec)
+                return fail(CLIFFS_ERR_STATE, "only the feed's last record reaches the child's threshold");
+            n->start = k;
+            n->irec = std::max(n->irec, k);
+            return CLIFFS_OK;
+        }
+        n->scanned = k;
+    }
+    return fail(CLIFFS_ERR_STATE, n->ended ? "feed ended below the child's threshold"
+                                           : "no feed record reaches the child's threshold yet");
+}
+
+}  // namespace
+
+extern "C" {
+
+int cliffs_nest_child_start(cliffs_nest* n, double cuke, double* t0) {
+    if (!n || !t0 || !(cuke >= 0.0)) return fail(CLIFFS_ERR_ARG, "bad child start arguments");
+    if (!n->child) return fail(CLIFFS_ERR_STATE, "nest not coupled to a child");
+    if (n->start_set && cuke != n->start_cuke)
+        return fail(CLIFFS_ERR_STATE, "the child's threshold is already set");
+    if (!n->start_set && n->start > 0)
+        return fail(CLIFFS_ERR_STATE, "the child already started from the first record");
+    HIPCHK(hipSetDevice(n->child->dev));
+    n->start_cuke = cuke;
+    n->start_set = true;
+    const int rc = nest_find_start(n);
+    if (rc) return rc;
+    *t0 = n->tinput[n->start - 1];
+    return CLIFFS_OK;
+}
+
 int cliffs_nest_child_step(cliffs_nest* n, double t) {
     if (!n || !n->child) return fail(CLIFFS_ERR_STATE, "nest not coupled to a child");
     cliffs_handle* c = n->child;
@@ -1580,6 +1685,9 @@ int cliffs_nest_child_step(cliffs_nest* n, double t) {
     const int nrec = (int)n->tinput.size();
     if (nrec < 1) return fail(CLIFFS_ERR_STATE, "no feed record yet");
     HIPCHK(hipSetDevice(c->dev));
+    int rc0;
+    if ((rc0 = nest_find_start(n))) return rc0;  // threshold 0 unless set: the first record
+    if (t < n->tinput[n->start - 1]) return fail(CLIFFS_ERR_ARG, "child time before its start record");
     // cliffs_main.f:305-315: tinput(irec-1) <= t < tinput(irec)
     while (n->bndr && t >= n->tinput[n->irec - 1]) {
         if (n->irec == nrec) {

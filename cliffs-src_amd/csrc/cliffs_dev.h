@@ -84,7 +84,7 @@ struct YArgs {
 
 // Fused 2-D step (step_fused.hip): tiles of kFStrip columns x `rows` rows.
 constexpr int kFStrip = 248;    // output columns per tile (x window of 256)
-constexpr int kFMaxRows = 1024; // rows per tile, at most
+constexpr int kFMaxRows = 256;  // rows per tile, at most (the tile's row metrics sit in LDS)
 constexpr int kXCDs = 8;        // MI355X: 8 accelerator complex dies, one L2 each
 
 // Line flags of the fused step are per-tile contributions (one byte per tile

@@ -110,6 +110,38 @@ __global__ void child_edges_kernel(const double* r0, const double* r1, double cc
     out[2 * len + q] = (float)(grav * val[2]);
 }
 
+// Largest |eta| over the four edges of one feed record (the quantity
+// get_boundary_feed.f:147-158 compares with the child's threshold): one
+// block of 256 threads; max of absolute values is exact in any order.
+__global__ void record_wave_kernel(const double* rec, int nxc, int nyc, double* wave) {
+    __shared__ double part[256];
+    double m = 0.0;
+    const int len = 2 * nyc + 2 * nxc;  // eta entries: W, E (nyc each), S, N (nxc each)
+    for (int q = threadIdx.x; q < len; q += 256) {
+        size_t o;
+        if (q < 2 * nyc) {
+            const int side = q / nyc, k = q % nyc;
+            o = (size_t)side * 3 * nyc + (size_t)2 * nyc + k;
+        } else {
+            const int r = q - 2 * nyc, side = r / nxc, k = r % nxc;
+            o = (size_t)6 * nyc + (size_t)side * 3 * nxc + (size_t)2 * nxc + k;
+        }
+        m = fmax(m, fabs(rec[o]));
+    }
+    part[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] = fmax(part[threadIdx.x], part[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *wave = part[0];
+}
+
+hipError_t launch_record_wave(const double* rec, int nxc, int nyc, double* wave, hipStream_t st) {
+    hipLaunchKernelGGL(record_wave_kernel, dim3(1), dim3(256), 0, st, rec, nxc, nyc, wave);
+    return hipGetLastError();
+}
+
 hipError_t launch_child_edges(const double* r0, const double* r1, double cc, double omc, double grav,
                               int nxc, int nyc, int zero, float* xe, float* ye, hipStream_t st) {
     const int len = nxc > nyc ? nxc : nyc;

@@ -64,6 +64,12 @@ __device__ __forceinline__ int tile_of_block(int b, int ntiles) {
 struct FShared {
     float row[2][4][4][256];      // [buffer][row of the group][s, cp, u, v][window node]
     float4 xmt[256];              // x metric of the tile's window nodes
+    // row metrics of the tile's stream rows, staged once per tile so that no
+    // row waits on a global load: y metric and zeta of rows j0-5 .. j1, x row
+    // factors of rows j0-3 .. j1+2
+    float4 ymt[kFMaxRows + 6];
+    float zt[kFMaxRows + 6];
+    float2 xrt[kFMaxRows + 6];
     // [wave][row of the tile]: the wave's columns of the row have a wet node
     // after the step.  Each wave writes each row of the tile exactly once
     // (all lanes, the same value): no clearing, no atomics.
@@ -111,6 +117,13 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
         sh.xmt[t] = (i >= 0 && i < nx) ? a.xm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
     const XLane XL{i0, xlane_vmask(nx, i0), &sh.xmt[4 * lane]};
+    const int yb = j0 - 5, xb = j0 - 3;  // first rows of the staged row tables
+    for (int k = t; k < j1 - j0 + 6; k += 256) {
+        const int ry = yb + k, rx = xb + k;
+        sh.ymt[k] = (ry >= 0 && ry < ny) ? a.ym[ry] : make_float4(0.f, 0.f, 0.f, 0.f);
+        sh.zt[k] = (ry >= 0 && ry < ny) ? a.zeta[ry] : 0.f;
+        sh.xrt[k] = (rx >= 0 && rx < ny) ? a.xr[rx] : make_float2(0.f, 0.f);
+    }
     // per-lane constants as integers (vector registers, not scalar lane masks)
     const int ld_lane = (i0 >= 0 && i0 < nx) ? 1 : 0;
     const int outmask = (lane >= 1 && lane <= 62) ? XL.vmask : 0;
@@ -136,7 +149,7 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
     YPipe pipe;
     pipe.init();
     int colwet_f = 0;  // bit 0: a node of the column is wet after the step; bit 1: deferred change
-    __syncthreads();  // xmt is complete
+    __syncthreads();  // xmt and the row tables are complete
 
     // rows j0-3 .. j1+2 stream through; group g holds stream rows 4g .. 4g+3
     const int nstream = j1 - j0 + 6;
@@ -197,8 +210,7 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
             if (valid) {
                 const bool sweep = q >= a.row_lo && q <= a.row_hi && fl;
                 Q4 ocp, ou, ov;
-                x_row(XL, p, nx, ny, q, sweep, sweep ? a.xr[q] : make_float2(0.f, 0.f), a.xedges, s,
-                      cp, u, v, ocp, ou, ov);
+                x_row(XL, p, nx, ny, q, sweep, sh.xrt[q - xb], a.xedges, s, cp, u, v, ocp, ou, ov);
                 st4(&buf[w][0][4 * lane], s);
                 st4(&buf[w][1][4 * lane], ocp);
                 st4(&buf[w][2][4 * lane], ou);
@@ -224,7 +236,7 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
                     rin.u = buf[k][3][t + 4];  // normal velocity of the y-sweep: yvel
                     rin.v = buf[k][2][t + 4];
                 }
-                const YOut f = pipe.advance(r, rin, rv, cc, ny, a.ym, a.zeta, p);
+                const YOut f = pipe.advance(r, rin, rv, cc, ny, sh.ymt, sh.zt, yb, p);
                 if (r >= j0 && r < j1) {
                     bool wet = false;
                     if (col_ok) {
@@ -233,7 +245,7 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
                         F.st(oUn + o, f.x);
                         F.st(oVn + o, f.y);
                         {  // max-wave (cliffs_main.f:347-355): store on change only
-                            const float sp2 = f.x * f.x + f.y * f.y;
+                            const float sp2 = speed2(f.x, f.y);
                             const bool ch = f.cp > hc[k], cv = sp2 > vc[k];
                             if (keepmax && (ch || cv)) {
                                 if (ch) F.st(oHM + o, f.cp);
@@ -297,7 +309,7 @@ __device__ __forceinline__ void fixup_tile(const FArgs& a, int tile, FShared& sh
         for (int r = j0; r < j1; r++) {
             const size_t o = (size_t)r * a.g.pitch + col;
             const float cp = a.g.CPn[o], u = a.g.Un[o], v = a.g.Vn[o];
-            const float sp2 = u * u + v * v;
+            const float sp2 = speed2(u, v);
             if (cp > a.g.HM[o]) a.g.HM[o] = cp;
             if (sp2 > a.g.VM[o]) a.g.VM[o] = sp2;
         }

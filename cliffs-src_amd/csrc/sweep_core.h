@@ -332,8 +332,10 @@ struct YPipe {
         okrun = 0;
     }
 
+    // ym/zeta hold the y metric and zeta of rows yb, yb+1, ... (global
+    // tables with yb = 0, or a tile's rows staged in LDS)
     __device__ __forceinline__ YOut advance(int r, const RawIn& rin, bool rin_valid, const YCol& col,
-                                            int ny, const float4* ym, const float* zeta,
+                                            int ny, const float4* ym, const float* zeta, int yb,
                                             const Params& p) {
         const bool swept = col.swept;
         // 1. new row r+3
@@ -378,24 +380,24 @@ struct YPipe {
         const bool end1 = (row1 == 0) || (row1 == ny - 1);
         if (fast) {
             if (swept) {
-                const float4 mr = ym[row1];
+                const float4 mr = ym[row1 - yb];
                 Metric m;
                 m.rfm = mr.x;
                 m.rfp = mr.y;
                 m.rsum = mr.z;
                 m.fpm = mr.w;
-                m.scl = zeta[row1];
+                m.scl = zeta[row1 - yb];
                 interior_update(pe0, pc0, pe1, pc1, peB, pcB, E0, E1, true, m, p, ocp1, ou1, ov1);
             }
         } else if (swept && !nl1 && row1 >= 0 && row1 < ny) {
             if (row1 >= 1 && row1 <= ny - 2) {
-                const float4 mr = ym[row1];
+                const float4 mr = ym[row1 - yb];
                 Metric m;
                 m.rfm = mr.x;
                 m.rfp = mr.y;
                 m.rsum = mr.z;
                 m.fpm = mr.w;
-                m.scl = zeta[row1];
+                m.scl = zeta[row1 - yb];
                 const Node L = nl0 ? mirror(pe1) : pe0;
                 const Node R = nlB ? mirror(pe1) : peB;
                 const float cL = nl0 ? pc1 : pc0;
@@ -407,13 +409,13 @@ struct YPipe {
             } else if (row1 == 0) {  // south edge
                 const float gbs = col.edge(2);
                 if (!nlB && floods(gbs, pe1.s, p))
-                    edge_first(pe1, pc1, peB, E1, ym[0].y, zeta[0], col.edge(0), col.edge(1), gbs, p,
-                               ocp1, ou1, ov1);
+                    edge_first(pe1, pc1, peB, E1, ym[0 - yb].y, zeta[0 - yb], col.edge(0), col.edge(1),
+                               gbs, p, ocp1, ou1, ov1);
             } else {  // north edge
                 const float gbn = col.edge(5);
                 if (!nl0 && floods(gbn, pe1.s, p))
-                    edge_last(pe1, pc1, pe0, E0, ym[ny - 1].x, zeta[ny - 1], col.edge(3), col.edge(4),
-                              gbn, p, ocp1, ou1, ov1);
+                    edge_last(pe1, pc1, pe0, E0, ym[ny - 1 - yb].x, zeta[ny - 1 - yb], col.edge(3),
+                              col.edge(4), gbn, p, ocp1, ou1, ov1);
             }
         }
         const float lc1 = c0_of(pe1.s) + (end1 ? pe1.cp : ocp1);

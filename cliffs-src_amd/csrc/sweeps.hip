@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void sweep_x_kernel(const XArgs a) {
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 hm.v[k] = fmaxf(hm.v[k], ocp.v[k]);
-                vm.v[k] = fmaxf(vm.v[k], ou.v[k] * ou.v[k] + ov.v[k] * ov.v[k]);
+                vm.v[k] = fmaxf(vm.v[k], speed2(ou.v[k], ov.v[k]));
             }
             st4(HMr, hm);
             st4(VMr, vm);
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256) void sweep_y_kernel(const YArgs a) {
         for (int k = 0; k < kYBatch; k++) {
             if (it + k < n_it) {
                 const int r = r_first + it + k;
-                const YOut f = pipe.advance(r, cur[k], row_ok(r + 3), cc, ny, a.ym, a.zeta, p);
+                const YOut f = pipe.advance(r, cur[k], row_ok(r + 3), cc, ny, a.ym, a.zeta, 0, p);
                 if (r >= j0 && r < j1) {
                     const size_t o = (size_t)r * a.g.pitch + col;
                     bool wet = false;
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void sweep_y_kernel(const YArgs a) {
                         a.g.Un[o] = f.x;
                         a.g.Vn[o] = f.y;
                         if (a.fuse_max) {  // fused max-wave (cliffs_main.f:347-355): store on change only
-                            const float sp2 = f.x * f.x + f.y * f.y;
+                            const float sp2 = speed2(f.x, f.y);
                             if (f.cp > hc[k]) a.g.HM[o] = f.cp;
                             if (sp2 > vc[k]) a.g.VM[o] = sp2;
                         }
@@ -251,7 +251,7 @@ __global__ void maxwave_kernel(DevGrid g) {
     const size_t o = (size_t)j * g.pitch + i;
     const float u = g.U[o], v = g.V[o];
     g.HM[o] = fmaxf(g.HM[o], g.CP[o]);
-    g.VM[o] = fmaxf(g.VM[o], u * u + v * v);
+    g.VM[o] = fmaxf(g.VM[o], speed2(u, v));
 }
 
 // surface elevation at gauges (1-based I, J) into row `slot` of series:

@@ -27,6 +27,12 @@
 //
 // Ghost nodes at shorelines (cliffs.f:137-152) are mirrors: (-Q, -P, v, d)
 // of the adjacent wet node is the node with u -> -u and c, d unchanged.
+//
+// Rounding is fixed by the source: every file is compiled with FP
+// contraction off, and each fused multiply-add is written as fmaf.  The
+// device kernels (fused or not) and the host emulator therefore round every
+// operation identically: their FP32 results agree bit for bit.  Divisions
+// and square roots are the correctly rounded IEEE operations on both.
 #pragma once
 
 #if defined(__HIPCC__)
@@ -64,27 +70,22 @@ struct Metric {
     float scl;   // zeta(i) in the y-sweep, 0 in the x-sweep
 };
 
-// Reciprocal: hardware v_rcp_f32 (1 ulp) on the device, a division on the host.
-CLIFFS_HD float fast_rcp(float x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __builtin_amdgcn_rcpf(x);
-#else
-    return 1.0f / x;
-#endif
-}
-
 CLIFFS_HD float c0_of(float s) { return fmaxf(s, 0.0f); }
 CLIFFS_HD float cel_of(const Node& n) { return c0_of(n.s) + n.cp; }
 CLIFFS_HD float geta_of(float s, float cp) {
     const float sn = fminf(s, 0.0f);
-    return fmaf(cp, 2.0f * c0_of(s) + cp, sn * sn);
+    return fmaf(cp, fmaf(2.0f, c0_of(s), cp), sn * sn);
 }
+// g*eta over deep water (s > 0): the background term min(s,0)^2 is 0
+CLIFFS_HD float geta_deep(float s, float cp) { return cp * fmaf(2.0f, s, cp); }
+// u^2 + v^2 as every kernel and the emulator round it
+CLIFFS_HD float speed2(float u, float v) { return fmaf(u, u, v * v); }
 CLIFFS_HD bool is_land(float c, const Params& p) { return c <= p.celmin; }
 // d > ground  <=>  s > celmin for the device bathymetry (cliffs.f:221)
 CLIFFS_HD bool is_deep(float s, const Params& p) { return s > p.celmin; }
 // eta_a + d_b > ground, in g-scaled form (cliffs.f:77, 92, 288, 309)
 CLIFFS_HD bool floods(float geta_a, float s_b, const Params& p) {
-    return geta_a + s_b * fabsf(s_b) > p.gground;
+    return fmaf(s_b, fabsf(s_b), geta_a) > p.gground;
 }
 CLIFFS_HD Node mirror(const Node& n) { return Node{n.s, n.cp, -n.u, n.v}; }
 
@@ -126,8 +127,7 @@ CLIFFS_HD void interior_update(const Node& L, float cL, const Node& X, float cX,
     float F = 0.0f;  // almost-Manning friction g n^2 u sqrt((u^2+v^2)/H)/H, H = c^2/g
     if (p.friction) {
         // = g n^2 * g * u * sqrt(g (u^2+v^2)) / c^3   (c >= celmin > 0 here)
-        const float rc = fast_rcp(c);
-        F = p.gn2 * p.grav * u * sqrtf(p.grav * (u * u + v * v)) * (rc * rc * rc);
+        F = (p.gn2 * p.grav * u * sqrtf(p.grav * speed2(u, v))) / (c * c * c);
     }
     const float S = kScl ? 8.0f * c * u * m.scl : 0.0f;  // (P-Q)(P+Q) zeta
 
@@ -140,21 +140,24 @@ CLIFFS_HD void interior_update(const Node& L, float cL, const Node& X, float cX,
     const float qjmQ = em.eq * m.rfm;
     float qsQ = 0.5f * (qpjQ + qjmQ);
 
-    // transcritical branch (cliffs.f:228-229, 241-242); rare.  On the device
-    // the test is one wave vote, so the common case costs no divergent branch.
-    const bool tcP = W && (L.u + cL < 0.0f) && (R.u + cR > 0.0f);
-    const bool tcQ = W && (L.u - cL < 0.0f) && (R.u - cR > 0.0f);
+    // transcritical branch (cliffs.f:228-229, 241-242); rare.  P needs
+    // u + c < 0 on the left (e1 = 4(u1+c1) < 0), Q needs u - c > 0 on the
+    // right (e2 = 4(u2-c2) > 0): one wave vote on that screen skips the branch
+    // in the common case, without a divergent branch.
+    const bool screen = (L.u + cL < 0.0f) || (R.u - cR > 0.0f);
 #if defined(__HIP_DEVICE_COMPILE__)
-    if (__ballot(tcP || tcQ) != 0) {
+    if (__ballot(screen) != 0) {
 #else
-    if (tcP || tcQ) {
+    if (screen) {
 #endif
+        const bool tcP = W && (L.u + cL < 0.0f) && (R.u + cR > 0.0f);
+        const bool tcQ = W && (L.u - cL < 0.0f) && (R.u - cR > 0.0f);
         const float duRL = R.u - L.u;
         const float dcpRL = R.cp - L.cp;
         const float dsRL = R.s - L.s;
         const float wP = u + c, wQ = u - c;
-        const float vP = (wP * duRL + 2.0f * wP * dcpRL + 2.0f * (u + X.cp) * dsRL) * m.rsum;
-        const float vQ = (wQ * duRL - 2.0f * wQ * dcpRL - 2.0f * (u - X.cp) * dsRL) * m.rsum;
+        const float vP = fmaf(2.0f * (u + X.cp), dsRL, fmaf(2.0f * wP, dcpRL, wP * duRL)) * m.rsum;
+        const float vQ = fmaf(-2.0f * (u - X.cp), dsRL, fmaf(-2.0f * wQ, dcpRL, wQ * duRL)) * m.rsum;
         qsP = tcP ? vP : qsP;
         qsQ = tcQ ? vQ : qsQ;
     }
@@ -163,18 +166,18 @@ CLIFFS_HD void interior_update(const Node& L, float cL, const Node& X, float cX,
 
     // P' = P - dt (qsP + F - S), Q' = Q - dt (qsQ + F + S);
     // c' = (P'-Q')/4, u' = (P'+Q')/2
-    cp_out = X.cp - 0.25f * p.dt * ((qsP - qsQ) - 2.0f * S);
-    u_out = u - 0.5f * p.dt * ((qsP + qsQ) + 2.0f * F);
+    cp_out = fmaf(-0.25f * p.dt, fmaf(-2.0f, S, qsP - qsQ), X.cp);
+    u_out = fmaf(-0.5f * p.dt, fmaf(2.0f, F, qsP + qsQ), u);
 
     // tangential velocity advection (cliffs.f:249-250)
-    const float B = (R.v - v) * m.rfp - (v - L.v) * m.rfm;
-    v_out = v - u * dtx * ((R.v - L.v) - B * fmaf(u, p.dt, m.fpm));
+    const float B = fmaf(R.v - v, m.rfp, -((v - L.v) * m.rfm));
+    v_out = fmaf(-(u * dtx), fmaf(-B, fmaf(u, p.dt, m.fpm), R.v - L.v), v);
 }
 
 // Incoming characteristic 2 (sqrt(g (eta_b + d)) - max(s,0)) from boundary
 // elevation eta_b (cliffs.f:299, 320), without cancellation in deep water.
 CLIFFS_HD float incoming_2dc(float geta_b, float s) {
-    const float X = geta_b + s * fabsf(s);
+    const float X = fmaf(s, fabsf(s), geta_b);
     const float r = sqrtf(fmaxf(X, 0.0f));
     return 2.0f * ((s > 0.0f) ? geta_b / (r + s) : r);
 }
@@ -186,11 +189,11 @@ CLIFFS_HD void edge_first(const Node& X, float cX, const Node& N, EdgeRes res, f
                           float scl, float ub, float vb, float geb, const Params& p,
                           float& cp_out, float& u_out, float& v_out) {
     const float S = 8.0f * cX * X.u * scl;
-    const float qn = (X.u - 2.0f * X.cp) - p.dt * fmaf(res.eq, rdx, S);
+    const float qn = fmaf(-p.dt, fmaf(res.eq, rdx, S), fmaf(-2.0f, X.cp, X.u));
     const float pn = ub + incoming_2dc(geb, X.s);
     cp_out = 0.25f * (pn - qn);
     u_out = 0.5f * (pn + qn);
-    v_out = (X.u < 0.0f) ? X.v - p.dt * (N.v - X.v) * X.u * rdx : vb;
+    v_out = (X.u < 0.0f) ? fmaf(-(p.dt * (N.v - X.v) * X.u), rdx, X.v) : vb;
 }
 
 // Open/forced edge at the last node (cliffs.f:309-329).  X = node n, N =
@@ -199,11 +202,11 @@ CLIFFS_HD void edge_last(const Node& X, float cX, const Node& N, EdgeRes res, fl
                          float scl, float ub, float vb, float geb, const Params& p,
                          float& cp_out, float& u_out, float& v_out) {
     const float S = 8.0f * cX * X.u * scl;
-    const float pn = (X.u + 2.0f * X.cp) - p.dt * (res.ep * rdx - S);
+    const float pn = fmaf(-p.dt, fmaf(res.ep, rdx, -S), fmaf(2.0f, X.cp, X.u));
     const float qn = ub - incoming_2dc(geb, X.s);
     cp_out = 0.25f * (pn - qn);
     u_out = 0.5f * (pn + qn);
-    v_out = (X.u > 0.0f) ? X.v - p.dt * (X.v - N.v) * X.u * rdx : vb;
+    v_out = (X.u > 0.0f) ? fmaf(-(p.dt * (X.v - N.v) * X.u), rdx, X.v) : vb;
 }
 
 // Shoreline surface-angle limiter (cliffs.f:264-285).  For a newly flooded

@@ -32,6 +32,7 @@ module cliffs_gpu_mod
   public :: cliffs_stream, cliffs_info, cliffs_last_error, cliffs_health
   public :: cliffs_nest_maps, cliffs_nest_create, cliffs_nest_destroy, cliffs_nest_feed
   public :: cliffs_nest_couple, cliffs_nest_record, cliffs_nest_end_feed, cliffs_nest_child_step
+  public :: cliffs_nest_child_start
   public :: cliffs_run_gauges, cliffs_set_fused, cliffs_set_tile_rows
   public :: cliffs_ensemble_create, cliffs_ensemble_destroy, cliffs_ensemble_run
   public :: cliffs_ensemble_run_timed, cliffs_ensemble_tune, cliffs_ensemble_sync
@@ -189,6 +190,12 @@ module cliffs_gpu_mod
     integer(c_int) function cliffs_nest_end_feed(nest) bind(C, name="cliffs_nest_end_feed")
       import :: c_int, c_ptr
       type(c_ptr), value :: nest
+    end function
+    integer(c_int) function cliffs_nest_child_start(nest, cuke, t0) bind(C, name="cliffs_nest_child_start")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: nest
+      real(c_double), value :: cuke
+      real(c_double), intent(out) :: t0
     end function
     integer(c_int) function cliffs_nest_child_step(nest, t) bind(C, name="cliffs_nest_child_step")
       import :: c_int, c_ptr, c_double

@@ -239,7 +239,7 @@ def ssl_condition(dep, depmin=1.0, alpha=2.0, passes=50):
 
 
 def shelf_coast(nx, ny, seed, dx, source_amp, source_len, steps=4000, cfl=0.7, source_seed=None,
-                source_x=(0.35, 0.55), dt=None):
+                source_x=(0.35, 0.55), dt=None, ground=0.01, crough=0.0):
     """Cartesian shelf and coastline for ensemble scenarios: fractal abyssal
     plain 4-6 km, a continental slope and a shelf whose sqrt-depth varies
     linearly (200 m shelf, 2 m at the shore), land rising to +300 m behind a
@@ -247,7 +247,8 @@ def shelf_coast(nx, ny, seed, dx, source_amp, source_len, steps=4000, cfl=0.7, s
     (+A, -A/2) of the given length with seeded strike and location (x in
     source_x of the domain width).  The terrain follows `seed`; the source
     follows it too, or `source_seed` when given (one site, many sources).
-    `dt` overrides the CFL time step."""
+    `dt` overrides the CFL time step; `ground` is the minimal flow depth and
+    `crough` the Manning n^2 (cliffs.f:204)."""
     rng = np.random.default_rng(seed)
     x = np.arange(nx, dtype=np.float64) * dx
     y = np.arange(ny, dtype=np.float64) * dx
@@ -269,12 +270,11 @@ def shelf_coast(nx, ny, seed, dx, source_amp, source_len, steps=4000, cfl=0.7, s
     rs = rng if source_seed is None else np.random.default_rng(source_seed)
     src = dict(amplitude=source_amp, sx=rs.uniform(*source_x) * Lx, sy=rs.uniform(0.3, 0.7) * Ly,
                strike=rs.uniform(0, np.pi), a=min(0.5 * source_len, 0.2 * Lx))
-    ground = 0.01
     cel = depth_to_celerity(dep + dipole(X, Y, dep, src), ground)
     if dt is None:
         dt = cfl * dx / math.sqrt(GRAV * (float(dep.max()) + 2 * source_amp))
     z = np.zeros_like(dep)
-    return Case("shelf_coast", dep, cel, z.copy(), z.copy(), x, y, True, dt, ground, 0.0, steps,
+    return Case("shelf_coast", dep, cel, z.copy(), z.copy(), x, y, True, dt, ground, crough, steps,
                 dict(seed=seed, amplitude=source_amp, dx=dx, source_len=source_len, source=src))
 
 
@@ -292,20 +292,38 @@ def dipole(X, Y, dep, src):
     return np.where(dep > 0, eta, 0.0)
 
 
-def ensemble_scenario(seed, steps=4000):
+# Run parameters of the ensemble workload's coarse (200-1000 m) grids: the
+# reference's own choice for its 30" and 10" Okushiri grids (paramsOK30s.txt,
+# paramsOK10s.txt in Examples/OkushiriTsunami.zip: minimal flow depth 0.5-1 m,
+# Manning n^2 = 0.000625).  Without friction and with a 1 cm flow depth, thin
+# films on the synthetic coasts run up at 60-150 m/s and the run-up is
+# chaotic at rounding level (the FP64 oracle and any FP32 arithmetic part
+# after ~1,250 steps).
+ENSEMBLE_GROUND = 1.0
+ENSEMBLE_CROUGH = 0.000625
+
+
+def ensemble_scenario(seed, steps=4000, ground=ENSEMBLE_GROUND, crough=ENSEMBLE_CROUGH):
     """One scenario of the batched-ensemble workload: nx, ny drawn in
     [512, 2048], Cartesian dx in [200, 1000] m, shelf and seeded coastline
     (shelf_coast), dipole source of length 50-300 km (at most 0.4 of the
-    domain width), seeded strike and location, amplitude 0.5-8 m; run-up on."""
+    domain width), seeded strike and location, amplitude 0.5-8 m; run-up on,
+    with the minimal flow depth and friction above."""
     rng = np.random.default_rng(10_000 + seed)
     nx = int(rng.integers(512, 2049))
     ny = int(rng.integers(512, 2049))
     dx = float(rng.uniform(200.0, 1000.0))
     length = float(rng.uniform(50e3, 300e3))
     amp = float(rng.uniform(0.5, 8.0))
-    c = shelf_coast(nx, ny, seed, dx, amp, length, steps=steps)
+    c = shelf_coast(nx, ny, seed, dx, amp, length, steps=steps, ground=ground, crough=crough)
     c.name = "ensemble"
     return c
+
+
+def land_cell_rise(case):
+    """Ground height of one land cell of a shelf_coast grid: land rises 300 m
+    over a quarter of the domain width, i.e. 300 / (0.25 (nx - 1)) m per cell."""
+    return 300.0 / (0.25 * (case.dep.shape[1] - 1))
 
 
 def max_runup(hmax, dep, celmin):

@@ -63,6 +63,8 @@ typedef struct cliffs_grid_desc {
  * MASTER arrays (cliffs_main.f:191-197) and precomputes celmin
  * (cliffs_main.f:227) and the device form of dep/s1/s2/zeta. */
 int cliffs_create(const cliffs_grid_desc *desc, cliffs_handle **out);
+/* Frees the grid.  Fails with CLIFFS_ERR_STATE (and frees nothing) while a
+ * nest (as parent or coupled child) or an ensemble still uses it. */
 int cliffs_destroy(cliffs_handle *h);
 
 /* cel (celerity sqrt(g H), cliffs_main.f:218-226), xvel, yvel: FP64
@@ -124,12 +126,13 @@ int cliffs_run_gauges(cliffs_handle *h, int every, int nsamples, int n, const in
  * bytes of HBM traffic per cell.  No counterpart in the reference, whose
  * step is always two sweeps (cliffs_main.f:317-345). */
 int cliffs_set_fused(cliffs_handle *h, int on);
-/* Tile height of the fused step: at most rows = 8..1024 (the grid's rows are
+/* Tile height of the fused step: at most rows = 8..256 (the grid's rows are
  * split into as many chunks as that height needs, of equal height), or 0
  * for the automatic choice (enough tiles to fill the device once, at least
  * 32 rows).  Results do not depend on it; small grids step faster with
- * short tiles (cliffs_ensemble_tune picks one by timing).  No counterpart
- * in the reference. */
+ * short tiles (cliffs_ensemble_tune picks one by timing).  Fails with
+ * CLIFFS_ERR_STATE for a member of an ensemble (its tuner owns the tile
+ * height).  No counterpart in the reference. */
 int cliffs_set_tile_rows(cliffs_handle *h, int rows);
 
 /* ---- ensembles: many grids stepped together ---------------------------- */
@@ -151,7 +154,7 @@ int cliffs_ensemble_run(cliffs_ensemble *e, int nsteps);
 /* As cliffs_ensemble_run; ms_total = device time of the nsteps. */
 int cliffs_ensemble_run_timed(cliffs_ensemble *e, int nsteps, double *ms_total);
 /* Autotuner: times trial_steps repetitions of one ensemble step for each
- * candidate tile height (16 .. 1024 rows of 248 columns) and keeps the
+ * candidate tile height (16 .. 256 rows of 248 columns) and keeps the
  * fastest (rows_out).  The trials change neither the state nor the running
  * maxima (their max-wave update is switched off). */
 int cliffs_ensemble_tune(cliffs_ensemble *e, int trial_steps, int *rows_out);
@@ -213,7 +216,8 @@ int cliffs_nest_feed(cliffs_nest *n, double cuke, double hmin, int *iarv, int *f
  * reference, which runs parent and child as separate processes. */
 int cliffs_nest_couple(cliffs_nest *n, cliffs_handle *child, int capacity);
 /* (Destroy a nest before its parent and child grids, and an ensemble before
- * its members: they use the grids' device memory and streams.) */
+ * its members: they use the grids' device memory and streams;
+ * cliffs_destroy refuses a grid still in use.) */
 /* feed_children_grid at parent time t: arrival test (the latch iarv as in
  * cliffs_nest_feed), then, once arrived, one more feed record (*nrec = records
  * so far).  Fails with CLIFFS_ERR_STATE if the record would overwrite one
@@ -222,10 +226,22 @@ int cliffs_nest_record(cliffs_nest *n, double cuke, double hmin, double t, int *
 /* No more records will come: once past the last record the child's edges
  * become 0 (cliffs_main.f:306-310). */
 int cliffs_nest_end_feed(cliffs_nest *n);
+/* The child's start (get_boundary_feed.f:144-167, cliffs_main.f:188): the
+ * first feed record whose largest |eta| over the four edges is >= cuke (the
+ * child's own threshold, its parameter file's "still sea threshold"); *t0 =
+ * that record's time, where the child's clock starts.  Fails with
+ * CLIFFS_ERR_STATE while no record has reached cuke yet (call again after
+ * more records; records found below it are released from the ring), when
+ * the feed ended without one, or when only its last record reaches it (the
+ * reference's "bndr signal below threshold").  Without this call the child
+ * starts at the first record (the rule with cuke = 0). */
+int cliffs_nest_child_start(cliffs_nest *n, double cuke, double *t0);
 /* One child step at child time t (after t = t + dt): records irec-1, irec
- * with tinput(irec-1) <= t < tinput(irec), starting from the first record,
+ * with tinput(irec-1) <= t < tinput(irec), starting from the start record,
  * edges = rec(irec-1)*cc + rec(irec)*(1-cc), cc = (tinput(irec)-t) /
- * (tinput(irec)-tinput(irec-1)) (cliffs_main.f:305-339), then the step.
+ * (tinput(irec)-tinput(irec-1)) (cliffs_main.f:305-339), then the step;
+ * irec counts from the child's start record (cliffs_nest_child_start), and
+ * t must not precede that record's time.
  * Fails with CLIFFS_ERR_STATE if t is beyond the parent's last record and
  * the feed has not ended. */
 int cliffs_nest_child_step(cliffs_nest *n, double t);

@@ -32,8 +32,10 @@ def build_emulator():
             os.path.getmtime(src),
             os.path.getmtime(os.path.join(PKG, "csrc", "vtcs_node.h")),
             os.path.getmtime(os.path.join(PKG, "csrc", "convert.h"))):
-        subprocess.run(["g++", "-O2", "-march=x86-64-v3", "-fPIC", "-shared", "-std=c++17",
-                        src, "-o", EMUL_SO], check=True)
+        # contraction off, as the library is built: the emulator rounds every
+        # operation as the device does
+        subprocess.run(["g++", "-O2", "-march=x86-64-v3", "-ffp-contract=off", "-fPIC", "-shared",
+                        "-std=c++17", src, "-o", EMUL_SO], check=True)
     return EMUL_SO
 
 

@@ -1,0 +1,54 @@
+"""Writes tests/golden/ensemble_runup.json: maximum run-up and flooded land
+cells of full-size scenarios of the batched-ensemble workload
+(synth.ensemble_scenario, 4000 steps each), from the FP64 oracle
+(oracle/cliffs_oracle.c, bit-exact to the reference's cliffs.f).
+
+    OMP_NUM_THREADS=8 python tests/golden/make_ensemble_runup.py [seed ...]
+
+The GPU test test_ensemble_runup_parity compares the device against these
+values; an oracle run of one scenario takes 1-3 minutes on 8 cores, too long
+to repeat inside a GPU test."""
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path[:0] = [os.path.join(ROOT, "cliffs-src_amd"), os.path.join(ROOT, "oracle")]
+
+import numpy as np  # noqa: E402
+
+import oracle  # noqa: E402
+import synth  # noqa: E402
+
+OUT = os.path.join(HERE, "ensemble_runup.json")
+SEEDS = [0, 1, 3, 5, 131]
+
+
+def run(seed, steps=4000):
+    case = synth.ensemble_scenario(seed, steps)
+    s1, s2, z = case.metrics()
+    orc = oracle.Oracle(case.dep, s1, s2, z, dt=case.dt, ground=case.ground, crough=case.crough)
+    orc.set_state(case.cel, case.xvel, case.yvel)
+    t0 = time.time()
+    orc.steps(steps)
+    cm = orc.celmin
+    fl = (orc.hmax > cm) & (case.dep < 0)
+    c = orc.cel
+    return dict(seed=seed, shape=list(case.dep.shape), steps=steps, ground=case.ground,
+                crough=case.crough, dt=case.dt,
+                max_runup_m=synth.max_runup(orc.hmax, case.dep, cm),
+                flooded_land_cells=int(fl.sum()),
+                land_cell_rise_m=synth.land_cell_rise(case),
+                finite=bool(np.isfinite(c).all()),
+                oracle_s=round(time.time() - t0, 1))
+
+
+if __name__ == "__main__":
+    seeds = [int(x) for x in sys.argv[1:]] or SEEDS
+    have = json.load(open(OUT)) if os.path.exists(OUT) else {}
+    for s in seeds:
+        have[str(s)] = run(s)
+        print(json.dumps(have[str(s)]), flush=True)
+        json.dump(dict(sorted(have.items(), key=lambda kv: int(kv[0]))), open(OUT, "w"), indent=1)

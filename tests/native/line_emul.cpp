@@ -1,8 +1,11 @@
 // Test-only host emulator of the device sweep algorithm.
 //
-// Runs the same FP32 node arithmetic as the HIP kernels (vtcs_node.h) and
-// the same local-rule form of the wet/dry front logic, line by line on the
-// CPU.  The test-suite uses it to (a) check the local rules against the
+// Runs the same FP32 node arithmetic as the HIP kernels (vtcs_node.h), the
+// same per-node metric tables (separable x spacing, FP32 reciprocals built
+// as the library builds them) and the same local-rule form of the wet/dry
+// front logic, line by line on the CPU.  With contraction off on both sides
+// and every fused multiply-add explicit, its FP32 state equals the device's
+// bit for bit.  The test-suite uses it to (a) check the local rules against the
 // sequential segment scan of cliffs.f:65-123 as the oracle executes it and
 // (b) measure the FP32 error budget against the FP64 oracle without a GPU.
 // It is not part of the product library and nothing in the package loads it.
@@ -18,11 +21,19 @@ using namespace cliffs;
 
 namespace {
 
+struct F4 {
+    float x, y, z, w;
+};
+
 struct Emul {
     int nx, ny;
     std::vector<double> dep;
     std::vector<float> S, CP, U, V, HM, VM;
-    std::vector<double> s1, s2, zeta;  // FP64 metrics as the reference holds them
+    // the library's metric tables: per column (1/b(i-1), 1/b(i), 1/(b(i-1)+b(i)),
+    // b(i)-b(i-1)), per row (1/a(j), a(j)) with s1(i,j) = a(j) b(i); per row
+    // of the y-sweep the same four from s2, and zeta
+    std::vector<F4> xm, ym;
+    std::vector<float> xra, xrb, zt;
     Params p;
     Consts64 k;
 };
@@ -31,11 +42,14 @@ struct BC {
     float ub, vb, geb;
 };
 
-// One line: nodes nd[0..n-1] (u normal, v tangential), spacings dx[0..n-2],
-// metric term scl[0..n-1].  Writes results back into nd.  Returns false if
-// the line had no wet node and was left untouched (cliffs.f:113).
-bool sweep_line(const Params& p, int n, Node* nd, const double* dx, const double* scl, BC bl,
-                BC br) {
+// One line: nodes nd[0..n-1] (u normal, v tangential), per-node metric
+// mt[0..n-1], edge spacings 1/dx(1) and 1/dx(n-1), metric term at the ends.
+// Writes results back into nd.  Returns false if the line had no wet node and
+// was left untouched (cliffs.f:113).
+// kScl: the y-sweep's metric term (the x-sweep drops it, as the kernels do).
+template <bool kScl>
+bool sweep_line(const Params& p, int n, Node* nd, const Metric* mt, float rdx0, float rdx1,
+                float scl0, float scl1, BC bl, BC br) {
     std::vector<float> c(n), ge(n);
     std::vector<char> land(n), RE(n, 0), LE(n, 0), start(n, 0), nl(n);
     bool any = false;
@@ -83,13 +97,8 @@ bool sweep_line(const Params& p, int n, Node* nd, const double* dx, const double
         const EdgeRes em = gl ? ghost_left_res(pc[i], X.u) : er[i - 1];
         const EdgeRes ep = gr ? ghost_right_res(pc[i], X.u) : er[i];
         const bool W = is_deep(X.s, p) && is_deep(L.s, p) && is_deep(R.s, p);
-        Metric m;
-        m.rfm = (float)(1.0 / dx[i - 1]);
-        m.rfp = (float)(1.0 / dx[i]);
-        m.rsum = (float)(1.0 / (dx[i - 1] + dx[i]));
-        m.fpm = (float)(dx[i] - dx[i - 1]);
-        m.scl = (float)scl[i];
-        interior_update(L, cL, X, pc[i], R, cR, em, ep, W, m, p, out[i].cp, out[i].u, out[i].v);
+        interior_update<kScl>(L, cL, X, pc[i], R, cR, em, ep, W, mt[i], p, out[i].cp, out[i].u,
+                              out[i].v);
     }
     // limiter: left cases (LE_i, newland_{i-1}), then right (RE_i, newland_{i+1})
     for (int i = 1; i < n; i++) {
@@ -106,12 +115,11 @@ bool sweep_line(const Params& p, int n, Node* nd, const double* dx, const double
     }
     // edges
     if (!nl[0] && !nl[1] && floods(bl.geb, pe[0].s, p))
-        edge_first(pe[0], pc[0], pe[1], er[0], (float)(1.0 / dx[0]), (float)scl[0], bl.ub, bl.vb,
-                   bl.geb, p, out[0].cp, out[0].u, out[0].v);
+        edge_first(pe[0], pc[0], pe[1], er[0], rdx0, scl0, bl.ub, bl.vb, bl.geb, p, out[0].cp,
+                   out[0].u, out[0].v);
     if (!nl[n - 2] && !nl[n - 1] && floods(br.geb, pe[n - 1].s, p))
-        edge_last(pe[n - 1], pc[n - 1], pe[n - 2], er[n - 2], (float)(1.0 / dx[n - 2]),
-                  (float)scl[n - 1], br.ub, br.vb, br.geb, p, out[n - 1].cp, out[n - 1].u,
-                  out[n - 1].v);
+        edge_last(pe[n - 1], pc[n - 1], pe[n - 2], er[n - 2], rdx1, scl1, br.ub, br.vb, br.geb, p,
+                  out[n - 1].cp, out[n - 1].u, out[n - 1].v);
     for (int i = 0; i < n; i++) nd[i] = out[i];
     return true;
 }
@@ -136,9 +144,34 @@ void* emul_create(int nx, int ny, const double* dep, const double* s1, const dou
     e->HM.resize(m);
     for (size_t i = 0; i < m; i++) e->HM[i] = -c0f(e->S[i]);
     e->VM.assign(m, 0.0f);
-    e->s1.assign(s1, s1 + (size_t)(nx > 1 ? nx - 1 : 1) * ny);
-    e->s2.assign(s2, s2 + (ny > 1 ? ny - 1 : 1));
-    e->zeta.assign(zeta, zeta + ny);
+    // metric tables exactly as cliffs_create builds them
+    e->xm.assign(nx, F4{0.f, 0.f, 0.f, 0.f});
+    e->xra.assign(ny, 1.0f);
+    e->xrb.assign(ny, 1.0f);
+    if (nx > 2) {
+        const int l = nx - 1;
+        std::vector<double> b(s1, s1 + l);
+        for (int i = 0; i < nx; i++) {
+            const double bm = i >= 1 ? b[i - 1] : b[0];
+            const double bp = i < l ? b[i] : b[l - 1];
+            e->xm[i] = F4{(float)(1.0 / bm), (float)(1.0 / bp), (float)(1.0 / (bm + bp)), (float)(bp - bm)};
+        }
+        for (int j = 0; j < ny; j++) {
+            const double aj = s1[(size_t)l * j] / b[0];
+            e->xra[j] = (float)(1.0 / aj);
+            e->xrb[j] = (float)aj;
+        }
+    }
+    e->ym.assign(ny, F4{0.f, 0.f, 0.f, 0.f});
+    e->zt.assign(ny, 0.0f);
+    if (ny > 2) {
+        for (int j = 0; j < ny; j++) {
+            const double bm = j >= 1 ? s2[j - 1] : s2[0];
+            const double bp = j < ny - 1 ? s2[j] : s2[ny - 2];
+            e->ym[j] = F4{(float)(1.0 / bm), (float)(1.0 / bp), (float)(1.0 / (bm + bp)), (float)(bp - bm)};
+            e->zt[j] = (float)zeta[j];
+        }
+    }
     e->p.dt = (float)dt;
     e->p.grav = 9.8f;
     e->p.celmin = e->k.celmin32;
@@ -188,14 +221,20 @@ void emul_sweep(void* h, int iu, int first, int last, int mx, const double* e1, 
         return b;
     };
     std::vector<Node> nd(nx > ny ? nx : ny);
-    std::vector<double> zero(nx > ny ? nx : ny, 0.0);
+    std::vector<Metric> mt(nx > ny ? nx : ny);
     if (iu == 1) {
         for (int j = first; j <= last; j++) {
             const size_t row = (size_t)nx * (j - 1);
-            for (int i = 0; i < nx; i++)
+            for (int i = 0; i < nx; i++) {
                 nd[i] = Node{e->S[row + i], e->CP[row + i], e->U[row + i], e->V[row + i]};
-            if (sweep_line(e->p, nx, nd.data(), &e->s1[(size_t)(nx - 1) * (j - 1)], zero.data(),
-                           bc(e1, j), bc(e2, j))) {
+                const float ra = e->xra[j - 1];
+                mt[i] = Metric{e->xm[i].x * ra, e->xm[i].y * ra, e->xm[i].z * ra,
+                               e->xm[i].w * e->xrb[j - 1], 0.0f};
+            }
+            const float rdx0 = e->xm[0].y * e->xra[j - 1];
+            const float rdx1 = e->xm[nx - 1].x * e->xra[j - 1];
+            if (sweep_line<false>(e->p, nx, nd.data(), mt.data(), rdx0, rdx1, 0.0f, 0.0f, bc(e1, j),
+                           bc(e2, j))) {
                 for (int i = 0; i < nx; i++) {
                     e->CP[row + i] = nd[i].cp;
                     e->U[row + i] = nd[i].u;
@@ -208,9 +247,10 @@ void emul_sweep(void* h, int iu, int first, int last, int mx, const double* e1, 
             for (int j = 0; j < ny; j++) {
                 const size_t o = (size_t)(i - 1) + (size_t)nx * j;
                 nd[j] = Node{e->S[o], e->CP[o], e->V[o], e->U[o]};
+                mt[j] = Metric{e->ym[j].x, e->ym[j].y, e->ym[j].z, e->ym[j].w, e->zt[j]};
             }
-            if (sweep_line(e->p, ny, nd.data(), e->s2.data(), e->zeta.data(), bc(e1, i),
-                           bc(e2, i))) {
+            if (sweep_line<true>(e->p, ny, nd.data(), mt.data(), e->ym[0].y, e->ym[ny - 1].x, e->zt[0],
+                           e->zt[ny - 1], bc(e1, i), bc(e2, i))) {
                 for (int j = 0; j < ny; j++) {
                     const size_t o = (size_t)(i - 1) + (size_t)nx * j;
                     e->CP[o] = nd[j].cp;
@@ -227,7 +267,7 @@ void emul_maxwave(void* h) {
     const size_t m = (size_t)e->nx * e->ny;
     for (size_t c = 0; c < m; c++) {
         if (e->CP[c] > e->HM[c]) e->HM[c] = e->CP[c];
-        const float vv = e->U[c] * e->U[c] + e->V[c] * e->V[c];
+        const float vv = speed2(e->U[c], e->V[c]);
         if (vv > e->VM[c]) e->VM[c] = vv;
     }
 }

@@ -124,3 +124,103 @@ def test_ring_capacity_guard(gpu_lib):
     n.close()
     g.close()
     c.close()
+
+
+def _record_wave(rec):
+    """Largest |eta| over the four edges of one feed record
+    (get_boundary_feed.f:147-157)."""
+    return max(float(np.abs(e[:, 2]).max()) for e in rec[1:5])
+
+
+def _host_start(recs, cuke):
+    """get_boundary_feed.f:148-163: the first record (1-based) whose wave
+    reaches cuke; None while no record does."""
+    for k, r in enumerate(recs, start=1):
+        if _record_wave(r) >= cuke:
+            return k
+    return None
+
+
+def test_child_start_rule(gpu_lib):
+    """The child starts at the first record whose edge |eta| reaches its own
+    threshold (get_boundary_feed.f:144-167) and its clock at that record's
+    time (cliffs_main.f:188); bit-identical to a host restatement of the
+    rule feeding cliffs_step.  Records found below the threshold are released
+    from the ring, so a ring of 4 holds the wait."""
+    case, maps, parent, child = _setup(gpu_lib)
+    cuke, cuke_child, bndout, cycles = 0.01, 1.3, 2, 30
+    gA, cA = parent(), child()
+    gB, cB = parent(), child()
+    nA, nB = gpu_lib.Nest(gA, maps), gpu_lib.Nest(gB, maps)
+    nB.couple(cB, capacity=4)
+    dtp, dtc = case.dt, case.dt / 3
+    recs, state = [], {"irec": 1, "bndr": True}
+    iarvA = iarvB = False
+    tp, tc, tcB, child_steps, startA = 0.0, None, None, 0, None
+    for _ in range(cycles):
+        gA.run(bndout)
+        gB.run(bndout)
+        tp += bndout * dtp
+        iarvA, fed, w, e, s, n = nA.feed(cuke, case.ground, iarvA)
+        if fed:
+            recs.append((tp, w, e, s, n))
+        iarvB, nrec = nB.record(cuke, case.ground, tp, iarvB)
+        assert nrec == len(recs)
+        if startA is None:
+            startA = _host_start(recs, cuke_child)
+            if startA is not None:
+                tc = recs[startA - 1][0]
+                state["irec"] = startA
+        if tcB is None and nrec:
+            try:
+                tcB = nB.child_start(cuke_child)
+            except gpu_lib.CliffsError as err:
+                assert err.code == -6 and startA is None
+        assert (tcB is None) == (startA is None)
+        if tc is None:
+            continue
+        assert tcB == tc
+        while tc + dtc < recs[-1][0]:
+            tc += dtc
+            cA.step(*_host_edges(recs, state, tc, cA.mx))
+            nB.child_step(tc)
+            child_steps += 1
+    assert startA is not None and startA >= 10, startA
+    assert child_steps >= 10
+    # a time before the start record is refused
+    with pytest.raises(gpu_lib.CliffsError):
+        nB.child_step(recs[startA - 1][0] - dtc)
+    a = cA.download() + cA.download_max()
+    b = cB.download() + cB.download_max()
+    for x, y, name in zip(a, b, ("cel", "xvel", "yvel", "hmax", "velmax")):
+        bad = int((x.view(np.int64) != y.view(np.int64)).sum())
+        assert bad == 0, f"child {name}: {bad} cells differ"
+    for g in (nA, nB, gA, gB, cA, cB):
+        g.close()
+
+
+def test_child_start_below_threshold(gpu_lib):
+    """A feed that ends without reaching the child's threshold is an error
+    (get_boundary_feed.f:160-162, 'bndr signal below threshold'), and so is
+    stepping the child before any record reaches it."""
+    case, maps, parent, child = _setup(gpu_lib)
+    g, c = parent(), child()
+    n = gpu_lib.Nest(g, maps)
+    n.couple(c, capacity=2)
+    t, iarv = 0.0, False
+    for _ in range(6):            # records below the threshold are released: no ring overflow
+        g.run(2)
+        t += 2 * case.dt
+        iarv, k = n.record(0.01, case.ground, t, iarv)
+    assert k == 6
+    with pytest.raises(gpu_lib.CliffsError) as ei:
+        n.child_start(1e3)
+    assert ei.value.code == -6
+    with pytest.raises(gpu_lib.CliffsError):
+        n.child_step(t)
+    n.end_feed()
+    with pytest.raises(gpu_lib.CliffsError) as ei:
+        n.child_start(1e3)
+    assert "ended below" in str(ei.value)
+    for x in (n, g, c):
+        x.close()
