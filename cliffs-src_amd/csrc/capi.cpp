@@ -1591,6 +1591,38 @@ int cliffs_nest_couple(cliffs_nest* n, cliffs_handle* child, int capacity) {
     return CLIFFS_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Start rule of get_boundary_feed.f:144-167 over the records written so
+// far: the first record whose largest edge |eta| reaches start_cuke, and an
+// error when the feed ended without one before its last record (it >= nTn).
+int nest_find_start(cliffs_nest* n) {
+    if (n->start > 0) return CLIFFS_OK;
+    const int nrec = (int)n->tinput.size();
+    while (n->scanned < nrec) {
+        const int k = n->scanned + 1, slot = (k - 1) % n->cap;
+        double w = 0.0;
+        HIPCHK(hipEventSynchronize(n->rec_ev[slot]));
+        HIPCHK(hipMemcpy(&w, n->dwave + slot, sizeof(double), hipMemcpyDeviceToHost));
+        if (w >= n->start_cuke) {
+            if (n->ended && k >= nrec)
+                return fail(CLIFFS_ERR_STATE, "only the feed's last record reaches the child's threshold");
+            n->start = k;
+            n->irec = std::max(n->irec, k);
+            return CLIFFS_OK;
+        }
+        n->scanned = k;
+    }
+    return fail(CLIFFS_ERR_STATE, n->ended ? "feed ended below the child's threshold"
+                                           : "no feed record reaches the child's threshold yet");
+}
+
+}  // namespace
+
+extern "C" {
+
 int cliffs_nest_record(cliffs_nest* n, double cuke, double hmin, double t, int* iarv, int* nrec) {
     if (!n || !iarv || !nrec) return fail(CLIFFS_ERR_ARG, "NULL argument");
     if (!n->child) return fail(CLIFFS_ERR_STATE, "nest not coupled to a child");
@@ -1621,6 +1653,12 @@ int cliffs_nest_record(cliffs_nest* n, double cuke, double hmin, double t, int* 
     HIPCHK(hipEventRecord(n->rec_ev[slot], h->st));
     n->tinput.push_back(t);
     *nrec = k;
+    // with the child's threshold known and its start not yet found, check
+    // this record now, so that records below the threshold leave the ring
+    if (n->start_set && n->start == 0) {
+        const int rc2 = nest_find_start(n);
+        if (rc2 && rc2 != CLIFFS_ERR_STATE) return rc2;
+    }
     return CLIFFS_OK;
 }
 
@@ -1632,33 +1670,6 @@ int cliffs_nest_end_feed(cliffs_nest* n) {
 
 }  // extern "C"
 
-namespace {
-
-// Start rule of get_boundary_feed.f:144-167 over the records written so
-// far: the first record whose largest edge |eta| reaches start_cuke, and an
-// error when the feed ended without one before its last record (it >= nTn).
-int nest_find_start(cliffs_nest* n) {
-    if (n->start > 0) return CLIFFS_OK;
-    const int nrec = (int)n->tinput.size();
-    while (n->scanned < nrec) {
-        const int k = n->scanned + 1, slot = (k - 1) % n->cap;
-        double w = 0.0;
-        HIPCHK(hipEventSynchronize(n->rec_ev[slot]));
-        HIPCHK(hipMemcpy(&w, n->dwave + slot, sizeof(double), hipMemcpyDeviceToHost));
-        if (w >= n->start_cuke) {
-            if (n->ended && k >= nrec)
-                return fail(CLIFFS_ERR_STATE, "only the feed's last record reaches the child's threshold");
-            n->start = k;
-            n->irec = std::max(n->irec, k);
-            return CLIFFS_OK;
-        }
-        n->scanned = k;
-    }
-    return fail(CLIFFS_ERR_STATE, n->ended ? "feed ended below the child's threshold"
-                                           : "no feed record reaches the child's threshold yet");
-}
-
-}  // namespace
 
 extern "C" {
 

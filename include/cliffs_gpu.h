@@ -230,8 +230,9 @@ int cliffs_nest_end_feed(cliffs_nest *n);
  * first feed record whose largest |eta| over the four edges is >= cuke (the
  * child's own threshold, its parameter file's "still sea threshold"); *t0 =
  * that record's time, where the child's clock starts.  Fails with
- * CLIFFS_ERR_STATE while no record has reached cuke yet (call again after
- * more records; records found below it are released from the ring), when
+ * CLIFFS_ERR_STATE while no record has reached cuke yet (from then on each
+ * new record is checked as cliffs_nest_record writes it, and records below
+ * the threshold leave the ring; call again for the start), when
  * the feed ended without one, or when only its last record reaches it (the
  * reference's "bndr signal below threshold").  Without this call the child
  * starts at the first record (the rule with cuke = 0). */

@@ -179,7 +179,7 @@ def test_child_start_rule(gpu_lib):
         assert (tcB is None) == (startA is None)
         if tc is None:
             continue
-        assert tcB == tc
+        assert tcB == recs[startA - 1][0]
         while tc + dtc < recs[-1][0]:
             tc += dtc
             cA.step(*_host_edges(recs, state, tc, cA.mx))
@@ -207,6 +207,8 @@ def test_child_start_below_threshold(gpu_lib):
     g, c = parent(), child()
     n = gpu_lib.Nest(g, maps)
     n.couple(c, capacity=2)
+    with pytest.raises(gpu_lib.CliffsError):
+        n.child_start(1e3)        # no record yet; the threshold is now known
     t, iarv = 0.0, False
     for _ in range(6):            # records below the threshold are released: no ring overflow
         g.run(2)
