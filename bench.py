@@ -82,7 +82,7 @@ def ensemble_bench(nscen, steps, seed0):
 
     def runup(gc):
         g, c = gc
-        return round(synth.max_runup(g.download_max()[0], c.dep, g.celmin), 2)
+        return synth.max_runup(g.download_max()[0], c.dep, g.celmin)
 
     # grids are created, uploaded and read back from host threads, one grid
     # (handle) per task: the library's calls release the interpreter lock, so
@@ -101,14 +101,42 @@ def ensemble_bench(nscen, steps, seed0):
     ens.close()
     for g in grids:
         g.close()
+    parity = runup_parity(seed0, nscen, steps, runups, cases)
     return {"scenarios": nscen, "steps": steps, "cells": cells, "tile_rows": rows,
             "wall_s": round(wall, 3), "device_ms": round(ms, 1),
             "scenarios_per_hr": round(nscen / (wall / 3600.0), 1),
             "gcell_updates_per_s": round(cells * steps / (ms * 1e-3) / 1e9, 2),
-            "max_runup_m": runups,
-            "sample": f"{nscen} scenarios (seeds {seed0}..{seed0 + nscen - 1}), {steps} steps each; "
-                      f"max run-up parity is tested at small sizes against the FP64 oracle "
-                      f"(within one cell)"}
+            "max_runup_m": [round(r, 2) for r in runups],
+            "runup_vs_oracle": parity,
+            "sample": f"{nscen} scenarios (seeds {seed0}..{seed0 + nscen - 1}), {steps} steps each, "
+                      f"friction n^2 = {cases[0].crough}, minimal flow depth {cases[0].ground} m"}
+
+
+def runup_parity(seed0, nscen, steps, runups, cases):
+    """Max run-up of the scenarios that the FP64 oracle has run at full size
+    (tests/golden/ensemble_runup.json, tests/golden/make_ensemble_runup.py)
+    against the device's: worst difference in land cells and relative."""
+    path = os.path.join(ROOT, "tests", "golden", "ensemble_runup.json")
+    if not os.path.exists(path):
+        return None
+    import json
+    import synth
+    fx = json.load(open(path))
+    rows = []
+    for k in range(nscen):
+        ref = fx.get(str(seed0 + k))
+        if ref is None or ref["steps"] != steps:
+            continue
+        d = runups[k] - ref["max_runup_m"]
+        rows.append((seed0 + k, round(runups[k], 2), round(ref["max_runup_m"], 2),
+                     round(abs(d) / synth.land_cell_rise(cases[k]), 3),
+                     round(abs(d) / max(ref["max_runup_m"], 1e-9), 4)))
+    if not rows:
+        return None
+    return {"seeds": [r[0] for r in rows], "device_m": [r[1] for r in rows],
+            "oracle_m": [r[2] for r in rows],
+            "max_cells_off": max(r[3] for r in rows), "max_rel_err": max(r[4] for r in rows),
+            "within_one_cell": sum(1 for r in rows if r[3] <= 1.0), "of": len(rows)}
 
 
 def nested_bench(steps, warmup):
