@@ -72,6 +72,8 @@ def load_library():
         "cliffs_run_gauges": ([H, C.c_int, C.c_int, C.c_int, _ip, _ip, C.POINTER(C.c_float)],
                               C.c_int),
         "cliffs_gauges": ([H, C.c_int, _ip, _ip, _dp, _dp, _dp], C.c_int),
+        "cliffs_run_gauges_noalloc": ([H, C.c_int, C.c_int, C.c_int, _ip, _ip, C.POINTER(C.c_float)],
+                                      C.c_int),
         "cliffs_health": ([H, _dp, C.POINTER(C.c_longlong)], C.c_int),
         "cliffs_ensemble_create": ([C.POINTER(H), C.c_int, C.POINTER(H)], C.c_int),
         "cliffs_ensemble_destroy": ([H], C.c_int),
@@ -247,15 +249,16 @@ class Grid:
                                              C.byref(y)))
         return t.value, x.value, y.value
 
-    def run_gauges(self, every, nsamples, I, J):
+    def run_gauges(self, every, nsamples, I, J, noalloc=False):
         """nsamples x every steps; -> (nsamples, n) FP32 surface elevation at
-        the gauges (1-based I, J) after every `every` steps (0 where dry)."""
+        the gauges (1-based I, J) after every `every` steps (0 where dry).
+        noalloc: through cliffs_run_gauges_noalloc (no device allocation)."""
         I = np.ascontiguousarray(I, dtype=np.int32)
         J = np.ascontiguousarray(J, dtype=np.int32)
         out = np.empty((int(nsamples), len(I)), dtype=np.float32)
-        _chk(load_library().cliffs_run_gauges(self._h, int(every), int(nsamples), len(I),
-                                              I.ctypes.data_as(_ip), J.ctypes.data_as(_ip),
-                                              out.ctypes.data_as(C.POINTER(C.c_float))))
+        f = load_library().cliffs_run_gauges_noalloc if noalloc else load_library().cliffs_run_gauges
+        _chk(f(self._h, int(every), int(nsamples), len(I), I.ctypes.data_as(_ip),
+               J.ctypes.data_as(_ip), out.ctypes.data_as(C.POINTER(C.c_float))))
         return out
 
     def gauges(self, I, J):

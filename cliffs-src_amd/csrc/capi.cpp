@@ -24,6 +24,8 @@ hipError_t launch_line_flags(const DevGrid& g, const Params& p, unsigned char* f
                              unsigned* fclr, int nclr, hipStream_t st);
 hipError_t launch_maxwave(const DevGrid& g, hipStream_t st);
 hipError_t launch_copy_lines(const DevGrid& g, int dir, int first, int last, hipStream_t st);
+hipError_t gauge_scratch(int dev, void** ptr);
+size_t gauge_scratch_words();
 hipError_t launch_gauge_sample(const DevGrid& g, const Params& p, const int* I, const int* J, int n,
                                float* series, int slot, hipStream_t st);
 hipError_t launch_reset_max(const DevGrid& g, hipStream_t st);
@@ -948,14 +950,48 @@ int cliffs_run_timed(cliffs_handle* h, int nsteps, double* ms_total, double* ms_
     return CLIFFS_OK;
 }
 
-int cliffs_run_gauges(cliffs_handle* h, int every, int nsamples, int n, const int* I, const int* J,
-                      float* eta) {
+}  // extern "C"
+
+namespace {
+
+int gauge_args(cliffs_handle* h, int every, int nsamples, int n, const int* I, const int* J,
+               float* eta) {
     if (!h || every < 1 || nsamples < 1 || n < 1 || !I || !J || !eta)
         return fail(CLIFFS_ERR_ARG, "bad gauge-series arguments");
     if (!h->state) return fail(CLIFFS_ERR_STATE, "no state uploaded");
     for (int q = 0; q < n; q++)
         if (I[q] < 1 || I[q] > h->nx || J[q] < 1 || J[q] > h->ny)
             return fail(CLIFFS_ERR_ARG, "gauge index out of grid");
+    return CLIFFS_OK;
+}
+
+// The steps and samples of a gauge series into device buffers dij (2n ints)
+// and ds (n * nsamples floats), the series copied to eta; waits for the end.
+int run_gauges_into(cliffs_handle* h, int every, int nsamples, int n, const int* I, const int* J,
+                    float* eta, int* dij, float* ds) {
+    int rc = CLIFFS_OK;
+    hipError_t e = hipMemcpyAsync(dij, I, sizeof(int) * n, hipMemcpyHostToDevice, h->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dij + n, J, sizeof(int) * n, hipMemcpyHostToDevice, h->st);
+    for (int k = 0; k < nsamples && e == hipSuccess && rc == CLIFFS_OK; k++) {
+        for (int s = 0; s < every && rc == CLIFFS_OK; s++) rc = do_step(h, nullptr, nullptr);
+        if (rc == CLIFFS_OK) e = launch_gauge_sample(h->grid(), h->p, dij, dij + n, n, ds, k, h->st);
+    }
+    if (e == hipSuccess && rc == CLIFFS_OK)
+        e = hipMemcpyAsync(eta, ds, sizeof(float) * (size_t)n * nsamples, hipMemcpyDeviceToHost, h->st);
+    if (e == hipSuccess && rc == CLIFFS_OK) e = hipStreamSynchronize(h->st);
+    if (rc) return rc;
+    if (e != hipSuccess) return hipfail(e, "gauge series");
+    return CLIFFS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cliffs_run_gauges(cliffs_handle* h, int every, int nsamples, int n, const int* I, const int* J,
+                      float* eta) {
+    int rc = gauge_args(h, every, nsamples, n, I, J, eta);
+    if (rc) return rc;
     HIPCHK(hipSetDevice(h->dev));
     int* dij = nullptr;
     float* ds = nullptr;
@@ -968,20 +1004,27 @@ int cliffs_run_gauges(cliffs_handle* h, int every, int nsamples, int n, const in
         cleanup();
         return fail(CLIFFS_ERR_NOMEM, "gauge series allocation");
     }
-    int rc = CLIFFS_OK;
-    hipError_t e = hipMemcpyAsync(dij, I, sizeof(int) * n, hipMemcpyHostToDevice, h->st);
-    if (e == hipSuccess) e = hipMemcpyAsync(dij + n, J, sizeof(int) * n, hipMemcpyHostToDevice, h->st);
-    for (int k = 0; k < nsamples && e == hipSuccess && rc == CLIFFS_OK; k++) {
-        for (int s = 0; s < every && rc == CLIFFS_OK; s++) rc = do_step(h, nullptr, nullptr);
-        if (rc == CLIFFS_OK) e = launch_gauge_sample(h->grid(), h->p, dij, dij + n, n, ds, k, h->st);
-    }
-    if (e == hipSuccess && rc == CLIFFS_OK)
-        e = hipMemcpyAsync(eta, ds, sizeof(float) * (size_t)n * nsamples, hipMemcpyDeviceToHost, h->st);
-    if (e == hipSuccess && rc == CLIFFS_OK) e = hipStreamSynchronize(h->st);
+    rc = run_gauges_into(h, every, nsamples, n, I, J, eta, dij, ds);
     cleanup();
+    return rc;
+}
+
+int cliffs_run_gauges_noalloc(cliffs_handle* h, int every, int nsamples, int n, const int* I,
+                              const int* J, float* eta) {
+    int rc = gauge_args(h, every, nsamples, n, I, J, eta);
     if (rc) return rc;
-    if (e != hipSuccess) return hipfail(e, "gauge series");
-    return CLIFFS_OK;
+    const size_t words = gauge_scratch_words();
+    if ((size_t)n * ((size_t)nsamples + 2) > words)
+        return fail(CLIFFS_ERR_ARG, "gauge series too large for the device scratch: n*(nsamples+2) must be <= " +
+                                        std::to_string(words));
+    HIPCHK(hipSetDevice(h->dev));
+    void* base = nullptr;
+    const hipError_t e = gauge_scratch(h->dev, &base);
+    if (e != hipSuccess) return hipfail(e, "hipGetSymbolAddress(scratch)");
+    int* dij = static_cast<int*>(base);
+    float* ds = reinterpret_cast<float*>(dij + 2 * (size_t)n);
+    // every slot of the series is written by its sample; nothing to clear
+    return run_gauges_into(h, every, nsamples, n, I, J, eta, dij, ds);
 }
 
 int cliffs_health(cliffs_handle* h, double* courant, long long* nonfinite) {

@@ -9,6 +9,8 @@
 // the unfused 2-D step; the fused 2-D step is in step_fused.hip.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "sweep_core.h"
 
 namespace cliffs {
@@ -332,6 +334,35 @@ hipError_t launch_health(const DevGrid& g, const Params& p, const float4* xm, co
     const dim3 grid((g.nx + 255) / 256, g.ny < 256 ? g.ny : 256);
     hipLaunchKernelGGL(health_kernel, grid, dim3(256), 0, st, g, p, xm, xr, ym, xrun, yrun, out);
     return hipGetLastError();
+}
+
+// Device scratch of cliffs_run_gauges_noalloc: the gauge index table and
+// the recorded series, kGaugeScratchWords 4-byte words per GPU, held from
+// the library's load to the end of the process.  Every call on a GPU shares
+// that GPU's copy (calls that overlap on one GPU race on it).  In host code
+// `scratch` names a host-side stand-in, not a device address: kernels and
+// copies take the per-device address that gauge_scratch looks up.
+constexpr size_t kGaugeScratchWords = (size_t)1 << 20;  // 4 MiB
+static __device__ __align__(256) unsigned scratch[kGaugeScratchWords];
+
+size_t gauge_scratch_words() { return kGaugeScratchWords; }
+
+// Device address of `scratch` on device `dev`, which must be current: looked
+// up once per device (hipGetSymbolAddress), then cached.
+hipError_t gauge_scratch(int dev, void** ptr) {
+    constexpr int kMaxDev = 64;
+    static std::mutex mu;
+    static void* addr[kMaxDev] = {};
+    if (dev < 0 || dev >= kMaxDev) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[dev] = p;
+    }
+    *ptr = addr[dev];
+    return hipSuccess;
 }
 
 hipError_t launch_gauge_sample(const DevGrid& g, const Params& p, const int* I, const int* J, int n,

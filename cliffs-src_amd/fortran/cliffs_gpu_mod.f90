@@ -32,7 +32,7 @@ module cliffs_gpu_mod
   public :: cliffs_stream, cliffs_info, cliffs_last_error, cliffs_health
   public :: cliffs_nest_maps, cliffs_nest_create, cliffs_nest_destroy, cliffs_nest_feed
   public :: cliffs_nest_couple, cliffs_nest_record, cliffs_nest_end_feed, cliffs_nest_child_step
-  public :: cliffs_nest_child_start
+  public :: cliffs_nest_child_start, cliffs_run_gauges_noalloc
   public :: cliffs_run_gauges, cliffs_set_fused, cliffs_set_tile_rows
   public :: cliffs_ensemble_create, cliffs_ensemble_destroy, cliffs_ensemble_run
   public :: cliffs_ensemble_run_timed, cliffs_ensemble_tune, cliffs_ensemble_sync
@@ -158,6 +158,15 @@ module cliffs_gpu_mod
     end function
     integer(c_int) function cliffs_run_gauges(h, every, nsamples, n, ig, jg, eta) &
         bind(C, name="cliffs_run_gauges")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: h
+      integer(c_int), value :: every, nsamples, n
+      integer(c_int), intent(in) :: ig(*), jg(*)
+      real(c_float), intent(out) :: eta(*)
+    end function
+    ! no device allocation: a library-wide device scratch, n*(nsamples+2) <= 2**20
+    integer(c_int) function cliffs_run_gauges_noalloc(h, every, nsamples, n, ig, jg, eta) &
+        bind(C, name="cliffs_run_gauges_noalloc")
       import :: c_int, c_ptr, c_float
       type(c_ptr), value :: h
       integer(c_int), value :: every, nsamples, n

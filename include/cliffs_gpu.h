@@ -119,6 +119,16 @@ int cliffs_run_timed(cliffs_handle *h, int nsteps, double *ms_total, double *ms_
  * Recorded on the device; one copy to the host at the end. */
 int cliffs_run_gauges(cliffs_handle *h, int every, int nsamples, int n, const int *I, const int *J,
                       float *eta);
+/* cliffs_run_gauges without any device allocation: the gauge table and the
+ * series live in a library-wide device scratch of 2^20 4-byte words (4 MiB)
+ * per GPU, held until the process ends; fails with CLIFFS_ERR_ARG when
+ * n*(nsamples+2) exceeds it.  Results equal cliffs_run_gauges'.  Calls that
+ * overlap on one GPU (handles used from several host threads) race on the
+ * scratch.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (serving loops and harnesses that forbid
+ * hipMalloc). */
+int cliffs_run_gauges_noalloc(cliffs_handle *h, int every, int nsamples, int n, const int *I,
+                              const int *J, float *eta);
 
 /* 2-D steps (cliffs_step, cliffs_run) run as one fused kernel per step
  * (on = 1, the default) or as an x-sweep kernel followed by a y-sweep kernel
