@@ -86,6 +86,8 @@ struct YArgs {
 constexpr int kFStrip = 248;    // output columns per tile (x window of 256)
 constexpr int kFMaxRows = 256;  // rows per tile, at most (the tile's row metrics sit in LDS)
 constexpr int kXCDs = 8;        // MI355X: 8 accelerator complex dies, one L2 each
+constexpr int kFRoleThreads = 256;  // threads of each role of a fused-step block (x rows, y columns)
+constexpr int kFThreads = 2 * kFRoleThreads;
 
 // Line flags of the fused step are per-tile contributions (one byte per tile
 // and line, written 0 or 1 by the tile that owns it), so a tile recomputed by

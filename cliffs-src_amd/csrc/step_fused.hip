@@ -95,6 +95,15 @@ struct Fields {
 
 // One tile of the step.  kExact = false: predicted column flags (step
 // kernel); kExact = true: the true flags recorded by the step kernel (fixup).
+//
+// The block's eight waves have two roles.  Waves 0-3 (the X role) x-sweep
+// the tile's stream rows four at a time, one row per wave, into the LDS
+// ring; waves 4-7 (the Y role, one thread per column) run the column
+// pipelines over the rows of the previous group, store the final state and
+// the max-wave fields.  Both roles meet at one barrier per group:
+//   iteration k:  X: rows of group k -> ring[k & 1]      Y: group k-1 from ring[(k-1) & 1]
+// so a role's registers are live only in its own waves, and the x-sweep of
+// one group overlaps the y-sweep of the group before.
 template <bool kExact>
 __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh) {
     const Params& p = a.p;
@@ -108,164 +117,177 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
     const int strip = tile % a.nstrips, chunk = tile / a.nstrips;
     const int cs = strip * kFStrip;
     const int j0 = chunk * a.rows, j1 = min(j0 + a.rows, ny);
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int t = threadIdx.x;
+    const bool xrole = t < kFRoleThreads;               // wave-uniform
+    const int lane = t & 63, w = (t >> 6) & 3;          // wave within its role
+    const int ty = t - kFRoleThreads;                   // column thread of the Y role
 
-    // x side: lane holds window nodes cs-4+4*lane .. +3
-    const int i0 = cs - 4 + 4 * lane;
-    {
+    // staged tables: x metric of the window nodes, row metrics of the stream rows
+    if (xrole) {
         const int i = cs - 4 + t;
         sh.xmt[t] = (i >= 0 && i < nx) ? a.xm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    const XLane XL{i0, xlane_vmask(nx, i0), &sh.xmt[4 * lane]};
     const int yb = j0 - 5, xb = j0 - 3;  // first rows of the staged row tables
-    for (int k = t; k < j1 - j0 + 6; k += 256) {
+    for (int k = t; k < j1 - j0 + 6; k += kFThreads) {
         const int ry = yb + k, rx = xb + k;
         sh.ymt[k] = (ry >= 0 && ry < ny) ? a.ym[ry] : make_float4(0.f, 0.f, 0.f, 0.f);
         sh.zt[k] = (ry >= 0 && ry < ny) ? a.zeta[ry] : 0.f;
         sh.xrt[k] = (rx >= 0 && rx < ny) ? a.xr[rx] : make_float2(0.f, 0.f);
     }
-    // per-lane constants as integers (vector registers, not scalar lane masks)
-    const int ld_lane = (i0 >= 0 && i0 < nx) ? 1 : 0;
-    const int outmask = (lane >= 1 && lane <= 62) ? XL.vmask : 0;
-    int colwet_x = 0;
-
-    // y side: thread t owns column cs+t (window node t+4)
-    const int col = cs + t;
-    const int col_ok = (t < kFStrip && col < nx) ? 1 : 0;
-    const int in_range = (col_ok && col >= a.col_lo && col <= a.col_hi) ? 1 : 0;
-    int yflag = 0;
-    if (kExact) {
-        yflag = (in_range && a.ctr[col] != 0) ? 1 : 0;
-    } else if (in_range) {
-        yflag = column_any(a.cpr_in, a.cstride, col) ? 1 : 0;
-    }
-    // max-wave of the column updated here (keepmax) or deferred (loadmax
-    // only: the tile is flagged if the update would change a maximum)
-    const int keepmax = (a.maxwave && col_ok && (kExact || !in_range || yflag)) ? 1 : 0;
-    const int loadmax = (a.maxwave && col_ok) ? 1 : 0;
-    YCol cc;
-    cc.swept = in_range && yflag;
-    cc.load_edges(a.yedges, nx, col, col_ok);
-    YPipe pipe;
-    pipe.init();
-    int colwet_f = 0;  // bit 0: a node of the column is wet after the step; bit 1: deferred change
     __syncthreads();  // xmt and the row tables are complete
 
     // rows j0-3 .. j1+2 stream through; group g holds stream rows 4g .. 4g+3
     const int nstream = j1 - j0 + 6;
     const int ngroups = (nstream + 3) / 4;
 
-    // the wave's next x row is loaded one group ahead
-    Q4 ns = zero4(), ncp = zero4(), nu = zero4(), nv = zero4();
-    int nflag = 0;
-    auto x_valid = [&](int g) {
-        const int q = j0 - 3 + 4 * g + w;
-        return 4 * g + w < nstream && q >= 0 && q < ny;
-    };
-    auto x_load = [&](int g) {
-        if (!x_valid(g)) return;
-        const int q = j0 - 3 + 4 * g + w;
-        if (ld_lane) {
-            const unsigned o = (unsigned)q * pitch + (unsigned)i0;
-            ns = F.ld4(o);
-            ncp = F.ld4(oCP + o);
-            nu = F.ld4(oU + o);
-            nv = F.ld4(oV + o);
-        }
-        int f = 0;  // row has a wet node in some strip (previous step)
-        for (int s0 = 0; s0 < a.nstrips; s0 += 64) {
-            const int s = s0 + lane;
-            f |= (s < a.nstrips && a.rfl_in[(size_t)s * ny + q] != 0) ? 1 : 0;
-        }
-        nflag = f;
-    };
-    // max-wave fields of the rows group g finalises (rows j0-6+4g .. +3),
-    // loaded after the group's barrier: prefetching them a group ahead costs
-    // more in registers (occupancy) than it saves in latency
-    float hc[4], vc[4];
-    auto hv_load = [&](int g, float* h, float* v) {
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int r = j0 - 6 + 4 * g + k;
-            const bool ok = loadmax && r >= j0 && r < j1;
-            const unsigned o = (unsigned)max(r, 0) * pitch + (unsigned)col;
-            h[k] = ok ? F.ld(oHM + o) : 0.f;
-            v[k] = ok ? F.ld(oVM + o) : 0.f;
-        }
-    };
+    // X role: lane holds window nodes cs-4+4*lane .. +3
+    const int i0 = cs - 4 + 4 * lane;
+    const int ld_lane = (xrole && i0 >= 0 && i0 < nx) ? 1 : 0;
+    const XLane XL{i0, xlane_vmask(nx, i0), &sh.xmt[4 * lane]};
+    const int outmask = (xrole && lane >= 1 && lane <= 62) ? XL.vmask : 0;
+    int colwet_x = 0;
 
-    x_load(0);
-    for (int g = 0; g < ngroups; g++) {
-        float(*buf)[4][256] = sh.row[g & 1];
-        // ---- X: wave w sweeps grid row q into the ring.  The block's next
-        // barrier waits for these rows, so the X phase runs at raised wave
-        // priority against the Y phases of other blocks on the CU.
-        __builtin_amdgcn_s_setprio(1);
-        {
+    // Y role: thread ty owns column cs+ty (window node ty+4)
+    const int col = cs + ty;
+    const int col_ok = (!xrole && ty < kFStrip && col < nx) ? 1 : 0;
+    const int in_range = (col_ok && col >= a.col_lo && col <= a.col_hi) ? 1 : 0;
+    // max-wave of the column updated here (keepmax) or deferred (loadmax
+    // only: the tile is flagged if the update would change a maximum)
+    int keepmax = 0, loadmax = 0;
+    int colwet_f = 0;  // bit 0: a node of the column is wet after the step; bit 1: deferred change
+
+    if (xrole) {
+        // ---- X role: the wave's next row is loaded one group ahead
+        Q4 ns = zero4(), ncp = zero4(), nu = zero4(), nv = zero4();
+        int nflag = 0;
+        auto x_valid = [&](int g) {
             const int q = j0 - 3 + 4 * g + w;
-            const bool valid = x_valid(g);
-            const Q4 s = ns, cp = ncp, u = nu, v = nv;
-            const bool fl = __ballot(nflag != 0) != 0;
-            if (g + 1 < ngroups) x_load(g + 1);
-            if (valid) {
-                const bool sweep = q >= a.row_lo && q <= a.row_hi && fl;
-                Q4 ocp, ou, ov;
-                x_row(XL, p, nx, ny, q, sweep, sh.xrt[q - xb], a.xedges, s, cp, u, v, ocp, ou, ov);
-                st4(&buf[w][0][4 * lane], s);
-                st4(&buf[w][1][4 * lane], ocp);
-                st4(&buf[w][2][4 * lane], ou);
-                st4(&buf[w][3][4 * lane], ov);
-                if (!kExact && q >= j0 && q < j1) colwet_x |= wet_bits(s, ocp, outmask, p);
+            return 4 * g + w < nstream && q >= 0 && q < ny;
+        };
+        auto x_load = [&](int g) {
+            if (!x_valid(g)) return;
+            const int q = j0 - 3 + 4 * g + w;
+            if (ld_lane) {
+                const unsigned o = (unsigned)q * pitch + (unsigned)i0;
+                ns = F.ld4(o);
+                ncp = F.ld4(oCP + o);
+                nu = F.ld4(oU + o);
+                nv = F.ld4(oV + o);
             }
+            int f = 0;  // row has a wet node in some strip (previous step)
+            for (int s0 = 0; s0 < a.nstrips; s0 += 64) {
+                const int s = s0 + lane;
+                f |= (s < a.nstrips && a.rfl_in[(size_t)s * ny + q] != 0) ? 1 : 0;
+            }
+            nflag = f;
+        };
+        x_load(0);
+        for (int g = 0; g <= ngroups; g++) {
+            if (g < ngroups) {
+                float(*buf)[4][256] = sh.row[g & 1];
+                const int q = j0 - 3 + 4 * g + w;
+                const bool valid = x_valid(g);
+                const Q4 s = ns, cp = ncp, u = nu, v = nv;
+                const bool fl = __ballot(nflag != 0) != 0;
+                if (g + 1 < ngroups) x_load(g + 1);
+                if (valid) {
+                    const bool sweep = q >= a.row_lo && q <= a.row_hi && fl;
+                    Q4 ocp, ou, ov;
+                    x_row(XL, p, nx, ny, q, sweep, sh.xrt[q - xb], a.xedges, s, cp, u, v, ocp, ou, ov);
+                    st4(&buf[w][0][4 * lane], s);
+                    st4(&buf[w][1][4 * lane], ocp);
+                    st4(&buf[w][2][4 * lane], ou);
+                    st4(&buf[w][3][4 * lane], ov);
+                    if (!kExact && q >= j0 && q < j1) colwet_x |= wet_bits(s, ocp, outmask, p);
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        __builtin_amdgcn_s_setprio(0);
-        hv_load(g, hc, vc);
-        // ---- Y: four rows through the column pipelines
+    } else {
+        // ---- Y role
+        int yflag = 0;
+        if (kExact) {
+            yflag = (in_range && a.ctr[col] != 0) ? 1 : 0;
+        } else if (in_range) {
+            yflag = column_any(a.cpr_in, a.cstride, col) ? 1 : 0;
+        }
+        keepmax = (a.maxwave && col_ok && (kExact || !in_range || yflag)) ? 1 : 0;
+        loadmax = (a.maxwave && col_ok) ? 1 : 0;
+        YCol cc;
+        cc.swept = in_range && yflag;
+        cc.load_edges(a.yedges, nx, col, col_ok);
+        YPipe pipe;
+        pipe.init();
+        // max-wave fields of the rows group g finalises (rows j0-6+4g .. +3),
+        // loaded one group ahead
+        float hc[4], vc[4];
+        auto hv_load = [&](int g) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int idx = 4 * g + k;
-            if (idx < nstream) {
-                const int r = j0 - 6 + idx;
-                const int q = r + 3;
-                const bool rv = col_ok && q >= 0 && q < ny;
-                RawIn rin{0.f, 0.f, 0.f, 0.f};
-                if (rv) {
-                    rin.s = buf[k][0][t + 4];
-                    rin.cp = buf[k][1][t + 4];
-                    rin.u = buf[k][3][t + 4];  // normal velocity of the y-sweep: yvel
-                    rin.v = buf[k][2][t + 4];
+            for (int k = 0; k < 4; k++) {
+                const int r = j0 - 6 + 4 * g + k;
+                const bool ok = loadmax && r >= j0 && r < j1;
+                const unsigned o = (unsigned)max(r, 0) * pitch + (unsigned)col;
+                hc[k] = ok ? F.ld(oHM + o) : 0.f;
+                vc[k] = ok ? F.ld(oVM + o) : 0.f;
+            }
+        };
+        hv_load(0);
+        for (int g = -1; g < ngroups; g++) {
+            if (g >= 0) {
+                float(*buf)[4][256] = sh.row[g & 1];
+                float h[4], vm[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    h[k] = hc[k];
+                    vm[k] = vc[k];
                 }
-                const YOut f = pipe.advance(r, rin, rv, cc, ny, sh.ymt, sh.zt, yb, p);
-                if (r >= j0 && r < j1) {
-                    bool wet = false;
-                    if (col_ok) {
-                        const unsigned o = (unsigned)r * pitch + (unsigned)col;
-                        F.st(oCPn + o, f.cp);
-                        F.st(oUn + o, f.x);
-                        F.st(oVn + o, f.y);
-                        {  // max-wave (cliffs_main.f:347-355): store on change only
-                            const float sp2 = speed2(f.x, f.y);
-                            const bool ch = f.cp > hc[k], cv = sp2 > vc[k];
-                            if (keepmax && (ch || cv)) {
-                                if (ch) F.st(oHM + o, f.cp);
-                                if (cv) F.st(oVM + o, sp2);
-                            }
-                            colwet_f |= (ch || cv) ? 2 : 0;  // read only for deferred columns
+                if (g + 1 < ngroups) hv_load(g + 1);
+                // four rows through the column pipelines
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int idx = 4 * g + k;
+                    if (idx < nstream) {
+                        const int r = j0 - 6 + idx;
+                        const int q = r + 3;
+                        const bool rv = col_ok && q >= 0 && q < ny;
+                        RawIn rin{0.f, 0.f, 0.f, 0.f};
+                        if (rv) {
+                            rin.s = buf[k][0][ty + 4];
+                            rin.cp = buf[k][1][ty + 4];
+                            rin.u = buf[k][3][ty + 4];  // normal velocity of the y-sweep: yvel
+                            rin.v = buf[k][2][ty + 4];
                         }
-                        wet = !is_land(c0_of(f.s) + f.cp, p);
-                        colwet_f |= wet ? 1 : 0;
+                        const YOut f = pipe.advance(r, rin, rv, cc, ny, sh.ymt, sh.zt, yb, p);
+                        if (r >= j0 && r < j1) {
+                            bool wet = false;
+                            if (col_ok) {
+                                const unsigned o = (unsigned)r * pitch + (unsigned)col;
+                                F.st(oCPn + o, f.cp);
+                                F.st(oUn + o, f.x);
+                                F.st(oVn + o, f.y);
+                                {  // max-wave (cliffs_main.f:347-355): store on change only
+                                    const float sp2 = speed2(f.x, f.y);
+                                    const bool ch = f.cp > h[k], cv = sp2 > vm[k];
+                                    if (keepmax && (ch || cv)) {
+                                        if (ch) F.st(oHM + o, f.cp);
+                                        if (cv) F.st(oVM + o, sp2);
+                                    }
+                                    colwet_f |= (ch || cv) ? 2 : 0;  // read only for deferred columns
+                                }
+                                wet = !is_land(c0_of(f.s) + f.cp, p);
+                                colwet_f |= wet ? 1 : 0;
+                            }
+                            sh.rowany[w][r - j0] = __ballot(wet) != 0 ? 1 : 0;
+                        }
                     }
-                    sh.rowany[w][r - j0] = __ballot(wet) != 0 ? 1 : 0;
                 }
             }
+            __syncthreads();
         }
-        // no second barrier: the next X phase fills the other buffer, and the
-        // one after it waits at the next barrier for this Y phase to finish
     }
     const int tile_mchange = __syncthreads_or((colwet_f & 2) && loadmax && !keepmax);
     if (!kExact && t == 0) a.mpend[tile] = tile_mchange ? 1 : 0;
-    for (int k = t; k < j1 - j0; k += 256) a.rfl_out[(size_t)strip * ny + j0 + k] =
+    for (int k = t; k < j1 - j0; k += kFThreads) a.rfl_out[(size_t)strip * ny + j0 + k] =
             (sh.rowany[0][k] | sh.rowany[1][k] | sh.rowany[2][k] | sh.rowany[3][k]) ? 1 : 0;
     if (col_ok) a.cpr_out[(size_t)col * a.cstride + chunk] = colwet_f & 1;
     if (!kExact && outmask && ld_lane && colwet_x) {
@@ -275,9 +297,9 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
     }
 }
 
-// 3 waves per SIMD (<= 168 VGPRs): 12 waves per CU hide the memory latency
-// that one barrier-synchronised block cannot.
-__global__ __launch_bounds__(256, 3) void step_fused_kernel(const FArgs a) {
+// Eight waves per block (two roles), two blocks per CU: four waves per SIMD
+// (<= 128 VGPRs), each wave's registers holding one role's state only.
+__global__ __launch_bounds__(kFThreads, 4) void step_fused_kernel(const FArgs a) {
     __shared__ FShared sh;
     const int tile = tile_of_block(blockIdx.x, a.ntiles);
     if (tile < 0) return;
@@ -316,7 +338,7 @@ __device__ __forceinline__ void fixup_tile(const FArgs& a, int tile, FShared& sh
     }
 }
 
-__global__ __launch_bounds__(256) void step_fixup_kernel(const FArgs a) {
+__global__ __launch_bounds__(kFThreads) void step_fixup_kernel(const FArgs a) {
     __shared__ FShared sh;
     const int tile = tile_of_block(blockIdx.x, a.ntiles);
     if (tile < 0) return;
@@ -325,7 +347,7 @@ __global__ __launch_bounds__(256) void step_fixup_kernel(const FArgs a) {
 
 // Ensembles: one launch steps every member grid.  bmap[b] = (member, tile)
 // or (-1, -1); args[member] is that member's argument block for this step.
-__global__ __launch_bounds__(256, 3) void step_fused_batch_kernel(const FArgs* __restrict__ args,
+__global__ __launch_bounds__(kFThreads, 4) void step_fused_batch_kernel(const FArgs* __restrict__ args,
                                                                   const int2* __restrict__ bmap) {
     __shared__ FShared sh;
     const int2 bm = bmap[blockIdx.x];
@@ -333,7 +355,7 @@ __global__ __launch_bounds__(256, 3) void step_fused_batch_kernel(const FArgs* _
     fused_tile<false>(args[bm.x], bm.y, sh);
 }
 
-__global__ __launch_bounds__(256) void step_fixup_batch_kernel(const FArgs* __restrict__ args,
+__global__ __launch_bounds__(kFThreads) void step_fixup_batch_kernel(const FArgs* __restrict__ args,
                                                                const int2* __restrict__ bmap) {
     __shared__ FShared sh;
     const int2 bm = bmap[blockIdx.x];
@@ -410,22 +432,22 @@ hipError_t launch_ensemble_gauges(const FArgs* args, int nm, const int* ij, int 
 }
 
 hipError_t launch_step_fused(const FArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(step_fused_kernel, dim3(fused_blocks(a)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(step_fused_kernel, dim3(fused_blocks(a)), dim3(kFThreads), 0, st, a);
     return hipGetLastError();
 }
 
 hipError_t launch_step_fixup(const FArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(step_fixup_kernel, dim3(fused_blocks(a)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(step_fixup_kernel, dim3(fused_blocks(a)), dim3(kFThreads), 0, st, a);
     return hipGetLastError();
 }
 
 hipError_t launch_step_batch(const FArgs* args, const int2* bmap, int nblocks, hipStream_t st) {
-    hipLaunchKernelGGL(step_fused_batch_kernel, dim3(nblocks), dim3(256), 0, st, args, bmap);
+    hipLaunchKernelGGL(step_fused_batch_kernel, dim3(nblocks), dim3(kFThreads), 0, st, args, bmap);
     return hipGetLastError();
 }
 
 hipError_t launch_fixup_batch(const FArgs* args, const int2* bmap, int nblocks, hipStream_t st) {
-    hipLaunchKernelGGL(step_fixup_batch_kernel, dim3(nblocks), dim3(256), 0, st, args, bmap);
+    hipLaunchKernelGGL(step_fixup_batch_kernel, dim3(nblocks), dim3(kFThreads), 0, st, args, bmap);
     return hipGetLastError();
 }
 
@@ -443,7 +465,7 @@ hipError_t launch_fused_flags(const FArgs& a, unsigned char* ctr0, unsigned char
 }
 
 hipError_t fused_blocks_per_cu(int* n) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, step_fused_kernel, 256, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, step_fused_kernel, kFThreads, 0);
 }
 
 // Block map of an ensemble step: every member's tiles, XCD by XCD.  Member
