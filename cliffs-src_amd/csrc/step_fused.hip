@@ -183,6 +183,9 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
         };
         x_load(0);
         for (int g = 0; g <= ngroups; g++) {
+            // the x rows gate the barrier both roles meet at: the X role
+            // runs at raised wave priority against the Y waves on its SIMD
+            __builtin_amdgcn_s_setprio(1);
             if (g < ngroups) {
                 float(*buf)[4][256] = sh.row[g & 1];
                 const int q = j0 - 3 + 4 * g + w;
@@ -202,6 +205,7 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
                 }
             }
             __syncthreads();
+            __builtin_amdgcn_s_setprio(0);
         }
     } else {
         // ---- Y role
