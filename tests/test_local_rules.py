@@ -6,9 +6,13 @@ gating cliffs.f:289-290, 310-311).  The HIP kernels use per-node rules of
 radius <= 3 instead.  This test restates the scan at the level of its
 decisions (which node is newland, expanded, interior, which neighbour is a
 ghost, which limiter case fires, which edge is integrated) and compares it
-with the local rules exhaustively over every land mask of length n <= 10,
-with the data-dependent conditions (flood > ground, velocity signs) drawn as
-free booleans.
+with the local rules over every land mask of length n <= 10.  The
+data-dependent conditions (flood > ground, velocity sign) enter both forms
+only as the conjunctions fr[i] & ur[i] (right expansion into land node i
+after water) and fl[i] & ul[i] (left expansion into land node i before
+water); for n <= 8 every combination of these conjunctions at the nodes
+where they can matter is enumerated (a false conjunction with either member
+false, the other conditions random).
 """
 import itertools
 
@@ -114,21 +118,40 @@ def local(land, fr, ur, fl, ul):
                 edge_last=(not newland[n - 2]) and (not newland[n - 1]))
 
 
+def _check(land, fr, ur, fl, ul):
+    a = sequential(land, fr, ur, fl, ul)
+    b = local(land, fr, ur, fl, ul)
+    assert (a is None) == (b is None), (land, fr, ur, fl, ul)
+    if a is None:
+        return
+    for key in a:
+        assert a[key] == b[key], (key, land, fr, ur, fl, ul, a[key], b[key])
+
+
 @pytest.mark.parametrize("n", range(3, 11))
-def test_local_rules_exhaustive_masks(n):
-    rng = np.random.default_rng(n)
-    draws = 24 if n <= 8 else 6
+def test_local_rules_exhaustive(n):
+    """Every land mask and every combination of the expansion conditions
+    where they can matter: right expansion into land node i after a water
+    node (land[i] and not land[i-1]), left expansion into land node i before
+    a water node (land[i] and not land[i+1])."""
+    rng = np.random.default_rng(100 + n)
+    cases = 0
     for mask in itertools.product([False, True], repeat=n):
         land = list(mask)
-        for _ in range(draws):
+        R = [i for i in range(1, n) if land[i] and not land[i - 1]]
+        L = [i for i in range(n - 1) if land[i] and not land[i + 1]]
+        for bits in itertools.product([False, True], repeat=len(R) + len(L)):
             fr, ur, fl, ul = (list(rng.integers(0, 2, n).astype(bool)) for _ in range(4))
-            a = sequential(land, fr, ur, fl, ul)
-            b = local(land, fr, ur, fl, ul)
-            assert (a is None) == (b is None), (land, fr, ur, fl, ul)
-            if a is None:
-                continue
-            for key in a:
-                assert a[key] == b[key], (key, land, fr, ur, fl, ul, a[key], b[key])
+            for (i, b), (x, y) in zip(zip(R + L, bits), [(fr, ur)] * len(R) + [(fl, ul)] * len(L)):
+                if b:
+                    x[i] = y[i] = True
+                elif rng.integers(0, 2):
+                    x[i] = False
+                else:
+                    y[i] = False
+            _check(land, fr, ur, fl, ul)
+            cases += 1
+    assert cases >= 2 ** n
 
 
 def test_local_rules_long_random_lines():
