@@ -10,7 +10,7 @@ with the local rules over every land mask of length n <= 10.  The
 data-dependent conditions (flood > ground, velocity sign) enter both forms
 only as the conjunctions fr[i] & ur[i] (right expansion into land node i
 after water) and fl[i] & ul[i] (left expansion into land node i before
-water); for n <= 8 every combination of these conjunctions at the nodes
+water); every combination of these conjunctions at the nodes
 where they can matter is enumerated (a false conjunction with either member
 false, the other conditions random).
 """
