@@ -139,65 +139,82 @@ def runup_parity(seed0, nscen, steps, runups, cases):
             "within_one_cell": sum(1 for r in rows if r[3] <= 1.0), "of": len(rows)}
 
 
-def nested_bench(steps, warmup):
-    """Full-size nested workload: the spherical 8192 x 6144 coastal parent
-    with run-up and a 2047^2 child at 3x on its coast, coupled on the device
-    (a feed record after every parent step, cliffs_nest_record; the child
-    stepped at dt/3 behind the feed, cliffs_nest_child_step, on its own
-    stream).  Times `steps` parent steps with their records and child steps
-    after `warmup` such cycles; value counts parent and child cell-updates."""
+def nested_bench(hours, every=900):
+    """The nested workload: the spherical 8192 x 6144 coastal parent (30
+    arc-seconds, 100 km shelf, run-up, 200 x 100 km dipole; synth.coastal_nest)
+    and its 2048^2 child at 3x on the coast, coupled on the device: a feed
+    record after every parent step (cliffs_nest_record), the child started at
+    the first record whose edge |eta| reaches 1 mm (cliffs_nest_child_start,
+    the reference examples' threshold) and stepped at dt/3 behind the feed
+    (cliffs_nest_child_step) -- for `hours` simulated hours.  Every `every`
+    parent steps the health of both grids (cliffs_health) and the child's
+    maximum run-up are logged.  value counts parent and child cell-updates
+    over the whole run's wall time."""
     import numpy as np
     import cliffs_gpu
     import synth
-    log("[bench] nested: generating the 8192x6144 parent and its 2047^2 child")
-    case, xc, yc, depc = synth.coastal_nest()
+    log("[bench] nested: generating the 8192x6144 parent and its 2048^2 child")
+    case, xc, yc, depc, meta = synth.coastal_nest()
     maps = cliffs_gpu.nest_maps(case.xcrd, case.ycrd, xc, yc)
     s1, s2, zeta = case.metrics()
-    g = cliffs_gpu.Grid(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground, device=0)
+    g = cliffs_gpu.Grid(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground,
+                        crough=case.crough, device=0)
     g.upload(case.cel, case.xvel, case.yvel)
     cs1, cs2, czeta = synth.metrics(xc, yc, False)
-    c = cliffs_gpu.Grid(depc, cs1, cs2, czeta, dt=case.dt / 3, ground=case.ground, device=0)
+    dtc = case.dt / 3
+    cg = meta["child_ground"]
+    c = cliffs_gpu.Grid(depc, cs1, cs2, czeta, dt=dtc, ground=cg, crough=case.crough, device=0)
     z = np.zeros_like(depc)
-    c.upload(synth.depth_to_celerity(depc, case.ground), z, z)
+    c.upload(synth.depth_to_celerity(depc, cg), z, z)
     nest = cliffs_gpu.Nest(g, maps)
     nest.couple(c, capacity=8)
-    st = {"tp": 0.0, "iarv": False, "trec": [], "kc": 0}
-
-    def cycles(n):
-        """n parent steps, each followed by its feed record and the child
-        steps that stay behind the last record; -> child steps taken."""
-        taken = 0
-        for _ in range(n):
-            g.run(1)
-            st["tp"] += case.dt
-            st["iarv"], nrec = nest.record(0.01, case.ground, st["tp"], st["iarv"])
-            if nrec > len(st["trec"]):
-                st["trec"].append(st["tp"])
-            while st["kc"] + 1 < 3 * (len(st["trec"]) - 1):
-                st["kc"] += 1
-                j = st["kc"] // 3
-                nest.child_step(st["trec"][j] + (st["kc"] - 3 * j) * case.dt / 3)
-                taken += 1
-        return taken
-
-    cycles(warmup)
-    g.sync()
-    c.sync()
+    nsteps = int(round(hours * 3600.0 / case.dt))
+    tp, iarv, trec, tc, nchild, t0c = 0.0, False, [], None, 0, None
+    health = []
+    log(f"[bench] nested: {nsteps} parent steps of {case.dt:.3f} s ({hours} h)")
     t0 = time.perf_counter()
-    nchild = cycles(steps)
+    for k in range(1, nsteps + 1):
+        g.run(1)
+        tp += case.dt
+        iarv, nrec = nest.record(0.01, case.ground, tp, iarv)
+        if nrec > len(trec):
+            trec.append(tp)
+        if tc is None and nrec:
+            try:
+                tc = t0c = nest.child_start(0.001)
+            except cliffs_gpu.CliffsError:
+                pass
+        while tc is not None and tc + dtc < trec[-1]:
+            tc += dtc
+            nest.child_step(tc)
+            nchild += 1
+        if k % every == 0 or k == nsteps:
+            cp, bp = g.health()
+            cc, bc = c.health()
+            ru = synth.max_runup(c.download_max()[0], depc, c.celmin)
+            health.append({"step": k, "t_s": round(tp, 1), "parent_courant": round(cp, 4),
+                           "parent_nonfinite": bp, "child_courant": round(cc, 4),
+                           "child_nonfinite": bc, "child_steps": nchild,
+                           "child_max_runup_m": round(ru, 2)})
+            log(f"[bench] nested: {health[-1]}")
     g.sync()
     c.sync()
     wall = time.perf_counter() - t0
-    finite = bool(np.isfinite(c.download()[0]).all() and np.isfinite(g.download()[0]).all())
+    hm_p = g.download_max()[0]
+    finite = all(h["parent_nonfinite"] == 0 and h["child_nonfinite"] == 0 for h in health)
     _, ms_p, ms_fix = g.run_timed(200)    # the parent's step kernels alone, for their roofline
     cells_p, cells_c = case.dep.size, depc.size
-    value = (cells_p * steps + cells_c * nchild) / wall / 1e9
-    log(f"[bench] nested: {steps} parent steps and {nchild} child steps in {wall:.3f} s, "
+    value = (cells_p * nsteps + cells_c * nchild) / wall / 1e9
+    log(f"[bench] nested: {nsteps} parent steps and {nchild} child steps in {wall:.3f} s, "
         f"{value:.2f} Gcell-updates/s")
+    out = {"parent_steps": nsteps, "child_steps": nchild, "records": len(trec),
+           "child_start_s": None if t0c is None else round(t0c, 1),
+           "parent_max_runup_m": round(synth.max_runup(hm_p, case.dep, g.celmin), 2),
+           "health": health, "meta": meta}
     nest.close()
     c.close()
     g.close()
-    return value, wall, nchild, (ms_p, ms_fix), cells_p, cells_c, case.dt, finite
+    return value, wall, out, (ms_p, ms_fix), cells_p, cells_c, case.dt, finite
 
 
 def cpu_baseline(case, steps):
@@ -339,9 +356,11 @@ def main():
                     help="instead of the open ocean: the spherical 8192 x 6144 coastal grid "
                          "(30 arc-seconds, shelf, coastline, run-up), whose tiles take the "
                          "wet/dry paths")
+    ap.add_argument("--nested-hours", type=float, default=6.0)
     ap.add_argument("--nested", action="store_true",
                     help="the full-size nested workload instead: the 8192 x 6144 coastal "
-                         "parent with a 2047^2 child at 3x coupled on the device")
+                         "parent with a 2048^2 child at 3x coupled on the device, for "
+                         "--nested-hours simulated hours")
     ap.add_argument("--unfused", action="store_true",
                     help="run the step as two sweep kernels instead of one fused kernel")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -379,15 +398,18 @@ def main():
     import synth
 
     if args.nested:
-        value, wall, nchild, ms_p, cells_p, cells_c, dt, finite = nested_bench(args.steps, args.warmup)
+        value, wall, run, ms_p, cells_p, cells_c, dt, finite = nested_bench(args.nested_hours)
         roof = roofline(cells_p, cells_p / (ms_p[0] * 1e-3) / 1e9, ms_p[0], ms_p[1], False,
                         "coastal_8192x6144", copy_bandwidth())
         roof["kernel"] = "step_fused (parent alone)"
-        out = bench_line(value, args.steps, args.warmup, wall,
-                         f"nested: 2D spherical coastal parent 8192x6144 (30 arc-seconds, run-up, "
-                         f"seed 3) and a 2047x2047 child at 3x on its coast, coupled on the device; "
-                         f"{nchild} child steps", 8192, 6144, dt, finite, roof, None, None)
-        out["config"]["child"] = {"nx": 2047, "ny": 2047, "steps": nchild}
+        out = bench_line(value, run["parent_steps"], 0, wall,
+                         f"nested: 2D spherical coastal parent 8192x6144 (30 arc-seconds, 100 km "
+                         f"shelf at 200 m, land to +300 m, run-up, 200x100 km dipole +3/-1.5 m, "
+                         f"seed 3) and a 2048x2048 child at 3x on its coast, coupled on the device, "
+                         f"{args.nested_hours} simulated hours", 8192, 6144, dt, finite, roof,
+                         None, None)
+        out["config"]["child"] = {"nx": 2048, "ny": 2048}
+        out["nested"] = run
         print(json.dumps(out), flush=True)
         return
 

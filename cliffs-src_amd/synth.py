@@ -191,21 +191,80 @@ def coastal(nx=256, ny=192, seed=0, cartesian=True, dx=500.0, steps=200, source_
                 steps, dict(seed=seed, amplitude=source_amp))
 
 
-def coastal_nest(seed=3, i0=5350, j0=3000, n=682, refine=3):
-    """The spherical 8192 x 6144 coastal parent (30 arc-seconds from 25 S,
-    run-up) and a child grid refine x finer over parent cells i0..i0+n,
-    j0..j0+n on its coast (a (refine*n+1)^2 lattice, 2047^2 by default),
-    its depth bilinear in the parent's.  -> (parent case, xc, yc, depc)."""
+def coastal_nest(seed=3, nx=8192, ny=6144, nc=2048, refine=3):
+    """Spherical coastal parent with run-up and one nested child, as the
+    nested workload names it: nx x ny at 30 arc-seconds from 140 E, 25 S;
+    fractal abyssal plain 4-6 km, a continental slope (60 km), a 100 km shelf
+    whose depth rises from 2 m at the coast to 200 m at its edge (square root
+    of depth linear, 5 % fractal roughness), land rising to +300 m within
+    50 km behind a seeded coastline, SSL conditioning (ssl_condition);
+    surface source an elliptical uplift/subsidence dipole (+3 m / -1.5 m)
+    200 km along its seeded strike and 100 km across, 250-400 km offshore;
+    child nc x nc nodes at refine x the parent's resolution over the coast
+    opposite the source (60 % sea, 40 % land), its depth bilinear in the
+    parent's.  Run parameters as the reference sets them for its 30" and 10"
+    Okushiri grids (paramsOK30s.txt, paramsOK10s.txt): minimal flow depth 1 m
+    on the parent and 0.5 m on the child (meta["child_ground"]), Manning
+    n^2 = 0.000625 on both.  -> (parent case, xc, yc, depc, meta)."""
     from scipy.interpolate import RegularGridInterpolator
-    case = coastal(nx=8192, ny=6144, seed=seed, cartesian=False, lat0=-25.0)
-    x, y = case.xcrd, case.ycrd
-    xc = np.linspace(x[i0], x[i0 + n], refine * n + 1)
-    yc = np.linspace(y[j0], y[j0 + n], refine * n + 1)
-    interp = RegularGridInterpolator((y[j0:j0 + n + 1], x[i0:i0 + n + 1]),
-                                     case.dep[j0:j0 + n + 1, i0:i0 + n + 1])
+    rng = np.random.default_rng(seed)
+    dl = 30.0 / 3600.0
+    lon0, lat0 = 140.0, -25.0
+    x = lon0 + np.arange(nx) * dl
+    y = lat0 + np.arange(ny) * dl
+    km = PI / 180.0 * EARTH_R / 1000.0                      # km per degree of latitude
+    coslat = np.cos(PI / 180.0 * y)[:, None]
+    # coastline longitude as a function of latitude: 0.75 of the width, wandering +-3 %
+    wander = fractal_field(ny, 1, rng, beta=2.0)[:, 0]
+    lon_c = (x[0] + 0.75 * (x[-1] - x[0]) + 0.03 * (x[-1] - x[0]) * wander)[:, None]
+    dist = (lon_c - x[None, :]) * km * coslat                # km offshore (> 0) / inland (< 0)
+    shelf, slope = 100.0, 60.0
+    abyss = 4000.0 + 2000.0 * (fractal_field(ny, nx, rng) * 0.5 + 0.5)
+    qa, qs, qc = np.sqrt(abyss), np.sqrt(200.0), np.sqrt(2.0)
+    t_slope = np.clip((dist - shelf) / slope, 0.0, 1.0)
+    t_shelf = np.clip(dist / shelf, 0.0, 1.0)
+    q = np.where(dist > shelf, qs + (qa - qs) * t_slope, qc + (qs - qc) * np.sqrt(t_shelf))
+    del t_slope, t_shelf, abyss
+    dep = q * q * (1.0 + 0.05 * fractal_field(ny, nx, rng) * (dist > 0))
+    del q
+    dep = np.where(dist <= 0, -np.minimum(300.0, 300.0 * (-dist) / 50.0) - 0.5, dep)
+    dep = ssl_condition(dep)
+    # source: seeded latitude in the middle half, offshore distance and strike
+    jc = int(rng.integers(ny // 4, 3 * ny // 4))
+    off = float(rng.uniform(250.0, 400.0))
+    strike = float(rng.uniform(-0.35, 0.35))                 # radians off north-south
+    src_lat = y[jc]
+    src_lon = float(lon_c[jc, 0]) - off / (km * np.cos(PI / 180.0 * src_lat))
+    X = (x[None, :] - src_lon) * km * coslat                 # km east of the source centre
+    Y = np.broadcast_to((y[:, None] - src_lat) * km, X.shape)
+    along = X * np.sin(strike) + Y * np.cos(strike)
+    across = X * np.cos(strike) - Y * np.sin(strike)
+    a_len, a_wid = 100.0, 25.0                               # semi-axes of each lobe (km)
+    eta = 3.0 * np.exp(-((across - a_wid) / a_wid) ** 2 - (along / a_len) ** 2) \
+        - 1.5 * np.exp(-((across + a_wid) / a_wid) ** 2 - (along / a_len) ** 2)
+    eta = np.where(dep > 0, eta, 0.0)
+    del X, Y, along, across
+    ground, crough = ENSEMBLE_GROUND, ENSEMBLE_CROUGH
+    cel = depth_to_celerity(dep + eta, ground)
+    del eta
+    s1, s2, _ = metrics(x, y, False)
+    dt = 0.8 * float(s1.min()) / math.sqrt(GRAV * (float(dep.max()) + 6.0))
+    z = np.zeros_like(dep)
+    case = Case("coastal_nest", dep, cel, z, z.copy(), x, y, False, dt, ground, crough, 0,
+                dict(seed=seed, amplitude=3.0, source=(src_lon, src_lat, off, strike)))
+    # child over the coast opposite the source: 60 % of its width offshore
+    span = (nc - 1) / refine                                  # parent cells it spans
+    ic = int(np.searchsorted(x, lon_c[jc, 0]))
+    i0 = int(np.clip(ic - 0.6 * span, 1, nx - span - 2))
+    j0 = int(np.clip(jc - span / 2, 1, ny - span - 2))
+    xc = x[i0] + np.arange(nc) * (dl / refine)
+    yc = y[j0] + np.arange(nc) * (dl / refine)
+    m = int(span) + 2
+    interp = RegularGridInterpolator((y[j0:j0 + m], x[i0:i0 + m]), dep[j0:j0 + m, i0:i0 + m])
     Yc, Xc = np.meshgrid(yc, xc, indexing="ij")
     depc = interp(np.stack([Yc, Xc], axis=-1))
-    return case, xc, yc, depc
+    meta = dict(i0=i0, j0=j0, span=span, coast_col=ic, source_row=jc, child_ground=0.5)
+    return case, xc, yc, depc, meta
 
 
 def ssl_condition(dep, depmin=1.0, alpha=2.0, passes=50):

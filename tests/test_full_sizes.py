@@ -12,6 +12,8 @@ the oracle to run the whole thing where it cannot:
   stepped from feed records brought to the host and blended there as
   cliffs_main.f:305-339 does.
 """
+import functools
+
 import numpy as np
 import pytest
 
@@ -81,23 +83,31 @@ def _host_edges(recs, state, t, mx):
     return out
 
 
+@functools.lru_cache(maxsize=1)
+def _c3():
+    """The nested workload's parent and child (synth.coastal_nest), generated
+    once per session (about a minute of numpy)."""
+    return synth.coastal_nest()
+
+
 def test_c3_nest_full_size(gpu_lib):
-    # child: parent cells 5350..6032 x 3000..3682 (coast) at 3x, 2047^2
-    case, xc, yc, depc = synth.coastal_nest()
+    # child: 2048^2 nodes at 3x over the parent's coast opposite the source
+    case, xc, yc, depc, meta = _c3()
     x, y = case.xcrd, case.ycrd
     maps = gpu_lib.nest_maps(x, y, xc, yc)
     s1, s2, zeta = case.metrics()
     cs1, cs2, czeta = synth.metrics(xc, yc, False)
 
     def parent():
-        p = gpu_lib.Grid(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground)
+        p = gpu_lib.Grid(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground, crough=case.crough)
         p.upload(case.cel, case.xvel, case.yvel)
         return p
 
     def child():
-        c = gpu_lib.Grid(depc, cs1, cs2, czeta, dt=case.dt / 3, ground=case.ground)
+        cg = meta["child_ground"]
+        c = gpu_lib.Grid(depc, cs1, cs2, czeta, dt=case.dt / 3, ground=cg, crough=case.crough)
         z = np.zeros_like(depc)
-        c.upload(synth.depth_to_celerity(depc, case.ground), z, z)
+        c.upload(synth.depth_to_celerity(depc, cg), z, z)
         return c
 
     gA, cA = parent(), child()
@@ -140,6 +150,51 @@ def test_c3_nest_full_size(gpu_lib):
         h.close()
     for h in (gA, gB, cA, cB):
         h.close()
+
+
+def test_c3_parent_window_parity(gpu_lib):
+    """A window of the nested workload's parent (spherical, 30 arc-seconds,
+    its shelf, coast and dipole source, 560 x 520 cells from the source to
+    the land behind the coast) stepped through the wave's arrival and run-up
+    on the device and by the FP64 oracle: first-step wet/dry masks equal,
+    surface elevation within 1e-5 of the source amplitude on cells wet in
+    both, run-up within one land cell's ground height."""
+    import oracle as O
+    case, xc, yc, depc, meta = _c3()
+    jc, ic = meta["source_row"], meta["coast_col"]
+    src_lon = case.meta["source"][0]
+    i_src = int(np.searchsorted(case.xcrd, src_lon))
+    i0, i1 = max(i_src - 120, 0), min(ic + 80, case.dep.shape[1])
+    j0, j1 = jc - 260, jc + 260
+    blk = (slice(j0, j1), slice(i0, i1))
+    dep = np.ascontiguousarray(case.dep[blk])
+    x, y = case.xcrd[i0:i1], case.ycrd[j0:j1]
+    s1, s2, zeta = synth.metrics(x, y, False)
+    cel = np.ascontiguousarray(case.cel[blk])
+    z = np.zeros_like(dep)
+    orc = O.Oracle(dep, s1, s2, zeta, dt=case.dt, ground=case.ground, crough=case.crough)
+    orc.set_state(cel, z, z)
+    g = gpu_lib.Grid(dep, s1, s2, zeta, dt=case.dt, ground=case.ground, crough=case.crough)
+    g.upload(cel, z, z)
+    orc.steps(1)
+    g.run(1)
+    cm = orc.celmin
+    assert np.array_equal(orc.get_state()[0] > cm, g.download()[0] > cm)
+    orc.steps(1799)
+    g.run(1799)
+    c_o, u_o, v_o, hm_o, vm_o = orc.get_state()
+    c_g = g.download()[0]
+    hm_g = g.download_max()[0]
+    both = (c_o > cm) & (c_g > cm)
+    err = float(np.abs((c_o ** 2 - c_g ** 2) / orc.grav)[both].max())
+    assert err <= 1e-5 * 3.0, err
+    ru_o = synth.max_runup(hm_o, dep, cm)
+    ru_g = synth.max_runup(hm_g, dep, cm)
+    rise = 300.0 / 50e3 * float(s1[j1 - j0 - 1].min())   # ground height of one land cell, 300 m over 50 km
+    assert ru_o > 0.5, "the wave must run up the coast in the window"
+    assert abs(ru_g - ru_o) <= rise, (ru_o, ru_g, rise)
+    print(f"C3 window {dep.shape}: eta error {err:.2e} m, run-up device {ru_g:.2f} m / oracle {ru_o:.2f} m")
+    g.close()
 
 
 def test_largest_grids_step_as_two_kernels(gpu_lib):
