@@ -69,6 +69,8 @@ def load_library():
         "cliffs_run_timed": ([H, C.c_int, _dp, _dp, _dp], C.c_int),
         "cliffs_set_fused": ([H, C.c_int], C.c_int),
         "cliffs_set_tile_rows": ([H, C.c_int], C.c_int),
+        "cliffs_set_tile_width": ([H, C.c_int], C.c_int),
+        "cliffs_ensemble_shape": ([H, _ip, _ip], C.c_int),
         "cliffs_run_gauges": ([H, C.c_int, C.c_int, C.c_int, _ip, _ip, C.POINTER(C.c_float)],
                               C.c_int),
         "cliffs_gauges": ([H, C.c_int, _ip, _ip, _dp, _dp, _dp], C.c_int),
@@ -235,6 +237,10 @@ class Grid:
         """2-D steps as one fused kernel (default) or as two sweep kernels."""
         _chk(load_library().cliffs_set_fused(self._h, 1 if on else 0))
 
+    def set_tile_width(self, width: int):
+        """Tile width of the fused step: 248 (default) or 120 columns."""
+        _chk(load_library().cliffs_set_tile_width(self._h, int(width)))
+
     def set_tile_rows(self, rows: int):
         """Tile height of the fused step (8..256, 0 = automatic)."""
         _chk(load_library().cliffs_set_tile_rows(self._h, int(rows)))
@@ -311,6 +317,12 @@ class Ensemble:
         r = C.c_int()
         _chk(load_library().cliffs_ensemble_tune(self._h, int(trial_steps), C.byref(r)))
         return r.value
+
+    def shape(self):
+        """-> (tile width, tile height) the ensemble steps with."""
+        w, r = C.c_int(), C.c_int()
+        _chk(load_library().cliffs_ensemble_shape(self._h, C.byref(w), C.byref(r)))
+        return w.value, r.value
 
     def sync(self):
         _chk(load_library().cliffs_ensemble_sync(self._h))

@@ -40,13 +40,13 @@ hipError_t launch_step_fused(const FArgs& a, hipStream_t st);
 hipError_t launch_step_fixup(const FArgs& a, hipStream_t st);
 hipError_t launch_fused_flags(const FArgs& a, unsigned char* ctr0, unsigned char* ctr1,
                               hipStream_t st);
-hipError_t fused_blocks_per_cu(int* n);
-hipError_t launch_step_batch(const FArgs* args, const int2* bmap, int nblocks, hipStream_t st);
-hipError_t launch_fixup_batch(const FArgs* args, const int2* bmap, int nblocks, hipStream_t st);
+hipError_t fused_blocks_per_cu(int strip_w, int* n);
+hipError_t launch_step_batch(const FArgs* args, const int2* bmap, int nblocks, int strip_w, hipStream_t st);
+hipError_t launch_fixup_batch(const FArgs* args, const int2* bmap, int nblocks, int strip_w, hipStream_t st);
 hipError_t launch_ensemble_gauges(const FArgs* args, int nm, const int* ij, int n, int nsamples, int slot,
                                   float* series, hipStream_t st);
 void ensemble_block_map(const int* ntiles, int n, std::vector<int2>& bmap);
-void fused_geometry(int nx, int ny, int cus, int blocks_per_cu, int* nstrips, int* nchunks,
+void fused_geometry(int nx, int ny, int strip_w, int cus, int blocks_per_cu, int* nstrips, int* nchunks,
                     int* rows);
 }  // namespace cliffs
 
@@ -120,8 +120,9 @@ struct cliffs_handle {
     // fused 2-D step (step_fused.hip)
     bool fused = true;         // 2-D steps run as one fused kernel (+ fixup)
     bool fused_ok = false;     // the nine fields span < 4 GiB (32-bit offsets)
+    int f_width = kFStrip;     // tile width (kFStrip or kFStripNarrow)
     int f_nstrips = 0, f_nchunks = 0, f_rows = 0;
-    int f_cus = 0, f_bpc = 0;  // CUs of the device, fused-kernel blocks per CU
+    int f_cus = 0, f_bpc = 0, f_bpc_narrow = 0;  // CUs of the device, fused-kernel blocks per CU
     int f_cstride = 0;         // bytes per column of cpr: roundup(nchunks, 16)
     unsigned char* fflags = nullptr;  // rfl[2][nstrips][ny], cpr[2][nx][cstride], ctr[2][nx], mpend[ntiles]
     int fn = 0;                // fused step parity
@@ -415,6 +416,7 @@ FArgs fargs(cliffs_handle* h, const float* xe, const float* ye) {
     a.zeta = h->zeta;
     a.yedges = ye;
     a.p = h->p;
+    a.strip_w = h->f_width;
     a.nstrips = h->f_nstrips;
     a.nchunks = h->f_nchunks;
     a.rows = h->f_rows;
@@ -449,26 +451,30 @@ int ensure_fused(cliffs_handle* h) {
     return CLIFFS_OK;
 }
 
-// Tile geometry of the fused step: rows per tile (0: enough tiles to fill the
-// device once, for a grid stepped alone) and the per-tile flag buffers.
-int set_fused_rows(cliffs_handle* h, int rows) {
+// Tile geometry of the fused step: tile width (kFStrip or kFStripNarrow),
+// rows per tile (0: enough tiles to fill the device once, for a grid stepped
+// alone) and the per-tile flag buffers.
+int set_fused_shape(cliffs_handle* h, int width, int rows) {
     int ns, nc, r;
     if (rows <= 0) {
-        fused_geometry(h->nx, h->ny, h->f_cus, h->f_bpc, &ns, &nc, &r);
+        fused_geometry(h->nx, h->ny, width, h->f_cus, width == kFStrip ? h->f_bpc : h->f_bpc_narrow,
+                       &ns, &nc, &r);
     } else {
-        ns = (h->nx + kFStrip - 1) / kFStrip;
+        ns = (h->nx + width - 1) / width;
         r = std::min(std::min(rows, kFMaxRows), h->ny);
         nc = (h->ny + r - 1) / r;
         // the same number of chunks, of equal height: no short last chunk
         const int rb = (h->ny + nc - 1) / nc;
         if (rb >= 8) r = rb;
     }
-    if (h->fflags && ns == h->f_nstrips && nc == h->f_nchunks && r == h->f_rows) return CLIFFS_OK;
+    if (h->fflags && width == h->f_width && ns == h->f_nstrips && nc == h->f_nchunks && r == h->f_rows)
+        return CLIFFS_OK;
     if (h->fflags) {
         HIPCHK(hipStreamSynchronize(h->st));
         HIPCHK(hipFree(h->fflags));
         h->fflags = nullptr;
     }
+    h->f_width = width;
     h->f_nstrips = ns;
     h->f_nchunks = nc;
     h->f_rows = r;
@@ -483,6 +489,8 @@ int set_fused_rows(cliffs_handle* h, int rows) {
     HIPCHK(hipMemset(h->fflags, 0, fb));
     return CLIFFS_OK;
 }
+
+int set_fused_rows(cliffs_handle* h, int rows) { return set_fused_shape(h, h->f_width, rows); }
 
 // Optional events bracketing the kernels of a step: ev[0], ev[1] around the
 // x-sweep (or the fused step kernel), ev[2], ev[3] around the y-sweep (or
@@ -669,7 +677,9 @@ int cliffs_create(const cliffs_grid_desc* d, cliffs_handle** out) {
     {  // fused step: tile geometry for this device, per-tile line flags
         if (hipDeviceGetAttribute(&h->f_cus, hipDeviceAttributeMultiprocessorCount, h->dev) !=
                 hipSuccess ||
-            fused_blocks_per_cu(&h->f_bpc) != hipSuccess || h->f_cus < 1 || h->f_bpc < 1)
+            fused_blocks_per_cu(kFStrip, &h->f_bpc) != hipSuccess ||
+            fused_blocks_per_cu(kFStripNarrow, &h->f_bpc_narrow) != hipSuccess || h->f_cus < 1 ||
+            h->f_bpc < 1 || h->f_bpc_narrow < 1)
             return bail(fail(CLIFFS_ERR_HIP, "device occupancy query"));
         const int rc = set_fused_rows(h, 0);
         if (rc) return bail(rc);
@@ -1055,6 +1065,17 @@ int cliffs_set_fused(cliffs_handle* h, int on) {
     return CLIFFS_OK;
 }
 
+int cliffs_set_tile_width(cliffs_handle* h, int width) {
+    if (!h) return fail(CLIFFS_ERR_ARG, "NULL handle");
+    if (width != kFStrip && width != kFStripNarrow)
+        return fail(CLIFFS_ERR_ARG, "tile width must be 248 or 120");
+    if (!h->fused_ok) return fail(CLIFFS_ERR_ARG, "grid has no fused step (1-D or too large)");
+    if (h->ens_users > 0)
+        return fail(CLIFFS_ERR_STATE, "grid belongs to an ensemble, whose tuner sets its tile shape");
+    HIPCHK(hipSetDevice(h->dev));
+    return set_fused_shape(h, width, 0);
+}
+
 int cliffs_set_tile_rows(cliffs_handle* h, int rows) {
     if (!h) return fail(CLIFFS_ERR_ARG, "NULL handle");
     if (rows != 0 && (rows < 8 || rows > kFMaxRows)) return fail(CLIFFS_ERR_ARG, "tile rows must be 0 or 8..256");
@@ -1072,6 +1093,7 @@ struct cliffs_ensemble {
     int dev = 0;
     hipStream_t st = nullptr;
     std::vector<cliffs_handle*> m;
+    int width = kFStrip;          // tile width of every member (one kernel per launch)
     int rows = 0;                 // common tile height (0: each member's own)
     FArgs* hargs = nullptr;       // pinned staging, [2][n]: this step's and the next step's
     FArgs* dargs = nullptr;       // device copy
@@ -1130,8 +1152,9 @@ int ensemble_rebuild(cliffs_ensemble* e) {
     HIPCHK(hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal));
     hipError_t err = hipSuccess;
     for (int k = 0; k < 2 && err == hipSuccess; k++) {
-        err = launch_step_batch(e->dargs + (size_t)k * n, e->dmap, e->nblocks, e->st);
-        if (err == hipSuccess) err = launch_fixup_batch(e->dargs + (size_t)k * n, e->dmap, e->nblocks, e->st);
+        err = launch_step_batch(e->dargs + (size_t)k * n, e->dmap, e->nblocks, e->width, e->st);
+        if (err == hipSuccess)
+            err = launch_fixup_batch(e->dargs + (size_t)k * n, e->dmap, e->nblocks, e->width, e->st);
     }
     hipGraph_t g = nullptr;
     const hipError_t ec = hipStreamEndCapture(e->st, &g);
@@ -1148,13 +1171,22 @@ int ensemble_rebuild(cliffs_ensemble* e) {
 int ensemble_take(cliffs_ensemble* e) {
     if (!e->gexec) return fail(CLIFFS_ERR_STATE, "ensemble has no step graph (a rebuild failed)");
     // a member shared with another ensemble may have been re-tiled by that
-    // ensemble's tuner: the block map and graph must match its tiles
-    for (size_t i = 0; i < e->m.size(); i++)
-        if (e->m[i]->f_nstrips * e->m[i]->f_nchunks != e->tiles[i]) {
-            const int rc = ensemble_rebuild(e);
+    // ensemble's tuner: give it this ensemble's tile width back, and make the
+    // block map and graph match its tiles
+    bool rebuild = false;
+    for (size_t i = 0; i < e->m.size(); i++) {
+        cliffs_handle* h = e->m[i];
+        if (h->f_width != e->width) {
+            HIPCHK(hipStreamSynchronize(e->st));
+            const int rc = set_fused_shape(h, e->width, e->rows);
             if (rc) return rc;
-            break;
         }
+        if (h->f_nstrips * h->f_nchunks != e->tiles[i]) rebuild = true;
+    }
+    if (rebuild) {
+        const int rc = ensemble_rebuild(e);
+        if (rc) return rc;
+    }
     for (cliffs_handle* h : e->m) {
         int rc;
         if ((rc = ensure_fused(h))) return rc;
@@ -1194,17 +1226,17 @@ int ensemble_args(cliffs_ensemble* e, int maxwave = 1) {
 
 int ensemble_single(cliffs_ensemble* e, int slot) {
     const FArgs* a = e->dargs + (size_t)slot * e->m.size();
-    hipError_t err = launch_step_batch(a, e->dmap, e->nblocks, e->st);
+    hipError_t err = launch_step_batch(a, e->dmap, e->nblocks, e->width, e->st);
     if (err != hipSuccess) return hipfail(err, "ensemble step");
-    err = launch_fixup_batch(a, e->dmap, e->nblocks, e->st);
+    err = launch_fixup_batch(a, e->dmap, e->nblocks, e->width, e->st);
     if (err != hipSuccess) return hipfail(err, "ensemble fixup");
     return CLIFFS_OK;
 }
 
-int ensemble_set_rows(cliffs_ensemble* e, int rows) {
+int ensemble_set_shape(cliffs_ensemble* e, int width, int rows) {
     HIPCHK(hipStreamSynchronize(e->st));
     for (cliffs_handle* h : e->m) {
-        const int rc = set_fused_rows(h, rows);
+        const int rc = set_fused_shape(h, width, rows);
         if (rc) {
             // the captured graph and block map no longer match the members'
             // tiles: refuse to run until a rebuild succeeds
@@ -1215,6 +1247,7 @@ int ensemble_set_rows(cliffs_ensemble* e, int rows) {
             return rc;
         }
     }
+    e->width = width;
     e->rows = rows;
     return ensemble_rebuild(e);
 }
@@ -1235,10 +1268,13 @@ int cliffs_ensemble_create(cliffs_handle* const* members, int n, cliffs_ensemble
         if (!h->state) return fail(CLIFFS_ERR_STATE, "member without uploaded state");
         for (int k = 0; k < i; k++)
             if (members[k] == h) return fail(CLIFFS_ERR_ARG, "member listed twice");
+        if (h->f_width != members[0]->f_width)
+            return fail(CLIFFS_ERR_ARG, "members with different tile widths");
     }
     cliffs_ensemble* e = new (std::nothrow) cliffs_ensemble;
     if (!e) return fail(CLIFFS_ERR_NOMEM, "host allocation");
     e->dev = members[0]->dev;
+    e->width = members[0]->f_width;
     e->m.assign(members, members + n);
     auto bail = [&](int rc) {
         free_ensemble(e);
@@ -1314,13 +1350,15 @@ int cliffs_ensemble_tune(cliffs_ensemble* e, int trial_steps, int* rows_out) {
     if (!e || trial_steps < 1) return fail(CLIFFS_ERR_ARG, "bad tune arguments");
     HIPCHK(hipSetDevice(e->dev));
     static const int kCand[] = {16, 24, 32, 48, 64, 96, 128, 160, 192, 256};
+    static const int kWidths[] = {kFStrip, kFStripNarrow};
     EventSet es;
     es.ev.assign(2, nullptr);
     for (auto& x : es.ev) HIPCHK(hipEventCreate(&x));
-    int best = 0, rc;
+    int best = 0, best_w = kFStrip, rc;
     float best_ms = 0.f;
+    for (int wd : kWidths)
     for (int c : kCand) {
-        if ((rc = ensemble_set_rows(e, c))) return rc;
+        if ((rc = ensemble_set_shape(e, wd, c))) return rc;
         if ((rc = ensemble_take(e)) || (rc = ensemble_args(e, 0))) return rc;
         // the same step, repeated: it reads the current state and writes the
         // other copy, so the state does not advance; the running maxima are
@@ -1336,13 +1374,21 @@ int cliffs_ensemble_tune(cliffs_ensemble* e, int trial_steps, int* rows_out) {
         for (cliffs_handle* h : e->m) h->fused_valid = false;
         if (best == 0 || t < best_ms) {
             best = c;
+            best_w = wd;
             best_ms = t;
         }
     }
-    if ((rc = ensemble_set_rows(e, best))) return rc;
+    if ((rc = ensemble_set_shape(e, best_w, best))) return rc;
     for (cliffs_handle* h : e->m) h->fused_valid = false;
     if ((rc = ensemble_give(e))) return rc;
     if (rows_out) *rows_out = best;
+    return CLIFFS_OK;
+}
+
+int cliffs_ensemble_shape(cliffs_ensemble* e, int* width, int* rows) {
+    if (!e || !width || !rows) return fail(CLIFFS_ERR_ARG, "NULL argument");
+    *width = e->width;
+    *rows = e->rows;
     return CLIFFS_OK;
 }
 

@@ -84,10 +84,15 @@ struct YArgs {
 
 // Fused 2-D step (step_fused.hip): tiles of kFStrip columns x `rows` rows.
 constexpr int kFStrip = 248;    // output columns per tile (x window of 256)
+constexpr int kFStripNarrow = 120;  // narrow tiles (x window of 128, 2 nodes per lane)
+// Strip width of a tile whose x window holds N nodes per lane (64 lanes)
+constexpr int strip_of(int n) { return 64 * n - 8; }
 constexpr int kFMaxRows = 256;  // rows per tile, at most (the tile's row metrics sit in LDS)
 constexpr int kXCDs = 8;        // MI355X: 8 accelerator complex dies, one L2 each
-constexpr int kFRoleThreads = 256;  // threads of each role of a fused-step block (x rows, y columns)
-constexpr int kFThreads = 2 * kFRoleThreads;
+constexpr int kFRoleThreads = 256;  // threads of the x role of a fused-step block
+// threads of a fused-step block with N nodes per lane: the x role's four
+// waves plus one y thread per window node
+constexpr int fused_threads(int n) { return kFRoleThreads + 64 * n; }
 
 // Line flags of the fused step are per-tile contributions (one byte per tile
 // and line, written 0 or 1 by the tile that owns it), so a tile recomputed by
@@ -104,6 +109,7 @@ struct FArgs {
     const float* zeta;
     const float* yedges;
     Params p;
+    int strip_w;                  // tile width: kFStrip or kFStripNarrow
     int nstrips, nchunks, rows, ntiles;
     int cstride;                  // bytes per column of cpr: roundup(nchunks, 16)
     const unsigned char* rfl_in;  // [nstrips][ny]: row j wet within strip s, state entering the step

@@ -61,9 +61,10 @@ __device__ __forceinline__ int tile_of_block(int b, int ntiles) {
     return lo + k < hi ? lo + k : -1;
 }
 
+template <int N>
 struct FShared {
-    float row[2][4][4][256];      // [buffer][row of the group][s, cp, u, v][window node]
-    float4 xmt[256];              // x metric of the tile's window nodes
+    float row[2][4][4][64 * N];   // [buffer][row of the group][s, cp, u, v][window node]
+    float4 xmt[64 * N];           // x metric of the tile's window nodes
     // row metrics of the tile's stream rows, staged once per tile so that no
     // row waits on a global load: y metric and zeta of rows j0-5 .. j1, x row
     // factors of rows j0-3 .. j1+2
@@ -73,7 +74,7 @@ struct FShared {
     // [wave][row of the tile]: the wave's columns of the row have a wet node
     // after the step.  Each wave writes each row of the tile exactly once
     // (all lanes, the same value): no clearing, no atomics.
-    unsigned char rowany[4][kFMaxRows];
+    unsigned char rowany[N][kFMaxRows];
 };
 
 // Field access by 32-bit element offset from the common base: global
@@ -84,9 +85,9 @@ struct Fields {
     __device__ __forceinline__ float ld(unsigned o) const {
         return *reinterpret_cast<const float*>(base + o * 4u);
     }
-    __device__ __forceinline__ Q4 ld4(unsigned o) const {
-        const float4 t = *reinterpret_cast<const float4*>(base + o * 4u);
-        return Q4{{t.x, t.y, t.z, t.w}};
+    template <int N>
+    __device__ __forceinline__ QN<N> ldq(unsigned o) const {
+        return cliffs::ldq<N>(reinterpret_cast<const float*>(base + o * 4u));
     }
     __device__ __forceinline__ void st(unsigned o, float v) const {
         *reinterpret_cast<float*>(base + o * 4u) = v;
@@ -104,8 +105,9 @@ struct Fields {
 //   iteration k:  X: rows of group k -> ring[k & 1]      Y: group k-1 from ring[(k-1) & 1]
 // so a role's registers are live only in its own waves, and the x-sweep of
 // one group overlaps the y-sweep of the group before.
-template <bool kExact>
-__device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh) {
+template <bool kExact, int N>
+__device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared<N>& sh) {
+    constexpr int kW = strip_of(N), kWin = 64 * N, kThreads = fused_threads(N);
     const Params& p = a.p;
     const int nx = a.g.nx, ny = a.g.ny;
     const unsigned pitch = (unsigned)a.g.pitch;
@@ -115,20 +117,21 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
     const unsigned oCPn = (4u - 3u * a.cur) * fs, oUn = oCPn + fs, oVn = oCPn + 2u * fs;
     const unsigned oHM = 7u * fs, oVM = 8u * fs;
     const int strip = tile % a.nstrips, chunk = tile / a.nstrips;
-    const int cs = strip * kFStrip;
+    const int cs = strip * kW;
     const int j0 = chunk * a.rows, j1 = min(j0 + a.rows, ny);
     const int t = threadIdx.x;
     const bool xrole = t < kFRoleThreads;               // wave-uniform
-    const int lane = t & 63, w = (t >> 6) & 3;          // wave within its role
+    const int lane = t & 63;
+    const int w = xrole ? (t >> 6) : ((t - kFRoleThreads) >> 6);  // wave within its role
     const int ty = t - kFRoleThreads;                   // column thread of the Y role
 
     // staged tables: x metric of the window nodes, row metrics of the stream rows
-    if (xrole) {
-        const int i = cs - 4 + t;
-        sh.xmt[t] = (i >= 0 && i < nx) ? a.xm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int k = t; k < kWin; k += kThreads) {
+        const int i = cs - 4 + k;
+        sh.xmt[k] = (i >= 0 && i < nx) ? a.xm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
     const int yb = j0 - 5, xb = j0 - 3;  // first rows of the staged row tables
-    for (int k = t; k < j1 - j0 + 6; k += kFThreads) {
+    for (int k = t; k < j1 - j0 + 6; k += kThreads) {
         const int ry = yb + k, rx = xb + k;
         sh.ymt[k] = (ry >= 0 && ry < ny) ? a.ym[ry] : make_float4(0.f, 0.f, 0.f, 0.f);
         sh.zt[k] = (ry >= 0 && ry < ny) ? a.zeta[ry] : 0.f;
@@ -140,16 +143,16 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
     const int nstream = j1 - j0 + 6;
     const int ngroups = (nstream + 3) / 4;
 
-    // X role: lane holds window nodes cs-4+4*lane .. +3
-    const int i0 = cs - 4 + 4 * lane;
+    // X role: lane holds window nodes cs-4+N*lane .. +N-1
+    const int i0 = cs - 4 + N * lane;
     const int ld_lane = (xrole && i0 >= 0 && i0 < nx) ? 1 : 0;
-    const XLane XL{i0, xlane_vmask(nx, i0), &sh.xmt[4 * lane]};
+    const XLane XL{i0, xlane_vmask<N>(nx, i0), &sh.xmt[N * lane]};
     const int outmask = (xrole && lane >= 1 && lane <= 62) ? XL.vmask : 0;
     int colwet_x = 0;
 
     // Y role: thread ty owns column cs+ty (window node ty+4)
     const int col = cs + ty;
-    const int col_ok = (!xrole && ty < kFStrip && col < nx) ? 1 : 0;
+    const int col_ok = (!xrole && ty < kW && col < nx) ? 1 : 0;
     const int in_range = (col_ok && col >= a.col_lo && col <= a.col_hi) ? 1 : 0;
     // max-wave of the column updated here (keepmax) or deferred (loadmax
     // only: the tile is flagged if the update would change a maximum)
@@ -158,7 +161,7 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
 
     if (xrole) {
         // ---- X role: the wave's next row is loaded one group ahead
-        Q4 ns = zero4(), ncp = zero4(), nu = zero4(), nv = zero4();
+        QN<N> ns = zeroq<N>(), ncp = zeroq<N>(), nu = zeroq<N>(), nv = zeroq<N>();
         int nflag = 0;
         auto x_valid = [&](int g) {
             const int q = j0 - 3 + 4 * g + w;
@@ -169,10 +172,10 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
             const int q = j0 - 3 + 4 * g + w;
             if (ld_lane) {
                 const unsigned o = (unsigned)q * pitch + (unsigned)i0;
-                ns = F.ld4(o);
-                ncp = F.ld4(oCP + o);
-                nu = F.ld4(oU + o);
-                nv = F.ld4(oV + o);
+                ns = F.template ldq<N>(o);
+                ncp = F.template ldq<N>(oCP + o);
+                nu = F.template ldq<N>(oU + o);
+                nv = F.template ldq<N>(oV + o);
             }
             int f = 0;  // row has a wet node in some strip (previous step)
             for (int s0 = 0; s0 < a.nstrips; s0 += 64) {
@@ -187,21 +190,21 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
             // runs at raised wave priority against the Y waves on its SIMD
             __builtin_amdgcn_s_setprio(1);
             if (g < ngroups) {
-                float(*buf)[4][256] = sh.row[g & 1];
+                float(*buf)[4][kWin] = sh.row[g & 1];
                 const int q = j0 - 3 + 4 * g + w;
                 const bool valid = x_valid(g);
-                const Q4 s = ns, cp = ncp, u = nu, v = nv;
+                const QN<N> s = ns, cp = ncp, u = nu, v = nv;
                 const bool fl = __ballot(nflag != 0) != 0;
                 if (g + 1 < ngroups) x_load(g + 1);
                 if (valid) {
                     const bool sweep = q >= a.row_lo && q <= a.row_hi && fl;
-                    Q4 ocp, ou, ov;
-                    x_row(XL, p, nx, ny, q, sweep, sh.xrt[q - xb], a.xedges, s, cp, u, v, ocp, ou, ov);
-                    st4(&buf[w][0][4 * lane], s);
-                    st4(&buf[w][1][4 * lane], ocp);
-                    st4(&buf[w][2][4 * lane], ou);
-                    st4(&buf[w][3][4 * lane], ov);
-                    if (!kExact && q >= j0 && q < j1) colwet_x |= wet_bits(s, ocp, outmask, p);
+                    QN<N> ocp, ou, ov;
+                    x_row<N>(XL, p, nx, ny, q, sweep, sh.xrt[q - xb], a.xedges, s, cp, u, v, ocp, ou, ov);
+                    stq<N>(&buf[w][0][N * lane], s);
+                    stq<N>(&buf[w][1][N * lane], ocp);
+                    stq<N>(&buf[w][2][N * lane], ou);
+                    stq<N>(&buf[w][3][N * lane], ov);
+                    if (!kExact && q >= j0 && q < j1) colwet_x |= wet_bits<N>(s, ocp, outmask, p);
                 }
             }
             __syncthreads();
@@ -238,7 +241,7 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
         hv_load(0);
         for (int g = -1; g < ngroups; g++) {
             if (g >= 0) {
-                float(*buf)[4][256] = sh.row[g & 1];
+                float(*buf)[4][kWin] = sh.row[g & 1];
                 float h[4], vm[4];
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
@@ -291,40 +294,48 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared& sh
     }
     const int tile_mchange = __syncthreads_or((colwet_f & 2) && loadmax && !keepmax);
     if (!kExact && t == 0) a.mpend[tile] = tile_mchange ? 1 : 0;
-    for (int k = t; k < j1 - j0; k += kFThreads) a.rfl_out[(size_t)strip * ny + j0 + k] =
-            (sh.rowany[0][k] | sh.rowany[1][k] | sh.rowany[2][k] | sh.rowany[3][k]) ? 1 : 0;
+    for (int k = t; k < j1 - j0; k += kThreads) {
+        unsigned char any = 0;
+#pragma unroll
+        for (int y = 0; y < N; y++) any |= sh.rowany[y][k];
+        a.rfl_out[(size_t)strip * ny + j0 + k] = any ? 1 : 0;
+    }
     if (col_ok) a.cpr_out[(size_t)col * a.cstride + chunk] = colwet_f & 1;
     if (!kExact && outmask && ld_lane && colwet_x) {
 #pragma unroll
-        for (int k = 0; k < 4; k++)
+        for (int k = 0; k < N; k++)
             if (bit(colwet_x, k)) a.ctr[i0 + k] = 1;
     }
 }
 
-// Eight waves per block (two roles), two blocks per CU: four waves per SIMD
-// (<= 128 VGPRs), each wave's registers holding one role's state only.
-__global__ __launch_bounds__(kFThreads, 4) void step_fused_kernel(const FArgs a) {
-    __shared__ FShared sh;
+// Eight waves per block (two roles) for 248-column tiles, six for 120-column
+// tiles; four waves per SIMD (<= 128 VGPRs), each wave's registers holding
+// one role's state only.
+template <int N>
+__global__ __launch_bounds__(fused_threads(N), 4) void step_fused_kernel(const FArgs a) {
+    __shared__ FShared<N> sh;
     const int tile = tile_of_block(blockIdx.x, a.ntiles);
     if (tile < 0) return;
-    fused_tile<false>(a, tile, sh);
+    fused_tile<false, N>(a, tile, sh);
 }
 
 // Checks the step kernel's column predictions of one tile: recomputes the
 // tile if a prediction was wrong, applies the deferred max-wave update of
 // columns that stayed dry, and clears the next step's ctr.
-__device__ __forceinline__ void fixup_tile(const FArgs& a, int tile, FShared& sh) {
+template <int N>
+__device__ __forceinline__ void fixup_tile(const FArgs& a, int tile, FShared<N>& sh) {
+    constexpr int kW = strip_of(N);
     const int nx = a.g.nx, ny = a.g.ny;
     const int strip = tile % a.nstrips, chunk = tile / a.nstrips;
     const int t = threadIdx.x;
-    const int col = strip * kFStrip + t;
-    const bool col_ok = t < kFStrip && col < nx;
+    const int col = strip * kW + t;
+    const bool col_ok = t < kW && col < nx;
     const bool in_range = col_ok && col >= a.col_lo && col <= a.col_hi;
     const bool pred = in_range && column_any(a.cpr_in, a.cstride, col);
     const bool tru = in_range && a.ctr[col] != 0;
     if (chunk == 0 && col_ok) a.ctr_next[col] = 0;
     if (__syncthreads_or(in_range && pred != tru)) {
-        fused_tile<true>(a, tile, sh);
+        fused_tile<true, N>(a, tile, sh);
         return;
     }
     // columns left unswept: max-wave of their final state, in tiles where the
@@ -342,29 +353,33 @@ __device__ __forceinline__ void fixup_tile(const FArgs& a, int tile, FShared& sh
     }
 }
 
-__global__ __launch_bounds__(kFThreads) void step_fixup_kernel(const FArgs a) {
-    __shared__ FShared sh;
+template <int N>
+__global__ __launch_bounds__(fused_threads(N)) void step_fixup_kernel(const FArgs a) {
+    __shared__ FShared<N> sh;
     const int tile = tile_of_block(blockIdx.x, a.ntiles);
     if (tile < 0) return;
-    fixup_tile(a, tile, sh);
+    fixup_tile<N>(a, tile, sh);
 }
 
-// Ensembles: one launch steps every member grid.  bmap[b] = (member, tile)
-// or (-1, -1); args[member] is that member's argument block for this step.
-__global__ __launch_bounds__(kFThreads, 4) void step_fused_batch_kernel(const FArgs* __restrict__ args,
-                                                                  const int2* __restrict__ bmap) {
-    __shared__ FShared sh;
+// Ensembles: one launch steps every member grid (all of one tile width).
+// bmap[b] = (member, tile) or (-1, -1); args[member] is that member's
+// argument block for this step.
+template <int N>
+__global__ __launch_bounds__(fused_threads(N), 4) void step_fused_batch_kernel(
+    const FArgs* __restrict__ args, const int2* __restrict__ bmap) {
+    __shared__ FShared<N> sh;
     const int2 bm = bmap[blockIdx.x];
     if (bm.x < 0) return;
-    fused_tile<false>(args[bm.x], bm.y, sh);
+    fused_tile<false, N>(args[bm.x], bm.y, sh);
 }
 
-__global__ __launch_bounds__(kFThreads) void step_fixup_batch_kernel(const FArgs* __restrict__ args,
-                                                               const int2* __restrict__ bmap) {
-    __shared__ FShared sh;
+template <int N>
+__global__ __launch_bounds__(fused_threads(N)) void step_fixup_batch_kernel(
+    const FArgs* __restrict__ args, const int2* __restrict__ bmap) {
+    __shared__ FShared<N> sh;
     const int2 bm = bmap[blockIdx.x];
     if (bm.x < 0) return;
-    fixup_tile(args[bm.x], bm.y, sh);
+    fixup_tile<N>(args[bm.x], bm.y, sh);
 }
 
 // Line flags of the current state (written to rfl_out, cpr_out) and cleared
@@ -376,9 +391,9 @@ __global__ void fused_rowflags_kernel(const FArgs a) {
     const int lane = threadIdx.x & 63;
     if (wid >= (long)a.nstrips * a.g.ny) return;
     const int s = (int)(wid / a.g.ny), j = (int)(wid % a.g.ny);
-    const int i1 = min((s + 1) * kFStrip, a.g.nx);
+    const int i1 = min((s + 1) * a.strip_w, a.g.nx);
     bool any = false;
-    for (int i = s * kFStrip + lane; i < i1; i += 64) {
+    for (int i = s * a.strip_w + lane; i < i1; i += 64) {
         const size_t o = (size_t)j * a.g.pitch + i;
         any = any | !is_land(c0_of(a.g.S[o]) + a.g.CP[o], a.p);
     }
@@ -436,22 +451,34 @@ hipError_t launch_ensemble_gauges(const FArgs* args, int nm, const int* ij, int 
 }
 
 hipError_t launch_step_fused(const FArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(step_fused_kernel, dim3(fused_blocks(a)), dim3(kFThreads), 0, st, a);
+    if (a.strip_w == kFStrip)
+        hipLaunchKernelGGL(step_fused_kernel<4>, dim3(fused_blocks(a)), dim3(fused_threads(4)), 0, st, a);
+    else
+        hipLaunchKernelGGL(step_fused_kernel<2>, dim3(fused_blocks(a)), dim3(fused_threads(2)), 0, st, a);
     return hipGetLastError();
 }
 
 hipError_t launch_step_fixup(const FArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(step_fixup_kernel, dim3(fused_blocks(a)), dim3(kFThreads), 0, st, a);
+    if (a.strip_w == kFStrip)
+        hipLaunchKernelGGL(step_fixup_kernel<4>, dim3(fused_blocks(a)), dim3(fused_threads(4)), 0, st, a);
+    else
+        hipLaunchKernelGGL(step_fixup_kernel<2>, dim3(fused_blocks(a)), dim3(fused_threads(2)), 0, st, a);
     return hipGetLastError();
 }
 
-hipError_t launch_step_batch(const FArgs* args, const int2* bmap, int nblocks, hipStream_t st) {
-    hipLaunchKernelGGL(step_fused_batch_kernel, dim3(nblocks), dim3(kFThreads), 0, st, args, bmap);
+hipError_t launch_step_batch(const FArgs* args, const int2* bmap, int nblocks, int strip_w, hipStream_t st) {
+    if (strip_w == kFStrip)
+        hipLaunchKernelGGL(step_fused_batch_kernel<4>, dim3(nblocks), dim3(fused_threads(4)), 0, st, args, bmap);
+    else
+        hipLaunchKernelGGL(step_fused_batch_kernel<2>, dim3(nblocks), dim3(fused_threads(2)), 0, st, args, bmap);
     return hipGetLastError();
 }
 
-hipError_t launch_fixup_batch(const FArgs* args, const int2* bmap, int nblocks, hipStream_t st) {
-    hipLaunchKernelGGL(step_fixup_batch_kernel, dim3(nblocks), dim3(kFThreads), 0, st, args, bmap);
+hipError_t launch_fixup_batch(const FArgs* args, const int2* bmap, int nblocks, int strip_w, hipStream_t st) {
+    if (strip_w == kFStrip)
+        hipLaunchKernelGGL(step_fixup_batch_kernel<4>, dim3(nblocks), dim3(fused_threads(4)), 0, st, args, bmap);
+    else
+        hipLaunchKernelGGL(step_fixup_batch_kernel<2>, dim3(nblocks), dim3(fused_threads(2)), 0, st, args, bmap);
     return hipGetLastError();
 }
 
@@ -468,8 +495,10 @@ hipError_t launch_fused_flags(const FArgs& a, unsigned char* ctr0, unsigned char
     return hipGetLastError();
 }
 
-hipError_t fused_blocks_per_cu(int* n) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, step_fused_kernel, kFThreads, 0);
+hipError_t fused_blocks_per_cu(int strip_w, int* n) {
+    if (strip_w == kFStrip)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, step_fused_kernel<4>, fused_threads(4), 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, step_fused_kernel<2>, fused_threads(2), 0);
 }
 
 // Block map of an ensemble step: every member's tiles, XCD by XCD.  Member
@@ -493,9 +522,9 @@ void ensemble_block_map(const int* ntiles, int n, std::vector<int2>& bmap) {
 
 // Tile geometry: enough tiles to fill every CU with `blocks_per_cu` blocks
 // once, chunks of at most kFMaxRows rows and at least 32.
-void fused_geometry(int nx, int ny, int cus, int blocks_per_cu, int* nstrips, int* nchunks,
+void fused_geometry(int nx, int ny, int strip_w, int cus, int blocks_per_cu, int* nstrips, int* nchunks,
                     int* rows) {
-    const int ns = (nx + kFStrip - 1) / kFStrip;
+    const int ns = (nx + strip_w - 1) / strip_w;
     const int target = cus * blocks_per_cu;
     int nc = target / ns;
     if (nc < 1) nc = 1;

@@ -44,35 +44,76 @@ __device__ __forceinline__ int from_right_i(int x) {
     return __builtin_amdgcn_mov_dpp(x, 0x130, 0xF, 0xF, true);
 }
 
-// Four consecutive nodes held by one lane; neighbour shifts along the row.
-struct Q4 {
-    float v[4];
+// N consecutive nodes held by one lane (N = 4: a 256-node window per wave,
+// N = 2: a 128-node window); neighbour shifts along the row.
+template <int N>
+struct QN {
+    float v[N];
 };
-__device__ __forceinline__ Q4 lft(const Q4& a) {
-    return Q4{{from_left(a.v[3]), a.v[0], a.v[1], a.v[2]}};
+using Q4 = QN<4>;
+template <int N>
+__device__ __forceinline__ QN<N> lft(const QN<N>& a) {
+    QN<N> r;
+    r.v[0] = from_left(a.v[N - 1]);
+#pragma unroll
+    for (int k = 1; k < N; k++) r.v[k] = a.v[k - 1];
+    return r;
 }
-__device__ __forceinline__ Q4 rgt(const Q4& a) {
-    return Q4{{a.v[1], a.v[2], a.v[3], from_right(a.v[0])}};
+template <int N>
+__device__ __forceinline__ QN<N> rgt(const QN<N>& a) {
+    QN<N> r;
+#pragma unroll
+    for (int k = 0; k + 1 < N; k++) r.v[k] = a.v[k + 1];
+    r.v[N - 1] = from_right(a.v[0]);
+    return r;
 }
-__device__ __forceinline__ int lbits(int b) { return ((b << 1) & 0xE) | ((from_left_i(b) >> 3) & 1); }
-__device__ __forceinline__ int rbits(int b) { return ((b >> 1) & 0x7) | ((from_right_i(b) & 1) << 3); }
+// bit k of the lane's nodes shifted by one node along the row
+template <int N>
+__device__ __forceinline__ int lbits(int b) {
+    return ((b << 1) & ((1 << N) - 2)) | ((from_left_i(b) >> (N - 1)) & 1);
+}
+template <int N>
+__device__ __forceinline__ int rbits(int b) {
+    return ((b >> 1) & ((1 << (N - 1)) - 1)) | ((from_right_i(b) & 1) << (N - 1));
+}
 __device__ __forceinline__ bool bit(int m, int k) { return (m >> k) & 1; }
 
-__device__ __forceinline__ Q4 ld4(const float* p) {
-    const float4 t = *reinterpret_cast<const float4*>(p);
-    return Q4{{t.x, t.y, t.z, t.w}};
+template <int N>
+__device__ __forceinline__ QN<N> ldq(const float* p) {
+    QN<N> q;
+    if constexpr (N == 4) {
+        const float4 t = *reinterpret_cast<const float4*>(p);
+        q.v[0] = t.x, q.v[1] = t.y, q.v[2] = t.z, q.v[3] = t.w;
+    } else {
+        const float2 t = *reinterpret_cast<const float2*>(p);
+        q.v[0] = t.x, q.v[1] = t.y;
+    }
+    return q;
 }
-__device__ __forceinline__ void st4(float* p, const Q4& q) {
-    *reinterpret_cast<float4*>(p) = make_float4(q.v[0], q.v[1], q.v[2], q.v[3]);
+template <int N>
+__device__ __forceinline__ void stq(float* p, const QN<N>& q) {
+    if constexpr (N == 4)
+        *reinterpret_cast<float4*>(p) = make_float4(q.v[0], q.v[1], q.v[2], q.v[3]);
+    else
+        *reinterpret_cast<float2*>(p) = make_float2(q.v[0], q.v[1]);
 }
-__device__ __forceinline__ Q4 zero4() { return Q4{{0.f, 0.f, 0.f, 0.f}}; }
+template <int N>
+__device__ __forceinline__ QN<N> zeroq() {
+    QN<N> q;
+#pragma unroll
+    for (int k = 0; k < N; k++) q.v[k] = 0.f;
+    return q;
+}
+__device__ __forceinline__ Q4 ld4(const float* p) { return ldq<4>(p); }
+__device__ __forceinline__ void st4(float* p, const Q4& q) { stq<4>(p, q); }
+__device__ __forceinline__ Q4 zero4() { return zeroq<4>(); }
 
 __device__ __forceinline__ bool wave_all(bool x) { return __ballot(!x) == 0; }
 
 // ---------------------------------------------------------------------------
 // x-sweep of one row
 
-// Per-lane part of the x window: the lane holds nodes i0 .. i0+3; mt points
+// Per-lane part of the x window: the lane holds nodes i0 .. i0+N-1; mt points
 // at their column metric (XArgs::xm), in registers or in LDS.
 struct XLane {
     int i0;
@@ -80,19 +121,21 @@ struct XLane {
     const float4* mt;   // mt[k]: metric of node i0+k (zero outside the grid)
 };
 
+template <int N = 4>
 __device__ __forceinline__ int xlane_vmask(int nx, int i0) {
     int m = 0;
 #pragma unroll
-    for (int k = 0; k < 4; k++) m |= (i0 + k >= 0 && i0 + k < nx) ? (1 << k) : 0;
+    for (int k = 0; k < N; k++) m |= (i0 + k >= 0 && i0 + k < nx) ? (1 << k) : 0;
     return m;
 }
 
 // General path of x_row: wet/dry front, shorelines, line ends.
+template <int N>
 __device__ __forceinline__ void x_row_general(const XLane& L, const Params& p, int nx, int ny,
-                                               int j, float2 rr, const float* edges, const Q4& s,
-                                               const Q4& cp, const Q4& u, const Q4& v, const Q4& c,
-                                               const Q4& ge, int land, int deep, Q4& ocp, Q4& ou,
-                                               Q4& ov) {
+                                               int j, float2 rr, const float* edges, const QN<N>& s,
+                                               const QN<N>& cp, const QN<N>& u, const QN<N>& v, const QN<N>& c,
+                                               const QN<N>& ge, int land, int deep, QN<N>& ocp, QN<N>& ou,
+                                               QN<N>& ov) {
     float ubw = 0.f, vbw = 0.f, gbw = 0.f, ube = 0.f, vbe = 0.f, gbe = 0.f;
     if (edges) {
         ubw = edges[j];
@@ -102,27 +145,27 @@ __device__ __forceinline__ void x_row_general(const XLane& L, const Params& p, i
         vbe = edges[4 * ny + j];
         gbe = edges[5 * ny + j];
     }
-    const Q4 Lu = lft(u), Lv = lft(v), Lg = lft(ge);
-    const int Lland = lbits(land);
+    const QN<N> Lu = lft(u), Lv = lft(v), Lg = lft(ge);
+    const int Lland = lbits<N>(land);
     int RE = 0;
 #pragma unroll
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < N; k++)
         RE |= (bit(land, k) && !bit(Lland, k) && floods(Lg.v[k], s.v[k], p) && Lu.v[k] >= 0.0f)
                   ? (1 << k)
                   : 0;
-    const int start = ~land & Lland & ~lbits(RE) & 0xF;
-    const int Rstart = rbits(start);
-    const Q4 Ru = rgt(u), Rv = rgt(v), Rg = rgt(ge);
+    const int start = ~land & Lland & ~lbits<N>(RE) & ((1 << N) - 1);
+    const int Rstart = rbits<N>(start);
+    const QN<N> Ru = rgt(u), Rv = rgt(v), Rg = rgt(ge);
     int LE = 0;
 #pragma unroll
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < N; k++)
         LE |= (bit(L.vmask, k) && bit(Rstart, k) && floods(Rg.v[k], s.v[k], p) && Ru.v[k] <= 0.0f)
                   ? (1 << k)
                   : 0;
-    const int nl = land & ~RE & ~LE & 0xF;
-    Q4 pcp, pu, pv, pc, pg;
+    const int nl = land & ~RE & ~LE & ((1 << N) - 1);
+    QN<N> pcp, pu, pv, pc, pg;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < N; k++) {
         const float c0 = c0_of(s.v[k]);
         const bool re = bit(RE, k), le = bit(LE, k), n0 = bit(nl, k);
         pcp.v[k] = (re || le) ? p.celmin - c0 : (n0 ? -c0 : cp.v[k]);
@@ -131,21 +174,21 @@ __device__ __forceinline__ void x_row_general(const XLane& L, const Params& p, i
         pc.v[k] = c0 + pcp.v[k];
         pg.v[k] = geta_of(s.v[k], pcp.v[k]);
     }
-    const int Lnl = lbits(nl), Rnl = rbits(nl);
-    const Q4 Rs = rgt(s), Rpcp = rgt(pcp), Rpu = rgt(pu), Rpv = rgt(pv), Rpc = rgt(pc), Rpg = rgt(pg);
-    const Q4 Ls = lft(s), Lpcp = lft(pcp), Lpu = lft(pu), Lpv = lft(pv), Lpc = lft(pc);
-    Q4 Ep, Eq;
+    const int Lnl = lbits<N>(nl), Rnl = rbits<N>(nl);
+    const QN<N> Rs = rgt(s), Rpcp = rgt(pcp), Rpu = rgt(pu), Rpv = rgt(pv), Rpc = rgt(pc), Rpg = rgt(pg);
+    const QN<N> Ls = lft(s), Lpcp = lft(pcp), Lpu = lft(pu), Lpv = lft(pv), Lpc = lft(pc);
+    QN<N> Ep, Eq;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < N; k++) {
         const EdgeRes e = edge_res(Node{s.v[k], pcp.v[k], pu.v[k], pv.v[k]}, pc.v[k], pg.v[k],
                                    Node{Rs.v[k], Rpcp.v[k], Rpu.v[k], Rpv.v[k]}, Rpc.v[k], Rpg.v[k]);
         Ep.v[k] = e.ep;
         Eq.v[k] = e.eq;
     }
-    const Q4 LEp = lft(Ep), LEq = lft(Eq);
-    Q4 lc, lu;
+    const QN<N> LEp = lft(Ep), LEq = lft(Eq);
+    QN<N> lc, lu;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < N; k++) {
         const int i = L.i0 + k;
         const Node X{s.v[k], pcp.v[k], pu.v[k], pv.v[k]};
         const Node Lr{Ls.v[k], Lpcp.v[k], Lpu.v[k], Lpv.v[k]};
@@ -184,9 +227,9 @@ __device__ __forceinline__ void x_row_general(const XLane& L, const Params& p, i
         lc.v[k] = c0_of(s.v[k]) + (edge_node ? pcp.v[k] : xc);
         lu.v[k] = edge_node ? pu.v[k] : xu;
     }
-    const Q4 Llc = lft(lc), Llu = lft(lu), Rlc = rgt(lc), Rlu = rgt(lu);
+    const QN<N> Llc = lft(lc), Llu = lft(lu), Rlc = rgt(lc), Rlu = rgt(lu);
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < N; k++) {
         const int i = L.i0 + k;
         if (i >= 1 && i <= nx - 2) {
             if (bit(LE, k) && bit(Lnl, k)) limiter(s.v[k], ocp.v[k], ou.v[k], Rlc.v[k], Rlu.v[k], true, p);
@@ -199,38 +242,39 @@ __device__ __forceinline__ void x_row_general(const XLane& L, const Params& p, i
 // uniform) is false for rows the sweep leaves untouched; they are returned
 // unchanged.  rr = (1/a(j), a(j)); edges: [6][ny] west ub, vb, g*eta, east
 // ub, vb, g*eta (or null for open edges at rest).
+template <int N = 4>
 __device__ __forceinline__ void x_row(const XLane& L, const Params& p, int nx, int ny, int j,
-                                      bool sweep, float2 rr, const float* edges, const Q4& s,
-                                      const Q4& cp, const Q4& u, const Q4& v, Q4& ocp, Q4& ou,
-                                      Q4& ov) {
+                                      bool sweep, float2 rr, const float* edges, const QN<N>& s,
+                                      const QN<N>& cp, const QN<N>& u, const QN<N>& v, QN<N>& ocp, QN<N>& ou,
+                                      QN<N>& ov) {
     ocp = cp;
     ou = u;
     ov = v;
     if (!sweep) return;
-    Q4 c, ge;
+    QN<N> c, ge;
     int land = 0, deep = 0;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < N; k++) {
         c.v[k] = c0_of(s.v[k]) + cp.v[k];
         ge.v[k] = geta_of(s.v[k], cp.v[k]);
         land |= (!bit(L.vmask, k) || is_land(c.v[k], p)) ? (1 << k) : 0;
         deep |= (bit(L.vmask, k) && is_deep(s.v[k], p)) ? (1 << k) : 0;
     }
-    if (wave_all(land == 0 && deep == 0xF)) {
+    if (wave_all(land == 0 && deep == (1 << N) - 1)) {
         // ---- fast path: no land, all deep, no line end in the window
-        const Q4 Rs = rgt(s), Rcp = rgt(cp), Ru = rgt(u), Rv = rgt(v), Rc = rgt(c), Rg = rgt(ge);
-        Q4 Ep, Eq;
+        const QN<N> Rs = rgt(s), Rcp = rgt(cp), Ru = rgt(u), Rv = rgt(v), Rc = rgt(c), Rg = rgt(ge);
+        QN<N> Ep, Eq;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < N; k++) {
             const EdgeRes e = edge_res(Node{s.v[k], cp.v[k], u.v[k], v.v[k]}, c.v[k], ge.v[k],
                                        Node{Rs.v[k], Rcp.v[k], Ru.v[k], Rv.v[k]}, Rc.v[k], Rg.v[k]);
             Ep.v[k] = e.ep;
             Eq.v[k] = e.eq;
         }
-        const Q4 LEp = lft(Ep), LEq = lft(Eq);
-        const Q4 Ls = lft(s), Lcp = lft(cp), Lu = lft(u), Lv = lft(v), Lc = lft(c);
+        const QN<N> LEp = lft(Ep), LEq = lft(Eq);
+        const QN<N> Ls = lft(s), Lcp = lft(cp), Lu = lft(u), Lv = lft(v), Lc = lft(c);
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < N; k++) {
             Metric m;
             m.rfm = L.mt[k].x * rr.x;
             m.rfp = L.mt[k].y * rr.x;
@@ -245,14 +289,15 @@ __device__ __forceinline__ void x_row(const XLane& L, const Params& p, int nx, i
         }
         return;
     }
-    x_row_general(L, p, nx, ny, j, rr, edges, s, cp, u, v, c, ge, land, deep, ocp, ou, ov);
+    x_row_general<N>(L, p, nx, ny, j, rr, edges, s, cp, u, v, c, ge, land, deep, ocp, ou, ov);
 }
 
 // Bits k of the lane's nodes that are wet in (s, cp) and selected by mask.
-__device__ __forceinline__ int wet_bits(const Q4& s, const Q4& cp, int mask, const Params& p) {
+template <int N = 4>
+__device__ __forceinline__ int wet_bits(const QN<N>& s, const QN<N>& cp, int mask, const Params& p) {
     int wet = 0;
 #pragma unroll
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < N; k++)
         wet |= (bit(mask, k) && !is_land(c0_of(s.v[k]) + cp.v[k], p)) ? (1 << k) : 0;
     return wet;
 }

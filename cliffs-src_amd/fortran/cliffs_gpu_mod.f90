@@ -32,7 +32,7 @@ module cliffs_gpu_mod
   public :: cliffs_stream, cliffs_info, cliffs_last_error, cliffs_health
   public :: cliffs_nest_maps, cliffs_nest_create, cliffs_nest_destroy, cliffs_nest_feed
   public :: cliffs_nest_couple, cliffs_nest_record, cliffs_nest_end_feed, cliffs_nest_child_step
-  public :: cliffs_nest_child_start, cliffs_run_gauges_noalloc
+  public :: cliffs_nest_child_start, cliffs_run_gauges_noalloc, cliffs_set_tile_width, cliffs_ensemble_shape
   public :: cliffs_run_gauges, cliffs_set_fused, cliffs_set_tile_rows
   public :: cliffs_ensemble_create, cliffs_ensemble_destroy, cliffs_ensemble_run
   public :: cliffs_ensemble_run_timed, cliffs_ensemble_tune, cliffs_ensemble_sync
@@ -182,6 +182,16 @@ module cliffs_gpu_mod
       import :: c_int, c_ptr
       type(c_ptr), value :: h
       integer(c_int), value :: rows
+    end function
+    integer(c_int) function cliffs_set_tile_width(h, width) bind(C, name="cliffs_set_tile_width")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: width
+    end function
+    integer(c_int) function cliffs_ensemble_shape(e, width, rows) bind(C, name="cliffs_ensemble_shape")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), intent(out) :: width, rows
     end function
     integer(c_int) function cliffs_nest_couple(nest, child, capacity) bind(C, name="cliffs_nest_couple")
       import :: c_int, c_ptr

@@ -144,6 +144,12 @@ int cliffs_set_fused(cliffs_handle *h, int on);
  * CLIFFS_ERR_STATE for a member of an ensemble (its tuner owns the tile
  * height).  No counterpart in the reference. */
 int cliffs_set_tile_rows(cliffs_handle *h, int rows);
+/* Tile width of the fused step: 248 columns (a 256-node x window, four
+ * nodes per lane; the default) or 120 (a 128-node window, two per lane),
+ * with the automatic tile height.  Results do not depend on it.  Fails with
+ * CLIFFS_ERR_STATE for a member of an ensemble.  No counterpart in the
+ * reference. */
+int cliffs_set_tile_width(cliffs_handle *h, int width);
 
 /* ---- ensembles: many grids stepped together ---------------------------- */
 
@@ -164,10 +170,15 @@ int cliffs_ensemble_run(cliffs_ensemble *e, int nsteps);
 /* As cliffs_ensemble_run; ms_total = device time of the nsteps. */
 int cliffs_ensemble_run_timed(cliffs_ensemble *e, int nsteps, double *ms_total);
 /* Autotuner: times trial_steps repetitions of one ensemble step for each
- * candidate tile height (16 .. 256 rows of 248 columns) and keeps the
- * fastest (rows_out).  The trials change neither the state nor the running
- * maxima (their max-wave update is switched off). */
+ * candidate tile shape (widths 248 and 120 columns, heights 16 .. 256 rows)
+ * and keeps the fastest (its height in rows_out, both in
+ * cliffs_ensemble_shape).  The trials change neither the state nor the
+ * running maxima (their max-wave update is switched off). */
 int cliffs_ensemble_tune(cliffs_ensemble *e, int trial_steps, int *rows_out);
+/* The tile shape the ensemble steps with (after cliffs_ensemble_tune: the
+ * fastest; rows 0 = each member's automatic height).  Members must share one
+ * tile width when the ensemble is created. */
+int cliffs_ensemble_shape(cliffs_ensemble *e, int *width, int *rows);
 int cliffs_ensemble_sync(cliffs_ensemble *e);
 /* nsamples x `every` steps of every member (`every` even); after each
  * `every` steps the surface elevation at each member's n gauges (1-based

@@ -58,6 +58,7 @@ def test_ensemble_matches_members_alone(gpu_lib, tune):
     ens = gpu_lib.Ensemble(grids)
     if tune:
         rows = ens.tune(2)
+        assert ens.shape()[0] in (248, 120) and ens.shape()[1] == rows
         assert 16 <= rows <= 256
     ens.run(20)
     ens.run(steps - 20)              # odd remainder: one step outside the graph
@@ -203,6 +204,33 @@ def test_ensemble_runup_parity(gpu_lib):
         assert abs(ru - ref["max_runup_m"]) <= synth.land_cell_rise(c), rows[-1]
         assert abs(fl - ref["flooded_land_cells"]) <= max(1, 0.01 * ref["flooded_land_cells"]), rows[-1]
     print("seed, run-up device / oracle (m), flooded device / oracle:", rows)
+    ens.close()
+    for g in grids:
+        g.close()
+
+
+def test_ensemble_narrow_tiles(gpu_lib):
+    """An ensemble whose members step with 120-column tiles: bit-identical to
+    the members stepped alone; members of different tile widths are refused."""
+    cases = _cases()[:3]
+    want = []
+    for case in cases:
+        g = _grid(gpu_lib, case)
+        g.run(9)
+        want.append(_state(g))
+        g.close()
+    grids = [_grid(gpu_lib, c) for c in cases]
+    grids[0].set_tile_width(120)
+    with pytest.raises(gpu_lib.CliffsError):
+        gpu_lib.Ensemble(grids)
+    for g in grids[1:]:
+        g.set_tile_width(120)
+    ens = gpu_lib.Ensemble(grids)
+    assert ens.shape()[0] == 120
+    ens.run(9)
+    ens.sync()
+    for k, (g, w) in enumerate(zip(grids, want)):
+        _same(_state(g), w, k)
     ens.close()
     for g in grids:
         g.close()

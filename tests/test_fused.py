@@ -252,3 +252,46 @@ def test_smallest_2d_grid(gpu_lib):
     case.cel = np.full((3, 3), 10.0)
     case.xvel = np.array([[0.0, 0.1, 0.0], [0.2, 0.3, -0.1], [0.0, 0.0, 0.1]])
     _same(_run(gpu_lib, case, True, 12, chunk=5), _run(gpu_lib, case, False, 12, chunk=5))
+
+
+def _narrow_cases():
+    nc = synth.coastal(nx=280, ny=200, seed=7)
+    cm = np.sqrt(synth.GRAV * nc.ground)
+    dry = nc.cel <= cm
+    rng = np.random.default_rng(1)
+    nc.cel = np.where(dry, cm * rng.uniform(0.2, 0.9, dry.shape), nc.cel)
+    nc.xvel = np.where(dry, rng.uniform(-0.5, 0.5, dry.shape), nc.xvel)
+    return [("coastal", synth.coastal(nx=300, ny=240, seed=6), 120, 30),
+            ("coastal_T", _transposed(synth.coastal(nx=240, ny=300, seed=5)), 120, 30),
+            ("plane_beach", _plane_beach(), 400, 50),
+            ("noncanonical", nc, 50, 1),
+            ("friction", synth.coastal(nx=250, ny=190, seed=4, cartesian=False, crough=0.0025), 150, 25),
+            ("thin", synth.open_ocean(n=121, seed=3, ny=33), 60, 20)]
+
+
+@pytest.mark.parametrize("name,case,nsteps,chunk", _narrow_cases(), ids=[c[0] for c in _narrow_cases()])
+def test_narrow_tiles(gpu_lib, name, case, nsteps, chunk):
+    """120-column tiles (a 128-node x window, two nodes per lane, six-wave
+    blocks) against the two-kernel step: bit-identical, on the branches the
+    248-column cases above cover."""
+    g = _grid(gpu_lib, case, True)
+    g.set_tile_width(120)
+    done = 0
+    while done < nsteps:
+        n = min(chunk, nsteps - done)
+        g.run(n)
+        done += n
+    a = _result(g)
+    g.close()
+    _same(a, _run(gpu_lib, case, False, nsteps, chunk))
+
+
+def test_tile_width_rejects_bad_values(gpu_lib):
+    case = synth.open_ocean(n=64, seed=1)
+    g = _grid(gpu_lib, case, True)
+    for w in (0, 100, 256):
+        with pytest.raises(gpu_lib.CliffsError):
+            g.set_tile_width(w)
+    g.set_tile_width(120)
+    g.set_tile_width(248)
+    g.close()
