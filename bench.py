@@ -55,22 +55,26 @@ def pmc_traffic(kernel, workload):
     return None
 
 
-def ensemble_bench(nscen, steps, seed0):
+def ensemble_bench(nscen, steps, seed0, workers=8):
     """Batched ensemble (config 4 at a sample size): nscen scenarios of
     uneven size (512..2048 per side, dx 200-1000 m, shelf and coastline,
     dipole sources), all advanced `steps` steps in one fused launch per step
     replayed from a HIP graph, tile height picked by the autotuner.
     scenarios_per_hr counts the wall time from device allocation and upload
-    to the per-scenario maxima on the host (inputs generated beforehand)."""
+    to the per-scenario maxima on the host (inputs generated beforehand, by
+    `workers` processes, or in this process when 0)."""
     import cliffs_gpu
     import synth
     log(f"[bench] ensemble: generating {nscen} scenarios")
     import functools
     import multiprocessing as mp
-    # fresh interpreters (spawn): the workers run numpy only and never touch the GPU
-    with mp.get_context("spawn").Pool(min(8, os.cpu_count() or 1)) as pool:
-        cases = pool.map(functools.partial(synth.ensemble_scenario, steps=steps),
-                         range(seed0, seed0 + nscen), chunksize=4)
+    gen = functools.partial(synth.ensemble_scenario, steps=steps)
+    if workers > 0:
+        # fresh interpreters (spawn): the workers run numpy only and never touch the GPU
+        with mp.get_context("spawn").Pool(min(workers, os.cpu_count() or 1)) as pool:
+            cases = pool.map(gen, range(seed0, seed0 + nscen), chunksize=4)
+    else:
+        cases = [gen(s) for s in range(seed0, seed0 + nscen)]
     cells = sum(c.dep.size for c in cases)
     cliffs_gpu.load_library()
 
@@ -93,7 +97,8 @@ def ensemble_bench(nscen, steps, seed0):
         grids = list(ex.map(make, cases))
     ens = cliffs_gpu.Ensemble(grids)
     rows = ens.tune(2)
-    log(f"[bench] ensemble: {cells / 1e6:.1f} M cells, tile height {rows}, {steps} steps")
+    width = ens.shape()[0]
+    log(f"[bench] ensemble: {cells / 1e6:.1f} M cells, tiles {width} x {rows}, {steps} steps")
     ms = ens.run_timed(steps)
     with ThreadPoolExecutor(max_workers=8) as ex:
         runups = list(ex.map(runup, zip(grids, cases)))
@@ -102,7 +107,7 @@ def ensemble_bench(nscen, steps, seed0):
     for g in grids:
         g.close()
     parity = runup_parity(seed0, nscen, steps, runups, cases)
-    return {"scenarios": nscen, "steps": steps, "cells": cells, "tile_rows": rows,
+    return {"scenarios": nscen, "steps": steps, "cells": cells, "tile_width": width, "tile_rows": rows,
             "wall_s": round(wall, 3), "device_ms": round(ms, 1),
             "scenarios_per_hr": round(nscen / (wall / 3600.0), 1),
             "gcell_updates_per_s": round(cells * steps / (ms * 1e-3) / 1e9, 2),
@@ -346,16 +351,21 @@ def main():
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=4096)
-    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=None,
+                    help="seed of the grid (open ocean: 0, coastal: 3) and the first ensemble "
+                         "scenario (0)")
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ensemble-scenarios", type=int, default=256,
                     help="scenarios of the batched-ensemble measurement (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=4000)
+    ap.add_argument("--ensemble-workers", type=int, default=8,
+                    help="processes generating the ensemble's inputs (0: in this process, "
+                         "as for a profiled run)")
     ap.add_argument("--coastal", action="store_true",
                     help="instead of the open ocean: the spherical 8192 x 6144 coastal grid "
-                         "(30 arc-seconds, shelf, coastline, run-up), whose tiles take the "
-                         "wet/dry paths")
+                         "of the nested workload (30 arc-seconds, shelf, coastline, run-up), "
+                         "whose tiles take the wet/dry paths")
     ap.add_argument("--nested-hours", type=float, default=6.0)
     ap.add_argument("--nested", action="store_true",
                     help="the full-size nested workload instead: the 8192 x 6144 coastal "
@@ -415,13 +425,15 @@ def main():
 
     n = args.n
     if args.coastal:
-        log(f"[bench] generating the 8192x6144 spherical coastal case, seed {args.seed}")
-        case = synth.coastal(nx=8192, ny=6144, seed=args.seed, cartesian=False, lat0=-25.0,
-                             steps=args.steps)
-        workload = (f"2D spherical coastal grid 8192x6144 at 30 arc-seconds from 25 S, shelf "
-                    f"and seeded coastline with land to +300 m, uplift/subsidence dipole 3 m, "
-                    f"run-up, seed {args.seed}")
+        seed = 3 if args.seed is None else args.seed
+        log(f"[bench] generating the 8192x6144 spherical coastal parent, seed {seed}")
+        case = synth.coastal_nest(seed=seed)[0]
+        workload = (f"2D spherical coastal grid 8192x6144 at 30 arc-seconds from 25 S (the nested "
+                    f"workload's parent alone): 100 km shelf at 200 m, land to +300 m, SSL "
+                    f"conditioning, 200x100 km dipole +3/-1.5 m, run-up with minimal flow depth "
+                    f"{case.ground:g} m and Manning n^2 {case.crough:g}, seed {seed}")
     else:
+        args.seed = 0 if args.seed is None else args.seed
         log(f"[bench] generating {n}x{n} open-ocean case, seed {args.seed}")
         case = synth.open_ocean(n=n, seed=args.seed, steps=args.steps)
         workload = (f"2D Cartesian open ocean {n}x{n}, dx=dy=500 m, depth 4000 m minus fractal "
@@ -447,7 +459,7 @@ def main():
     value = cells * args.steps / wall / 1e9
     log(f"[bench] {wall:.3f} s wall, {ms_dev:.1f} ms device, {value:.2f} Gcell-updates/s")
     c, u, v = g.download()
-    finite = bool(np.isfinite(c).all() and np.isfinite(u).all())
+    finite = bool(np.isfinite(c).all() and np.isfinite(u).all() and np.isfinite(v).all())
     g.close()
 
     traffic_key = "coastal_8192x6144" if args.coastal else f"open_ocean_{n}"
@@ -456,7 +468,8 @@ def main():
     roof = roofline(nx * ny, value, ms_x, ms_y, args.unfused, traffic_key, copy_gbs)
     ensemble = None
     if args.ensemble_scenarios > 0:
-        ensemble = ensemble_bench(args.ensemble_scenarios, args.ensemble_steps, args.seed)
+        ensemble = ensemble_bench(args.ensemble_scenarios, args.ensemble_steps,
+                                  0 if args.seed is None else args.seed, args.ensemble_workers)
         log(f"[bench] ensemble: {ensemble['scenarios_per_hr']} scenarios/hr, "
             f"{ensemble['gcell_updates_per_s']} Gcell-updates/s")
     cpu = None

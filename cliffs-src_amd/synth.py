@@ -138,11 +138,17 @@ def beach_1d(seed=0, n=2000, dx=0.25, steps=20000, amplitude=None):
 
 
 def coastal(nx=256, ny=192, seed=0, cartesian=True, dx=500.0, steps=200, source_amp=3.0,
-            crough=0.0, dlon=None, lat0=38.0):
+            crough=0.0, dlon=None, lat0=38.0, ground=0.01):
     """Shelf + seeded coastline with land up to +300 m; elliptical uplift/
     subsidence dipole (+A, -A/2) offshore.  Land is on the high-x side.
     Spherical grids have spacing dlon (30 arc-seconds by default) and start
-    at longitude 140, latitude lat0."""
+    at longitude 140, latitude lat0.  `ground` is the minimal flow depth and
+    `crough` the Manning n^2 (cliffs.f:204).  The shelf edge is a step from
+    4-6 km to 200 m with no SSL conditioning: a stress input for parity
+    tests at small sizes.  At 8192 x 6144 with the 3 m dipole the flow over
+    the step grows past 180 m/s and the state goes non-finite after ~415
+    steps, in the FP64 oracle as on the device; the benchmark workloads use
+    SSL-conditioned terrain (shelf_coast, coastal_nest)."""
     rng = np.random.default_rng(seed)
     if cartesian:
         x = np.arange(nx, dtype=np.float64) * dx
@@ -180,7 +186,6 @@ def coastal(nx=256, ny=192, seed=0, cartesian=True, dx=500.0, steps=200, source_
     yr = -(X - sx) * np.sin(th) + (Y - sy) * np.cos(th)
     q = source_amp * np.exp(-((xr - 0.5 * a) / a) ** 2 - (yr / b) ** 2) \
         - 0.5 * source_amp * np.exp(-((xr + 0.5 * a) / a) ** 2 - (yr / b) ** 2)
-    ground = 0.01
     H = dep + q
     cel = depth_to_celerity(H, ground)
     s1, s2, _ = metrics(x, y, cartesian)
