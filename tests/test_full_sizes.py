@@ -154,7 +154,7 @@ def test_c3_nest_full_size(gpu_lib):
 
 def test_c3_parent_window_parity(gpu_lib):
     """A window of the nested workload's parent (spherical, 30 arc-seconds,
-    its shelf, coast and dipole source, 560 x 520 cells from the source to
+    its shelf, coast and dipole source, about 600 x 520 cells from the source to
     the land behind the coast) stepped through the wave's arrival and run-up
     on the device and by the FP64 oracle: first-step wet/dry masks equal,
     surface elevation within 1e-5 of the source amplitude on cells wet in
