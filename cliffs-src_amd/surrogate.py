@@ -2,7 +2,8 @@
 
 The network (hand-written BF16 MFMA kernels, csrc/surrogate.hip) maps the
 surface-elevation series of 64 gauges on a nested grid's offshore boundary
-(256 samples) to the nested grid's 64 x 64 max-inundation map.  This module
+(256 samples) to the nested grid's 64 x 64 max-inundation map (per pixel the
+highest ground the water reached, so the map's maximum is the run-up).  This module
 defines the nested site, generates data with the solver (parents stepped as
 an ensemble with device gauge recording for the inputs, the coupled
 parent/child nested solve for the targets), permutes maps to and from the
@@ -176,10 +177,13 @@ def gemm_bf16(A, B, device=0):
 # One site: a parent grid (192 x 192, 900 m, shelf and coastline of
 # synth.shelf_coast with terrain seed SITE_SEED) and a child grid at 5x
 # refinement (180 m, dt/5) over the coastal strip, fed one-way by the parent
-# (cliffs_nest_*).  Scenarios differ only in the source.  The network's input
-# is the parent's surface elevation at 64 gauges along the child's offshore
-# (west) edge, every EVERY parent steps; its output is the child's
-# max-inundation map over the coastal window.
+# (cliffs_nest_*).  Minimal flow depth and friction as the reference sets
+# them for its 30" and 10" Okushiri grids (paramsOK30s.txt, paramsOK10s.txt:
+# 1 m and 0.5 m, Manning n^2 0.000625), at which run-up is a well-posed
+# function of the source.  Scenarios differ only in the source.  The
+# network's input is the parent's surface elevation at 64 gauges along the
+# child's offshore (west) edge, every EVERY parent steps; its output is the
+# child's max-inundation map over the coastal window.
 
 SITE_SEED = 5
 PNX = PNY = 192
@@ -192,8 +196,13 @@ CUKE = 1.0e-3                # arrival threshold of the feed (m)
 MAP_I0, MAP_W = 56, 128      # map window: child columns MAP_I0.. (2 per pixel),
 MAP_J0, MAP_H = 3, 640        # rows MAP_J0.. (10 per pixel), clear of the child's edges
 
+P_GROUND = synth.ENSEMBLE_GROUND   # minimal flow depth of the parent (m)
+C_GROUND = 0.5                     # and of the child
+CROUGH = synth.ENSEMBLE_CROUGH     # Manning n^2 of both
+
 X_SCALE = 2.0     # gauge elevation (m) -> network input
-Y_SCALE = 0.125   # max surface elevation (m) -> network output
+Y_SCALE = 0.125   # inundation height (m) -> network output
+RUNUP_MIN = 1.0   # run-ups below this (m) are compared in absolute terms
 
 _site = None
 
@@ -202,7 +211,8 @@ class Site:
     """Terrain, child grid and nest maps of the site (built once)."""
 
     def __init__(self):
-        base = synth.shelf_coast(PNX, PNY, SITE_SEED, PDX, 4.0, 10e3)
+        base = synth.shelf_coast(PNX, PNY, SITE_SEED, PDX, 4.0, 10e3, ground=P_GROUND,
+                                 crough=CROUGH)
         self.dt = base.dt                           # CFL step for the largest source amplitude
         self.dep, self.ground = base.dep, base.ground
         self.x, self.y = base.xcrd, base.ycrd
@@ -216,10 +226,13 @@ class Site:
         # gauges: parent cells on the child's west edge (1-based)
         self.gI = np.full(G, CI0 + 1, dtype=np.int32)
         self.gJ = np.round(np.linspace(CJ0 + 1, CJ1 + 1, G)).astype(np.int32)
-        # land ground heights per map pixel, sorted, -inf for sea cells
-        w = _window(np.where(self.depc < 0, -self.depc, -np.inf))
-        self.pix_ground = np.sort(w.reshape(MAP, MAP_H // MAP, MAP, MAP_W // MAP)
-                                  .transpose(0, 2, 1, 3).reshape(NOUT, -1), axis=1)
+        # ground rise from one land cell to the next across the coast (median
+        # over the window): the step in which the solver's run-up moves
+        wg = _window(np.where(self.depc < 0, -self.depc, np.nan))
+        self.land_step = float(np.nanmedian(np.abs(np.diff(wg, axis=1))))
+        # lowest land ground height per map pixel (inf where the pixel is all sea)
+        w = _window(np.where(self.depc < 0, -self.depc, np.inf))
+        self.pix_ground_min = w.reshape(MAP, MAP_H // MAP, MAP, MAP_W // MAP).min(axis=(1, 3))
 
 
 def site():
@@ -254,33 +267,33 @@ def scenario(seed):
     rng = np.random.default_rng(50_000 + seed)
     return synth.shelf_coast(PNX, PNY, SITE_SEED, PDX, float(rng.uniform(0.5, 4.0)),
                              float(rng.uniform(8e3, 30e3)), steps=EVERY * T,
-                             source_seed=60_000 + seed, source_x=(0.3, 0.5), dt=S.dt)
+                             source_seed=60_000 + seed, source_x=(0.3, 0.5), dt=S.dt,
+                             ground=P_GROUND, crough=CROUGH)
 
 
 def _parent(case, device):
     s1, s2, zeta = case.metrics()
-    g = cliffs_gpu.Grid(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground, device=device)
+    g = cliffs_gpu.Grid(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground,
+                        crough=case.crough, device=device)
     g.upload(case.cel, case.xvel, case.yvel)
     return g
 
 
 def inundation_map(hmax, depc, celmin):
-    """Max water-surface elevation (MaxE = hmax^2/g - dep, output_maxwave.f)
-    where the cell was ever wet, 0 elsewhere; block maximum over 10 x 2 child
-    cells of the coastal window -> 64 x 64."""
-    e = np.where(hmax > celmin, hmax * hmax / synth.GRAV - depc, 0.0)
-    e = np.maximum(_window(e), 0.0)
+    """Inundation height: the ground height (-dep) of land cells the water
+    reached (celerity maximum above celmin, as synth.max_runup counts them),
+    0 elsewhere; block maximum over 10 x 2 child cells of the coastal window
+    -> 64 x 64.  The map's maximum is the run-up over the window."""
+    e = np.where((hmax > celmin) & (depc < 0.0), -depc, 0.0)
+    e = _window(e)
     return e.reshape(MAP, MAP_H // MAP, MAP, MAP_W // MAP).max(axis=(1, 3))
 
 
 def runup_from_map(maps):
-    """Run-up read from max-surface maps: per pixel, the highest land cell
-    below the pixel's max surface; the maximum over pixels (0: none)."""
-    S = site()
+    """Run-up read from inundation-height maps: the maximum over pixels
+    (exact on the solver's own map)."""
     m = np.asarray(maps, dtype=np.float64).reshape(-1, NOUT)
-    g = S.pix_ground[None]
-    reach = np.where(g < m[:, :, None], g, -np.inf).max(axis=(1, 2))
-    return np.maximum(reach, 0.0)
+    return np.maximum(m.max(axis=1), 0.0)
 
 
 def _tuned_rows(grid):
@@ -298,10 +311,11 @@ def nested_hmax(case, device=0):
     heights tuned once per site.  -> (child hmax, child celmin)."""
     S = site()
     g = _parent(case, device)
-    c = cliffs_gpu.Grid(S.depc, *S.cmetrics, dt=case.dt / REFINE, ground=case.ground, device=device)
+    c = cliffs_gpu.Grid(S.depc, *S.cmetrics, dt=case.dt / REFINE, ground=C_GROUND,
+                        crough=case.crough, device=device)
     src = case.meta["source"]
     z = np.zeros_like(S.depc)
-    c.upload(synth.depth_to_celerity(S.depc + synth.dipole(S.Xc, S.Yc, S.depc, src), case.ground), z, z)
+    c.upload(synth.depth_to_celerity(S.depc + synth.dipole(S.Xc, S.Yc, S.depc, src), C_GROUND), z, z)
     if S.rows is None:
         S.rows = (_tuned_rows(g), _tuned_rows(c))
     g.set_tile_rows(S.rows[0])
@@ -373,29 +387,33 @@ def predict_maps(net, x):
 
 def answer(cases, nets, tol, device=0):
     """Max-inundation maps of `cases` at the site: the parents stepped as one
-    ensemble give the gauge series; where the two networks' maps agree
-    within tol (gate) their mean is the answer, elsewhere the full nested
-    solve runs.  -> (maps (n, 64, 64), answered by the surrogate (n,))."""
+    ensemble give the gauge series; where the networks' run-ups agree within
+    tol (gate) the mean of their maps is the answer, elsewhere the full
+    nested solve runs.  -> (maps (n, 64, 64), answered by the surrogate (n,))."""
     x = parent_gauges(cases, device) * X_SCALE
-    pa, pb = predict_maps(nets[0], x), predict_maps(nets[1], x)
-    ok = gate(pa, pb, tol)
-    maps = (0.5 * (pa + pb)).astype(np.float64)
+    preds = [predict_maps(n, x) for n in nets]
+    ok = gate(*preds, tol=tol)
+    maps = np.mean(preds, axis=0).astype(np.float64)
     for k in np.flatnonzero(~ok):
         maps[k] = nested_solve(cases[k], device)[0]
     return maps, ok
 
 
-def flooded_area(maps, thr=0.05):
-    """Pixels whose max surface stands above their lowest land cell."""
+def flooded_area(maps):
+    """Pixels with flooded land: inundation height at or above the pixel's
+    lowest land cell."""
     S = site()
-    gmin = S.pix_ground[:, 0].reshape(MAP, MAP)
-    return ((np.asarray(maps) > thr) & (np.asarray(maps) > gmin)).sum(axis=(-2, -1))
+    return (np.asarray(maps) >= S.pix_ground_min).sum(axis=(-2, -1))
 
 
-def gate(maps_a, maps_b, tol):
-    """Confidence gate: the two networks' maps agree within tol (relative L2
-    of their difference to their mean)."""
-    a, b = np.asarray(maps_a), np.asarray(maps_b)
-    num = np.sqrt(((a - b) ** 2).sum(axis=(-2, -1)))
-    den = np.sqrt((((a + b) * 0.5) ** 2).sum(axis=(-2, -1))) + 1e-3
-    return num / den <= tol
+def gate_score(*maps):
+    """Disagreement of networks' run-ups (map maxima): their spread (largest
+    minus smallest) relative to their mean, or to RUNUP_MIN where the mean
+    is smaller."""
+    r = np.stack([runup_from_map(m) for m in maps])
+    return (r.max(axis=0) - r.min(axis=0)) / np.maximum(r.mean(axis=0), RUNUP_MIN)
+
+
+def gate(*maps, tol):
+    """Confidence gate: the networks' run-ups agree within tol."""
+    return gate_score(*maps) <= tol
