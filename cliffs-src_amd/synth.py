@@ -303,7 +303,7 @@ def ssl_condition(dep, depmin=1.0, alpha=2.0, passes=50):
 
 
 def shelf_coast(nx, ny, seed, dx, source_amp, source_len, steps=4000, cfl=0.7, source_seed=None,
-                source_x=(0.35, 0.55), dt=None, ground=0.01, crough=0.0):
+                source_x=(0.35, 0.55), dt=None, ground=0.01, crough=0.0, land=None):
     """Cartesian shelf and coastline for ensemble scenarios: fractal abyssal
     plain 4-6 km, a continental slope and a shelf whose sqrt-depth varies
     linearly (200 m shelf, 2 m at the shore), land rising to +300 m behind a
@@ -312,7 +312,9 @@ def shelf_coast(nx, ny, seed, dx, source_amp, source_len, steps=4000, cfl=0.7, s
     source_x of the domain width).  The terrain follows `seed`; the source
     follows it too, or `source_seed` when given (one site, many sources).
     `dt` overrides the CFL time step; `ground` is the minimal flow depth and
-    `crough` the Manning n^2 (cliffs.f:204)."""
+    `crough` the Manning n^2 (cliffs.f:204); `land`, when given, maps the
+    distance inland (m) to the ground height (m) instead of the uniform rise
+    to 300 m."""
     rng = np.random.default_rng(seed)
     x = np.arange(nx, dtype=np.float64) * dx
     y = np.arange(ny, dtype=np.float64) * dx
@@ -328,8 +330,11 @@ def shelf_coast(nx, ny, seed, dx, source_amp, source_len, steps=4000, cfl=0.7, s
     t_shelf = np.clip(dist / shelf_w, 0.0, 1.0)                 # 0 at the coast
     q = np.where(dist > shelf_w, qs + (qa - qs) * t_slope, qc + (qs - qc) * np.sqrt(t_shelf))
     dep = q * q * (1.0 + 0.05 * fractal_field(ny, nx, rng) * (dist > 0))
-    land = dist <= 0
-    dep = np.where(land, dist / (0.25 * Lx) * 300.0 - 0.5, dep)
+    on_land = dist <= 0
+    if land is None:
+        dep = np.where(on_land, dist / (0.25 * Lx) * 300.0 - 0.5, dep)
+    else:
+        dep = np.where(on_land, -land(np.maximum(-dist, 0.0)) - 0.5, dep)
     dep = ssl_condition(dep)
     rs = rng if source_seed is None else np.random.default_rng(source_seed)
     src = dict(amplitude=source_amp, sx=rs.uniform(*source_x) * Lx, sy=rs.uniform(0.3, 0.7) * Ly,
