@@ -196,6 +196,14 @@ CUKE = 1.0e-3                # arrival threshold of the feed (m)
 MAP_I0, MAP_W = 56, 128      # map window: child columns MAP_I0.. (2 per pixel),
 MAP_J0, MAP_H = 3, 640        # rows MAP_J0.. (10 per pixel), clear of the child's edges
 
+def plain(d):
+    """Ground height (m) at distance d (m) inland: a coastal plain rising
+    0.1 m per km for 25 km, then 10 m per km.  On the child's 180 m cells
+    the plain rises 1.8 cm per cell, so the solver's run-up (the ground
+    height of the highest flooded cell) moves in steps of ~1 % of a metre."""
+    return np.where(d < 25e3, 1e-4 * d, 2.5 + 1e-2 * (d - 25e3))
+
+
 P_GROUND = synth.ENSEMBLE_GROUND   # minimal flow depth of the parent (m)
 C_GROUND = 0.5                     # and of the child
 CROUGH = synth.ENSEMBLE_CROUGH     # Manning n^2 of both
@@ -212,7 +220,7 @@ class Site:
 
     def __init__(self):
         base = synth.shelf_coast(PNX, PNY, SITE_SEED, PDX, 4.0, 10e3, ground=P_GROUND,
-                                 crough=CROUGH)
+                                 crough=CROUGH, land=plain)
         self.dt = base.dt                           # CFL step for the largest source amplitude
         self.dep, self.ground = base.dep, base.ground
         self.x, self.y = base.xcrd, base.ycrd
@@ -268,7 +276,7 @@ def scenario(seed):
     return synth.shelf_coast(PNX, PNY, SITE_SEED, PDX, float(rng.uniform(0.5, 4.0)),
                              float(rng.uniform(8e3, 30e3)), steps=EVERY * T,
                              source_seed=60_000 + seed, source_x=(0.3, 0.5), dt=S.dt,
-                             ground=P_GROUND, crough=CROUGH)
+                             ground=P_GROUND, crough=CROUGH, land=plain)
 
 
 def _parent(case, device):
