@@ -102,6 +102,8 @@ def load_library():
         "cliffs_nest_end_feed": ([H], C.c_int),
         "cliffs_nest_child_step": ([H, C.c_double], C.c_int),
         "cliffs_nest_child_start": ([H, C.c_double, _dp], C.c_int),
+        "cliffs_nest_run": ([H, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double,
+                             C.c_double, C.c_double, _ip], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -438,6 +440,17 @@ class Nest:
 
     def child_step(self, t):
         _chk(load_library().cliffs_nest_child_step(self._h, float(t)))
+
+    def run(self, nrec, every, dt, per, dt_child, cuke, hmin, iarv=False, t0=0.0):
+        """`nrec` record intervals of the coupled solve in one call
+        (cliffs_nest_run): per interval `every` parent steps of dt, a record,
+        and the child's steps of dt_child (`per` per interval) as far as the
+        records allow.  -> arrival latch."""
+        a = C.c_int(1 if iarv else 0)
+        _chk(load_library().cliffs_nest_run(self._h, int(nrec), int(every), float(t0), float(dt),
+                                            int(per), float(dt_child), float(cuke), float(hmin),
+                                            C.byref(a)))
+        return bool(a.value)
 
 
 NAN32 = np.float32(-1.0e32)   # the reference's real*4 NAN (global_modules.f:18)

@@ -195,6 +195,8 @@ struct cliffs_nest {
     bool start_set = false;            // threshold given by cliffs_nest_child_start
     int start = 0;                     // 1-based start record, 0 while unknown
     int scanned = 0;                   // records found below the threshold
+    long run_k = 0;                    // cliffs_nest_run: record intervals run so far
+    long run_kc = 0;                   // and child steps taken
     bool counted = false;              // the parent's use count includes this nest
     NestArgs args(double cuke) const;
 };
@@ -1677,6 +1679,8 @@ int cliffs_nest_couple(cliffs_nest* n, cliffs_handle* child, int capacity) {
     n->start_set = false;
     n->start = 0;
     n->scanned = 0;
+    n->run_k = 0;
+    n->run_kc = 0;
     return CLIFFS_OK;
 }
 
@@ -1816,6 +1820,38 @@ int cliffs_nest_child_step(cliffs_nest* n, double t) {
         HIPCHK(hipEventRecord(n->use_ev[(n->irec - 1) % n->cap], c->st));
     }
     return do_step(c, n->cedges, n->cedges + 6 * (size_t)n->nyc);
+}
+
+int cliffs_nest_run(cliffs_nest* n, int nrec, int every, double t0, double dt, int per,
+                    double dt_child, double cuke, double hmin, int* iarv) {
+    if (!n || !iarv || nrec < 0 || every < 1 || per < 1 || !(dt > 0.0) || !(dt_child > 0.0))
+        return fail(CLIFFS_ERR_ARG, "bad coupled-run arguments");
+    if (!n->child) return fail(CLIFFS_ERR_STATE, "nest not coupled to a child");
+    for (int q = 0; q < nrec; q++) {
+        int rc = cliffs_run(n->parent, every);
+        if (rc) return rc;
+        n->run_k++;
+        const double t = t0 + (double)(n->run_k * every) * dt;
+        int nr = 0;
+        if ((rc = cliffs_nest_record(n, cuke, hmin, t, iarv, &nr))) return rc;
+        if (nr < 2) continue;
+        // the child's clock starts at its start record (the first one
+        // without a threshold); until that is known the child waits
+        if (n->start == 0) {
+            rc = nest_find_start(n);
+            if (rc == CLIFFS_ERR_STATE) continue;
+            if (rc) return rc;
+        }
+        const long s = n->start - 1, have = (long)n->tinput.size() - s;
+        // step kc at tinput(s + j) + (kc - j per) dt_child, j = kc / per,
+        // while the records bracketing the next step's time exist
+        while (n->run_kc + 1 < (long)per * (have - 1)) {
+            const long kc = ++n->run_kc, j = kc / per;
+            if ((rc = cliffs_nest_child_step(n, n->tinput[s + j] + (double)(kc - j * per) * dt_child)))
+                return rc;
+        }
+    }
+    return CLIFFS_OK;
 }
 
 int cliffs_sync(cliffs_handle* h) {

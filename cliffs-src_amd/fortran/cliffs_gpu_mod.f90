@@ -33,6 +33,7 @@ module cliffs_gpu_mod
   public :: cliffs_nest_maps, cliffs_nest_create, cliffs_nest_destroy, cliffs_nest_feed
   public :: cliffs_nest_couple, cliffs_nest_record, cliffs_nest_end_feed, cliffs_nest_child_step
   public :: cliffs_nest_child_start, cliffs_run_gauges_noalloc, cliffs_set_tile_width, cliffs_ensemble_shape
+  public :: cliffs_nest_run
   public :: cliffs_run_gauges, cliffs_set_fused, cliffs_set_tile_rows
   public :: cliffs_ensemble_create, cliffs_ensemble_destroy, cliffs_ensemble_run
   public :: cliffs_ensemble_run_timed, cliffs_ensemble_tune, cliffs_ensemble_sync
@@ -220,6 +221,14 @@ module cliffs_gpu_mod
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: nest
       real(c_double), value :: t
+    end function
+    integer(c_int) function cliffs_nest_run(nest, nrec, every, t0, dt, per, dt_child, cuke, hmin, &
+                                            iarv) bind(C, name="cliffs_nest_run")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: nest
+      integer(c_int), value :: nrec, every, per
+      real(c_double), value :: t0, dt, dt_child, cuke, hmin
+      integer(c_int), intent(inout) :: iarv
     end function
     ! members: an array of n grid handles
     integer(c_int) function cliffs_ensemble_create(members, n, e) bind(C, name="cliffs_ensemble_create")

@@ -267,6 +267,19 @@ int cliffs_nest_child_start(cliffs_nest *n, double cuke, double *t0);
  * Fails with CLIFFS_ERR_STATE if t is beyond the parent's last record and
  * the feed has not ended. */
 int cliffs_nest_child_step(cliffs_nest *n, double t);
+/* A coupled run on one device: the loop a driver writes around cliffs_run,
+ * cliffs_nest_record and cliffs_nest_child_step (cliffs_main.f:288-365 with
+ * the child stepped alongside instead of from files), in one call.  For
+ * each of `nrec` record intervals the parent advances `every` steps of dt,
+ * a record is written at parent time t0 + k*every*dt (k = intervals this
+ * nest has run, from 1; arrival test with cuke and hmin as in
+ * cliffs_nest_record, latch *iarv), and the child advances, `per` steps of
+ * dt_child per record interval, as far as the records written so far allow:
+ * its step kc (from 1) at time tinput(s + kc/per) + (kc mod per)*dt_child,
+ * s its start record.  The counts carry over between calls, so a run may
+ * be split into any number of calls. */
+int cliffs_nest_run(cliffs_nest *n, int nrec, int every, double t0, double dt, int per,
+                    double dt_child, double cuke, double hmin, int *iarv);
 
 /* Health of the current state: *courant = the largest Courant number
  * (|u| + c) dt / dx over wet cells (x along rows where the x-sweep runs, y

@@ -17,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def test_library_exports_every_header_symbol():
     lib = cliffs_gpu.load_library()
     names = cliffs_gpu.header_symbols()
-    assert len(names) == 39
+    assert len(names) == 40
     for n in names:
         assert hasattr(lib, n), n
     out = subprocess.run(["nm", "-D", "--defined-only", cliffs_gpu.LIB_PATH], check=True,

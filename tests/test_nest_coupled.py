@@ -226,3 +226,40 @@ def test_child_start_below_threshold(gpu_lib):
     assert "ended below" in str(ei.value)
     for x in (n, g, c):
         x.close()
+
+
+def test_nest_run_matches_record_and_child_step_loop(gpu_lib):
+    """cliffs_nest_run (the coupled loop in one call, split here over
+    several calls) ends bit-identical to the same loop written around
+    cliffs_run, cliffs_nest_record and cliffs_nest_child_step."""
+    case, maps, parent, child = _setup(gpu_lib)
+    cuke, every, per, nrec = 0.01, 2, 6, 40
+    dtp, dtc = case.dt, case.dt / 3
+    gA, cA = parent(), child()
+    nA = gpu_lib.Nest(gA, maps)
+    nA.couple(cA, capacity=4)
+    iarv, trec, kc = False, [], 0
+    for k in range(nrec):
+        gA.run(every)
+        tp = (k + 1) * every * dtp
+        iarv, nr = nA.record(cuke, case.ground, tp, iarv)
+        if nr > len(trec):
+            trec.append(tp)
+        while kc + 1 < per * (len(trec) - 1):
+            kc += 1
+            j = kc // per
+            nA.child_step(trec[j] + (kc - j * per) * dtc)
+    gB, cB = parent(), child()
+    nB = gpu_lib.Nest(gB, maps)
+    nB.couple(cB, capacity=4)
+    iarvB = False
+    for chunk in (7, 1, 20, 12):
+        iarvB = nB.run(chunk, every, dtp, per, dtc, cuke, case.ground, iarvB)
+    assert iarvB == iarv and len(trec) > 10 and kc > 50
+    for a, b in ((gA, gB), (cA, cB)):
+        for fa, fb in zip(a.download() + a.download_max(), b.download() + b.download_max()):
+            assert np.array_equal(fa, fb)
+    for n in (nA, nB):
+        n.close()
+    for g in (cA, gA, cB, gB):
+        g.close()
