@@ -488,7 +488,9 @@ int set_fused_shape(cliffs_handle* h, int width, int rows) {
         h->fflags = nullptr;  // ensure_fused refuses to step until a geometry is set again
         return fail(CLIFFS_ERR_NOMEM, "flag allocation");
     }
-    HIPCHK(hipMemset(h->fflags, 0, fb));
+    // on the grid's own (non-blocking) stream: a null-stream clear would not
+    // be ordered before the flag kernels that follow it there
+    HIPCHK(hipMemsetAsync(h->fflags, 0, fb, h->st));
     return CLIFFS_OK;
 }
 
@@ -687,12 +689,17 @@ int cliffs_create(const cliffs_grid_desc* d, cliffs_handle** out) {
         if (rc) return bail(rc);
     }
 
-    if (hipMemcpy(h->xm, xm.data(), sizeof(float4) * nx, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(h->xr, xr.data(), sizeof(float2) * ny, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(h->ym, ym.data(), sizeof(float4) * ny, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(h->zeta, zt.data(), sizeof(float) * ny, hipMemcpyHostToDevice) != hipSuccess)
+    // every copy and clear on the grid's stream, which is non-blocking: work
+    // on the null stream would not be ordered with the grid's kernels (with
+    // several host threads creating and stepping grids, a late clear of the
+    // fields or flags has wiped state already uploaded)
+    if (hipMemcpyAsync(h->xm, xm.data(), sizeof(float4) * nx, hipMemcpyHostToDevice, h->st) != hipSuccess ||
+        hipMemcpyAsync(h->xr, xr.data(), sizeof(float2) * ny, hipMemcpyHostToDevice, h->st) != hipSuccess ||
+        hipMemcpyAsync(h->ym, ym.data(), sizeof(float4) * ny, hipMemcpyHostToDevice, h->st) != hipSuccess ||
+        hipMemcpyAsync(h->zeta, zt.data(), sizeof(float) * ny, hipMemcpyHostToDevice, h->st) != hipSuccess ||
+        hipStreamSynchronize(h->st) != hipSuccess)
         return bail(fail(CLIFFS_ERR_HIP, "metric upload"));
-    if (hipMemset(h->fields, 0, 9 * h->fstride * sizeof(float)) != hipSuccess)
+    if (hipMemsetAsync(h->fields, 0, 9 * h->fstride * sizeof(float), h->st) != hipSuccess)
         return bail(fail(CLIFFS_ERR_HIP, "state clear"));
 
     // background celerity
@@ -1149,7 +1156,8 @@ int ensemble_rebuild(cliffs_ensemble* e) {
     e->nblocks = (int)bmap.size();
     if (hipMalloc(&e->dmap, sizeof(int2) * bmap.size()) != hipSuccess)
         return fail(CLIFFS_ERR_NOMEM, "ensemble block map");
-    HIPCHK(hipMemcpy(e->dmap, bmap.data(), sizeof(int2) * bmap.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(e->dmap, bmap.data(), sizeof(int2) * bmap.size(), hipMemcpyHostToDevice, e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
     // two steps: the argument blocks of this step, then of the next
     HIPCHK(hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal));
     hipError_t err = hipSuccess;
@@ -1533,10 +1541,12 @@ int cliffs_nest_create(cliffs_handle* h, int nxc, int nyc, const int* we_ny, con
         cliffs_nest_destroy(n);
         return fail(CLIFFS_ERR_NOMEM, "nest device allocation");
     }
-    if (hipMemcpy(n->we_ny, we_ny, sizeof(int) * nyc, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(n->sn_nx, sn_nx, sizeof(int) * nxc, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(n->we_cy, we_cy, sizeof(double) * nyc, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(n->sn_cx, sn_cx, sizeof(double) * nxc, hipMemcpyHostToDevice) != hipSuccess) {
+    // on the parent's (non-blocking) stream, which runs the nest's kernels
+    if (hipMemcpyAsync(n->we_ny, we_ny, sizeof(int) * nyc, hipMemcpyHostToDevice, h->st) != hipSuccess ||
+        hipMemcpyAsync(n->sn_nx, sn_nx, sizeof(int) * nxc, hipMemcpyHostToDevice, h->st) != hipSuccess ||
+        hipMemcpyAsync(n->we_cy, we_cy, sizeof(double) * nyc, hipMemcpyHostToDevice, h->st) != hipSuccess ||
+        hipMemcpyAsync(n->sn_cx, sn_cx, sizeof(double) * nxc, hipMemcpyHostToDevice, h->st) != hipSuccess ||
+        hipStreamSynchronize(h->st) != hipSuccess) {
         cliffs_nest_destroy(n);
         return fail(CLIFFS_ERR_HIP, "nest map upload");
     }
