@@ -1,26 +1,38 @@
-// Learned nest surrogate: offshore gauge series -> max-inundation map.
+// Learned nest surrogate: offshore gauge series -> max-inundation map and run-up.
 //
-// A small network, trained on runs the solver generates, that maps the
-// surface-elevation series of 64 gauges at a nested grid's boundary (256
-// samples) to the grid's 64 x 64 max-inundation map.  Every layer is a GEMM
-// on the MFMA units (BF16 operands, FP32 accumulation), written by hand:
+// A network, trained on runs the solver generates, that maps the surface-
+// elevation series of 64 gauges on a nested grid's boundary (512 samples)
+// to the grid's 512 x 512 max-inundation map and to its maximum run-up.
+// Every layer is a GEMM on the MFMA units (BF16 operands, FP32
+// accumulation), written by hand:
 //
-//   encoder  L1  1-D conv over time, kernel 8 = stride 8:  (B*32, 512) x (512, 128)
-//            L2  1-D conv, kernel 4 = stride 4:           (B*8, 512)  x (512, 256)
-//            L3  dense:                                  (B, 2048)   x (2048, 512)
-//   decoder  L4  dense to an 8 x 8 x 64 image:             (B, 512)    x (512, 4096)
-//            L5  2-D transposed conv 2x2, stride 2:      (B*64, 64)  x (64, 128)
-//            L6  2-D transposed conv 2x2, stride 2:      (B*256, 32) x (32, 64)
-//            L7  2-D transposed conv 2x2, stride 2:      (B*1024, 16) x (16, 4)
+//   encoder (1-D convolutions along time, all gauges as input channels)
+//     E1  kernel 4, stride 4:       (B*128, 4*64)  x (256, 128)
+//     E2  kernel 5, stride 1:       (B*132, 5*128) x (640, 128)
+//     E3  kernel 5, stride 2:       (B*66, 5*128)  x (640, 256)
+//     E4  kernel 5, stride 2:       (B*34, 5*256)  x (1280, 256)
+//     pool: maximum and mean over the 32 time rows -> 512 features
+//   trunk   D1, D2 dense 512 -> 512
+//   heads   HR  dense 512 -> 1                       (run-up)
+//           M1  dense 512 -> 4096 (an 8 x 8 x 64 image)
+//           T1..T6  2-D transposed convolutions 2 x 2, stride 2:
+//                   8^2 x 64 -> 16^2 x 32 -> 32^2 x 16 -> 64^2 x 8 ->
+//                   128^2 x 8 -> 256^2 x 4 -> 512^2 x 1
 //
-// With kernel = stride, a convolution is a GEMM on a reshaped activation
-// (no im2col copies): the input rows (time-major gauge samples) and every
-// intermediate activation are laid out so that each layer's output, read
-// with the next layer's row length, is exactly the next layer's input.  The
-// 64 x 64 output comes out in quadtree order (8 x 8 base pixel, then three
-// 2 x 2 sub-pixel levels); the host permutes maps to and from that order.
-// ReLU after every layer but the last; MSE loss; Adam on FP32 master
-// weights with BF16 copies for the GEMMs.
+// A convolution along time is a GEMM over an overlapping strided view of
+// its input, A(m, k) = X[m * stride * cin + k], when each sample's rows are
+// stored with zero rows before and after them ("same" padding): no im2col
+// copy.  Each sample then has a few compute rows whose window runs into the
+// next sample; the GEMM's epilogue drops them and stores the valid rows at
+// their place in the next layer's padded layout.  The backward pass forms
+// the window gradients dZ W^T with the same GEMM and sums overlapping
+// windows back onto the input rows (col2im), masked by the input's ReLU.
+// With kernel = stride, a transposed convolution is a GEMM on a reshaped
+// activation; the map comes out in quadtree order (8 x 8 base pixel, then
+// six 2 x 2 sub-pixel levels) and the host permutes maps to and from it.
+// ReLU after every layer but HR and T6.  Loss: w_map * mean squared map
+// error + w_runup * mean squared run-up error; Adam on FP32 master weights
+// with BF16 copies for the GEMMs.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,6 +45,9 @@
 #include "../../include/cliffs_surrogate.h"
 
 namespace {
+
+constexpr int kMapSide = 512, kLevels = 6;         // 8 x 8 base pixels, six 2 x 2 levels
+constexpr long kMapPix = (long)kMapSide * kMapSide;
 
 typedef __attribute__((__vector_size__(4 * sizeof(short)))) short bf16x4;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
@@ -48,11 +63,14 @@ __device__ __forceinline__ float from_bf16(short s) {
 }
 
 // ---------------------------------------------------------------------------
-// GEMM:  C(m,n) = epi( sum_k A(m,k) B(k,n) )
+// GEMM:  C(row(m), n) = epi( sum_k A(m,k) B(k,n) )
 //   A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn]  (BF16; any strides,
-//   so transposed operands of the backward pass need no copies)
-//   epi: + bias[n]; ReLU; or mask: C = (mask(m,n) > 0) ? acc : 0 (ReLU
+//   so overlapping convolution windows and the transposed operands of the
+//   backward pass need no copies)
+//   epi: + bias[n]; ReLU; or mask: C = (mask(row,n) > 0) ? acc : 0 (ReLU
 //   backward); accumulate: C += result.  Outputs FP32 C (ldc) and/or BF16 Cb.
+//   row(m) = m, or with rp > 0: t = m % rp, rows with t >= rv are dropped
+//   and the others stored at row (m / rp) * op + osh + t.
 // Tiles: 64 x 64 per 256-thread block, K in steps of 16 through LDS; each
 // wave computes a 32 x 32 quarter as 2 x 2 MFMA 16x16x16 tiles.
 constexpr int kTM = 64, kTN = 64, kTK = 16;
@@ -69,6 +87,9 @@ struct GemmArgs {
     const float* bias;
     const short* mask;  // BF16, same layout as C (or null)
     int relu, accumulate;
+    int rp, rv, op, osh;  // output row map (rp = 0: identity)
+    int kchunk;           // split K: block z sums k in [z kchunk, (z+1) kchunk) ...
+    float* part;          // ... into part[z][M][N] (no epilogue; 0: no split)
 };
 
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmArgs g) {
@@ -77,23 +98,25 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmArgs g) {
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int m0 = blockIdx.y * kTM, n0 = blockIdx.x * kTN;
     const int wm = (w >> 1) * 32, wn = (w & 1) * 32;  // wave's quarter
+    const int kbeg = g.part ? blockIdx.z * g.kchunk : 0;
+    const int kend = g.part ? min(g.K, kbeg + g.kchunk) : g.K;
     f32x4 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    for (int k0 = 0; k0 < g.K; k0 += kTK) {
+    for (int k0 = kbeg; k0 < kend; k0 += kTK) {
         // stage A (64 x 16) and B (16 x 64): 1024 elements each, 4 per thread
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int e = t + 256 * r;
             const int mm = e >> 4, kk = e & 15;      // A: row-major over k
             const int m = m0 + mm, k = k0 + kk;
-            As[mm][kk] = (m < g.M && k < g.K) ? g.A[(long)m * g.sam + (long)k * g.sak] : (short)0;
+            As[mm][kk] = (m < g.M && k < kend) ? g.A[(long)m * g.sam + (long)k * g.sak] : (short)0;
             const int nn = e & 63, kb = e >> 6;      // B: consecutive n
             const int n = n0 + nn, kB = k0 + kb;
-            Bs[nn][kb] = (n < g.N && kB < g.K) ? g.B[(long)kB * g.sbk + (long)n * g.sbn] : (short)0;
+            Bs[nn][kb] = (n < g.N && kB < kend) ? g.B[(long)kB * g.sbk + (long)n * g.sbn] : (short)0;
         }
         __syncthreads();
         // MFMA 16x16x16 operand layout: lane l holds row (A) / column (B)
@@ -125,8 +148,18 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmArgs g) {
                 const int m = m0 + wm + 16 * i + rq + q;
                 const int n = n0 + wn + 16 * j + c16;
                 if (m >= g.M || n >= g.N) continue;
+                if (g.part) {
+                    g.part[((long)blockIdx.z * g.M + m) * g.N + n] = acc[i][j][q];
+                    continue;
+                }
+                long row = m;
+                if (g.rp > 0) {
+                    const int tt = m % g.rp;
+                    if (tt >= g.rv) continue;
+                    row = (long)(m / g.rp) * g.op + g.osh + tt;
+                }
                 float v = acc[i][j][q];
-                const long o = (long)m * g.ldc + n;
+                const long o = row * g.ldc + n;
                 if (g.bias) v += g.bias[n];
                 if (g.relu) v = fmaxf(v, 0.f);
                 if (g.mask && !(from_bf16(g.mask[o]) > 0.f)) v = 0.f;
@@ -136,9 +169,43 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmArgs g) {
             }
 }
 
-hipError_t gemm(const GemmArgs& g, hipStream_t st) {
+// sum of the split-K partials in a fixed order: C = sum_z part[z] (+= C)
+__global__ void ksum_kernel(const float* part, int splits, int M, int N, float* C, long ldc, int accumulate) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)M * N) return;
+    float v = 0.f;
+    for (int z = 0; z < splits; z++) v += part[(long)z * M * N + i];
+    const long o = (i / N) * ldc + i % N;
+    if (accumulate) v += C[o];
+    C[o] = v;
+}
+
+constexpr long kSplitWords = 1L << 22;   // split-K partials (FP32 words)
+
+// A product whose few output tiles would leave most of the device idle over
+// a long K (the weight gradients of the decoder: 4 x 4 outputs summed over
+// 2M rows) is split along K when a partial buffer is given and the
+// epilogue is a plain FP32 store.
+hipError_t gemm(GemmArgs g, hipStream_t st, float* part = nullptr) {
     const dim3 grid((g.N + kTN - 1) / kTN, (g.M + kTM - 1) / kTM);
-    hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, st, g);
+    const long tiles = (long)grid.x * grid.y;
+    int splits = 1;
+    if (part && g.C && !g.Cb && !g.bias && !g.relu && !g.mask && g.rp == 0 && tiles < 256 && g.K >= 4096) {
+        splits = (int)std::min<long>({(g.K + 2047) / 2048, 1024 / tiles + 1, kSplitWords / ((long)g.M * g.N)});
+        splits = std::max(splits, 1);
+    }
+    if (splits > 1) {
+        g.kchunk = (int)(((g.K + splits - 1) / splits + kTK - 1) / kTK * kTK);
+        splits = (g.K + g.kchunk - 1) / g.kchunk;
+        g.part = part;
+        hipLaunchKernelGGL(gemm_bf16_kernel, dim3(grid.x, grid.y, splits), dim3(256), 0, st, g);
+        const long n = (long)g.M * g.N;
+        hipLaunchKernelGGL(ksum_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (const float*)part,
+                           splits, g.M, g.N, g.C, g.ldc, g.accumulate);
+    } else {
+        g.part = nullptr;
+        hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, st, g);
+    }
     return hipGetLastError();
 }
 
@@ -171,14 +238,85 @@ __global__ void to_bf16_kernel(const float* x, short* y, long n) {
     if (i < n) y[i] = to_bf16(x[i]);
 }
 
-// MSE: loss = mean (p - y)^2, dP = 2 (p - y) / n (FP32 and BF16)
-__global__ void mse_kernel(const float* p, const float* y, long n, float* dp, short* dpb, float* part) {
+// Pool over time: per sample and channel, the maximum (first row holding
+// it) and the mean of the T rows of X (B*T x C, BF16) -> feat (B x 2C, BF16:
+// maxima then means) and the maxima's rows.
+__global__ void pool_kernel(const short* X, int B, int T, int C, short* feat, int* arg) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B * C) return;
+    const int b = i / C, c = i - b * C;
+    const short* x = X + (long)b * T * C + c;
+    float mx = from_bf16(x[0]), sum = 0.f;
+    int am = 0;
+    for (int t = 0; t < T; t++) {
+        const float v = from_bf16(x[(long)t * C]);
+        sum += v;
+        if (v > mx) {
+            mx = v;
+            am = t;
+        }
+    }
+    feat[(long)b * 2 * C + c] = to_bf16(mx);
+    feat[(long)b * 2 * C + C + c] = to_bf16(sum / (float)T);
+    arg[i] = am;
+}
+
+// Pool backward: dfeat (B x 2C, FP32) -> dZ of the pooled layer in its
+// compute layout (B*P x C, P >= T rows per sample, rows t >= T zero),
+// masked by that layer's ReLU output X (B*T x C): FP32 and BF16.
+__global__ void pool_back_kernel(const float* dfeat, const int* arg, const short* X, int B, int T,
+                                 int P, int C, float* dz, short* dzb) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)B * P * C) return;
+    const int c = (int)(i % C);
+    const long r = i / C;
+    const int b = (int)(r / P), t = (int)(r - (long)b * P);
+    float v = 0.f;
+    if (t < T && from_bf16(X[((long)b * T + t) * C + c]) > 0.f) {
+        v = dfeat[(long)b * 2 * C + C + c] / (float)T;
+        if (arg[b * C + c] == t) v += dfeat[(long)b * 2 * C + c];
+    }
+    dz[i] = v;
+    dzb[i] = to_bf16(v);
+}
+
+// col2im of a convolution along time (kernel k, stride s, compute rows pc
+// per sample, valid vc): the window gradients dcols (B*pc x k*C, FP32) summed
+// onto the input rows, which the previous layer stored at period pin with
+// shift sh; the result goes to the previous layer's compute layout (pp rows
+// per sample, valid vp), masked by its ReLU output act (input layout).
+__global__ void col2im_kernel(const float* dcols, const short* act, int B, int C, int k, int s,
+                              int pc, int vc, int pin, int sh, int pp, int vp, float* dz, short* dzb) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)B * pp * C) return;
+    const int c = (int)(i % C);
+    const long r = i / C;
+    const int b = (int)(r / pp), t = (int)(r - (long)b * pp);
+    float v = 0.f;
+    if (t < vp) {
+        const int row = sh + t;  // the input row in the padded layout
+        if (from_bf16(act[((long)b * pin + row) * C + c]) > 0.f)
+            for (int dt = 0; dt < k; dt++) {
+                const int u = row - dt;
+                if (u < 0 || u % s) continue;
+                const int tc = u / s;
+                if (tc >= vc) continue;
+                v += dcols[((long)b * pc + tc) * (long)(k * C) + (long)dt * C + c];
+            }
+    }
+    dz[i] = v;
+    dzb[i] = to_bf16(v);
+}
+
+// map loss: sum of (p - y)^2 per block, dP = 2 w (p - y) / n (FP32 and BF16)
+__global__ void mse_kernel(const float* p, const float* y, long n, float w, float* dp, short* dpb,
+                           float* part) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     float e2 = 0.f;
     if (i < n) {
         const float d = p[i] - y[i];
         e2 = d * d;
-        const float gd = 2.f * d / (float)n;
+        const float gd = 2.f * w * d / (float)n;
         dp[i] = gd;
         dpb[i] = to_bf16(gd);
     }
@@ -190,6 +328,44 @@ __global__ void mse_kernel(const float* p, const float* y, long n, float* dp, sh
         __syncthreads();
     }
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// run-up loss of a batch (one block): sum of squared errors into *part,
+// dr = 2 w (r - y) / B (FP32 and BF16)
+__global__ void runup_loss_kernel(const float* r, const float* y, int B, float w, float* dr, short* drb,
+                                  float* part) {
+    __shared__ float red[256];
+    float e2 = 0.f;
+    for (int i = threadIdx.x; i < B; i += blockDim.x) {
+        const float d = r[i] - y[i];
+        e2 += d * d;
+        const float gd = 2.f * w * d / (float)B;
+        dr[i] = gd;
+        drb[i] = to_bf16(gd);
+    }
+    red[threadIdx.x] = e2;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *part = w * red[0];
+}
+
+// the batch's loss from the partial sums of mse_kernel (blocks of them)
+// and runup_loss_kernel (part[blocks]), summed in a fixed order in FP64:
+// w_map * mean squared map error + (already weighted) run-up term
+__global__ void loss_final_kernel(const float* part, int blocks, long n, float wmap, int b, float* out) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < blocks; i += blockDim.x) s += (double)part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = (float)((double)wmap * red[0] / (double)n + (double)part[blocks] / (double)b);
 }
 
 // Adam (FP32 master weights), BF16 copy of the new weights
@@ -207,7 +383,31 @@ __global__ void adam_kernel(float* w, short* wb, const float* g, float* m, float
     if (wb) wb[i] = to_bf16(wi);
 }
 
-// rows of a (n_rows x row_len) FP32 array picked by index, as BF16
+// acc (+)= p over n elements (the networks' quadtree outputs summed)
+__global__ void accumulate_kernel(const float* p, long n, int first, float* acc) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) acc[i] = first ? p[i] : acc[i] + p[i];
+}
+
+// Inundation maps in raster order from the networks' summed flood levels
+// (quadtree order, `nets` networks): per pixel the level is the mean, the
+// map the ground height where ground <= level, else 0.
+__global__ void levels_to_maps_kernel(const float* acc, int b, float inv_nets, const float* ground,
+                                      float* maps) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)b * kMapPix) return;
+    const long bi = i / kMapPix;
+    const int pix = (int)(i - bi * kMapPix);
+    const int iy = pix / kMapSide, ix = pix % kMapSide;
+    int q = (iy >> kLevels) * 8 + (ix >> kLevels);
+    for (int k = 1; k <= kLevels; k++)
+        q = q * 4 + 2 * ((iy >> (kLevels - k)) & 1) + ((ix >> (kLevels - k)) & 1);
+    const float level = acc[bi * kMapPix + q] * inv_nets;
+    const float g = ground[pix];
+    maps[i] = g <= level ? g : 0.f;
+}
+
+// rows of a (n_rows x row_len) FP32 array picked by index, as BF16 and/or FP32
 __global__ void gather_rows_kernel(const float* src, const int* idx, int nrows, long row_len,
                                    short* dstb, float* dstf) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -231,14 +431,37 @@ int fail(int code, const std::string& m) {
                                                                   hipGetErrorString(e_));  \
     } while (0)
 
-// layer table: (rows per sample, in, out)
-struct Layer {
-    int rows, in, out;
+constexpr int kG = 64, kT = 512, kMap = 512;
+constexpr long kIn = (long)kG * kT, kOut = (long)kMap * kMap;
+
+// encoder convolutions along time: kernel, stride, channels in/out, input
+// rows per sample (padded), compute rows (pin / stride), valid rows, output
+// rows per sample as stored for the next layer, and their shift
+struct Conv {
+    int k, s, cin, cout, pin, pc, valid, pout, sh;
 };
-constexpr Layer kL[7] = {{32, 512, 128}, {8, 512, 256}, {1, 2048, 512}, {1, 512, 4096},
-                         {64, 64, 128},  {256, 32, 64}, {1024, 16, 4}};
-constexpr int kG = 64, kT = 256, kMap = 64;
-constexpr int kIn = kG * kT, kOut = kMap * kMap;
+constexpr Conv kE[4] = {{4, 4, 64, 128, 512, 128, 128, 132, 2},
+                        {5, 1, 128, 128, 132, 132, 128, 132, 2},
+                        {5, 2, 128, 256, 132, 66, 64, 68, 2},
+                        {5, 2, 256, 256, 68, 34, 32, 32, 0}};
+constexpr int kPoolT = 32, kPoolC = 256, kFeat = 2 * kPoolC;
+// dense and transposed-convolution layers: rows per sample, in, out, ReLU
+struct Dense {
+    int rows, in, out, relu;
+};
+enum { D1, D2, HR, M1, T1, T2, T3, T4, T5, T6, kND };
+constexpr Dense kD[kND] = {{1, 512, 512, 1},    {1, 512, 512, 1},     {1, 512, 1, 0},
+                           {1, 512, 4096, 1},   {64, 64, 128, 1},     {256, 32, 64, 1},
+                           {1024, 16, 32, 1},   {4096, 8, 32, 1},     {16384, 8, 16, 1},
+                           {65536, 4, 4, 0}};
+constexpr int kLayers = 4 + kND;
+
+long conv_k(int l) { return (long)kE[l].k * kE[l].cin; }
+// elements of a stored encoder activation (input of E(l+1)) per sample, and
+// the zero slack after the last sample that the next layer's overlapping
+// windows of its garbage rows read
+long enc_elems(int l, int b) { return (long)b * kE[l].pout * kE[l].cout; }
+long enc_slack(int l) { return l < 3 ? conv_k(l + 1) : 0; }
 
 }  // namespace
 
@@ -248,45 +471,91 @@ struct cliffs_sur {
     long np = 0;                         // parameter count
     float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;  // FP32 [np]
     short* wb = nullptr;                 // BF16 copy [np]
-    long woff[7], boff[7];               // offsets of W_l (in x out, row-major) and b_l
-    short* act[8] = {};                  // BF16 activations: act[0] input, act[l+1] output of L_l
-    float* pred = nullptr;               // FP32 output (bmax * 4096)
-    float* yb = nullptr;                 // targets of the batch
-    float *dF = nullptr;                 // FP32 gradient scratch (largest activation)
+    long woff[kLayers], boff[kLayers];   // offsets of W_l (in x out, row-major) and b_l
+    short* ax = nullptr;                 // input (BF16)
+    short* ae[4] = {};                   // encoder outputs, padded layouts
+    short* feat = nullptr;               // pooled features (B x 512)
+    int* arg = nullptr;                  // rows of the pooled maxima
+    short* ad[kND] = {};                 // outputs of D1, D2, M1, T1..T5 (BF16)
+    float* pred = nullptr;               // FP32 map output (bmax * kOut)
+    float* rpred = nullptr;              // FP32 run-up output (bmax)
+    float* yb = nullptr;                 // map targets of the batch
+    float* yr = nullptr;                 // run-up targets of the batch
+    float* dF = nullptr;                 // FP32 gradient scratch (largest activation)
     short *dA = nullptr, *dB = nullptr;  // BF16 gradient ping-pong
+    float* dC = nullptr;                 // FP32 window gradients (col2im input)
+    float* dH = nullptr;                 // FP32 trunk gradient (B x 512)
+    short* dHb = nullptr;
+    float* dR = nullptr;                 // FP32 run-up gradient (B)
+    short* dRb = nullptr;
     float* part = nullptr;               // reduction partials
+    float* kpart = nullptr;              // split-K partials (kSplitWords)
     float* hpart = nullptr;              // pinned
-    int* didx = nullptr;
-    float *dx = nullptr, *dy = nullptr;  // training data on the device
+    int* didx = nullptr;                 // batch rows of every training step
+    long nidx = 0;
+    float* dloss = nullptr;              // per-step losses
+    long nloss = 0;
+    float* ground = nullptr;             // map ground heights (raster), for cliffs_sur_predict
+    float *dx = nullptr, *dy = nullptr, *dyr = nullptr;  // training data on the device
     long ndata = 0;
     int t = 0;                           // Adam step
 };
 
 namespace {
 
-long act_elems(int l, int b) {  // elements of act[l] for batch b
-    if (l == 0) return (long)b * kIn;
-    return (long)b * kL[l - 1].rows * kL[l - 1].out;
+long dense_out(int l, int b) { return (long)b * kD[l].rows * kD[l].out; }
+
+GemmArgs plain_gemm(int M, int N, int K, const short* A, long sam, long sak, const short* B, long sbk,
+                    long sbn) {
+    GemmArgs a{};
+    a.M = M;
+    a.N = N;
+    a.K = K;
+    a.A = A;
+    a.sam = sam;
+    a.sak = sak;
+    a.B = B;
+    a.sbk = sbk;
+    a.sbn = sbn;
+    return a;
 }
 
 int forward(cliffs_sur* s, int b) {
-    for (int l = 0; l < 7; l++) {
-        const Layer& L = kL[l];
-        GemmArgs a{};
-        a.M = b * L.rows;
-        a.N = L.out;
-        a.K = L.in;
-        a.A = s->act[l];
-        a.sam = L.in;
-        a.sak = 1;
-        a.B = s->wb + s->woff[l];
-        a.sbk = L.out;
-        a.sbn = 1;
-        a.ldc = L.out;
+    // encoder
+    for (int l = 0; l < 4; l++) {
+        const Conv& E = kE[l];
+        const short* in = l == 0 ? s->ax : s->ae[l - 1];
+        GemmArgs a = plain_gemm(b * E.pc, E.cout, E.k * E.cin, in, (long)E.s * E.cin, 1,
+                                s->wb + s->woff[l], E.cout, 1);
         a.bias = s->w + s->boff[l];
-        a.relu = l < 6;
-        a.Cb = l < 6 ? s->act[l + 1] : nullptr;
-        a.C = l < 6 ? nullptr : s->pred;
+        a.relu = 1;
+        a.Cb = s->ae[l];
+        a.ldc = E.cout;
+        a.rp = E.pc;
+        a.rv = E.valid;
+        a.op = E.pout;
+        a.osh = E.sh;
+        CHK(gemm(a, s->st));
+    }
+    hipLaunchKernelGGL(pool_kernel, dim3((b * kPoolC + 255) / 256), dim3(256), 0, s->st, s->ae[3], b,
+                       kPoolT, kPoolC, s->feat, s->arg);
+    CHK(hipGetLastError());
+    // trunk, run-up head, decoder
+    for (int l = 0; l < kND; l++) {
+        const Dense& L = kD[l];
+        const short* in = l == D1 ? s->feat : (l == D2 || l == HR || l == M1) ? s->ad[l == D2 ? D1 : D2]
+                                                                             : s->ad[l - 1];
+        const int j = 4 + l;
+        GemmArgs a = plain_gemm(b * L.rows, L.out, L.in, in, L.in, 1, s->wb + s->woff[j], L.out, 1);
+        a.bias = s->w + s->boff[j];
+        a.relu = L.relu;
+        a.ldc = L.out;
+        if (l == HR)
+            a.C = s->rpred;
+        else if (l == T6)
+            a.C = s->pred;
+        else
+            a.Cb = s->ad[l];
         CHK(gemm(a, s->st));
     }
     return CLIFFS_SUR_OK;
@@ -303,76 +572,113 @@ int colsum(cliffs_sur* s, const float* X, int M, int N, float* out) {
     return CLIFFS_SUR_OK;
 }
 
-// backward from dP (FP32 in dF, BF16 in dA): parameter gradients into s->g
-int backward(cliffs_sur* s, int b) {
-    short* dcur = s->dA;   // BF16 gradient of layer l's output (pre-activation)
-    short* dnext = s->dB;
+// Gradients of a dense layer j (param index) from dZ (FP32 dz, BF16 dzb,
+// M x out): bias and weights; the input's gradient dX = dZ W^T, masked by
+// `mask` (the input's ReLU output) when given, into dx (FP32) / dxb (BF16),
+// accumulated when `acc`.
+int dense_back(cliffs_sur* s, int j, int M, int in, int out, const short* X, const float* dz,
+               const short* dzb, float* dx, short* dxb, const short* mask, int acc) {
     int rc;
-    for (int l = 6; l >= 0; l--) {
-        const Layer& L = kL[l];
+    if ((rc = colsum(s, dz, M, out, s->g + s->boff[j]))) return rc;
+    GemmArgs gw = plain_gemm(in, out, M, X, 1, in, dzb, out, 1);
+    gw.C = s->g + s->woff[j];
+    gw.ldc = out;
+    CHK(gemm(gw, s->st, s->kpart));
+    if (!dx && !dxb) return CLIFFS_SUR_OK;
+    GemmArgs gx = plain_gemm(M, in, out, dzb, out, 1, s->wb + s->woff[j], 1, out);
+    gx.C = dx;
+    gx.Cb = dxb;
+    gx.ldc = in;
+    gx.mask = mask;
+    gx.accumulate = acc;
+    CHK(gemm(gx, s->st));
+    return CLIFFS_SUR_OK;
+}
+
+// backward from dP (map, FP32 in dF and BF16 in dA) and dR (run-up):
+// parameter gradients into s->g
+int backward(cliffs_sur* s, int b) {
+    int rc;
+    short* dcur = s->dA;
+    short* dnext = s->dB;
+    // decoder T6 .. M1; M1's input gradient goes to the trunk gradient dH
+    for (int l = T6; l >= M1; l--) {
+        const Dense& L = kD[l];
         const int M = b * L.rows;
-        // dZ_l is in dF (FP32) and dcur (BF16); bias gradient
-        if ((rc = colsum(s, s->dF, M, L.out, s->g + s->boff[l]))) return rc;
-        // dW_l = X_l^T dZ_l: A(m=i, k=r) = X[r*in + i]
-        GemmArgs gw{};
-        gw.M = L.in;
-        gw.N = L.out;
-        gw.K = M;
-        gw.A = s->act[l];
-        gw.sam = 1;
-        gw.sak = L.in;
-        gw.B = dcur;
-        gw.sbk = L.out;
-        gw.sbn = 1;
+        const short* X = l == M1 ? s->ad[D2] : s->ad[l - 1];
+        float* dx = l == M1 ? s->dH : s->dF;
+        short* dxb = l == M1 ? s->dHb : dnext;
+        if ((rc = dense_back(s, 4 + l, M, L.in, L.out, X, s->dF, dcur, dx, dxb, X, 0))) return rc;
+        std::swap(dcur, dnext);
+    }
+    // run-up head, added to dH
+    if ((rc = dense_back(s, 4 + HR, b, 512, 1, s->ad[D2], s->dR, s->dRb, s->dH, s->dHb, s->ad[D2], 1)))
+        return rc;
+    // trunk: dH is dZ of D2 (masked by its ReLU)
+    if ((rc = dense_back(s, 4 + D2, b, 512, 512, s->ad[D1], s->dH, s->dHb, s->dF, s->dA, s->ad[D1], 0)))
+        return rc;
+    if ((rc = dense_back(s, 4 + D1, b, kFeat, 512, s->feat, s->dF, s->dA, s->dH, s->dHb, nullptr, 0)))
+        return rc;
+    // pool -> dZ of E4 in its compute layout
+    {
+        const Conv& E = kE[3];
+        const long n = (long)b * E.pc * E.cout;
+        hipLaunchKernelGGL(pool_back_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->st, s->dH,
+                           s->arg, s->ae[3], b, kPoolT, E.pc, kPoolC, s->dF, s->dA);
+        CHK(hipGetLastError());
+    }
+    // encoder E4 .. E1: dZ in dF / dA
+    for (int l = 3; l >= 0; l--) {
+        const Conv& E = kE[l];
+        const int M = b * E.pc;
+        const long K = conv_k(l);
+        const short* X = l == 0 ? s->ax : s->ae[l - 1];
+        if ((rc = colsum(s, s->dF, M, E.cout, s->g + s->boff[l]))) return rc;
+        // dW = A^T dZ over the overlapping windows A(m, k) = X[m s cin + k]
+        GemmArgs gw = plain_gemm((int)K, E.cout, M, X, 1, (long)E.s * E.cin, s->dA, E.cout, 1);
         gw.C = s->g + s->woff[l];
-        gw.ldc = L.out;
-        CHK(gemm(gw, s->st));
+        gw.ldc = E.cout;
+        CHK(gemm(gw, s->st, s->kpart));
         if (l == 0) break;
-        // dX_l = dZ_l W_l^T, masked by ReLU'(previous activation) = dZ_{l-1}
-        GemmArgs gx{};
-        gx.M = M;
-        gx.N = L.in;
-        gx.K = L.out;
-        gx.A = dcur;
-        gx.sam = L.out;
-        gx.sak = 1;
-        gx.B = s->wb + s->woff[l];
-        gx.sbk = 1;
-        gx.sbn = L.out;
-        gx.C = s->dF;
-        gx.Cb = dnext;
-        gx.ldc = L.in;
-        gx.mask = s->act[l];
-        CHK(gemm(gx, s->st));
-        short* tmp = dcur;
-        dcur = dnext;
-        dnext = tmp;
+        // window gradients dZ W^T, then summed onto the input rows
+        GemmArgs gc = plain_gemm(M, (int)K, E.cout, s->dA, E.cout, 1, s->wb + s->woff[l], 1, E.cout);
+        gc.C = s->dC;
+        gc.ldc = K;
+        CHK(gemm(gc, s->st));
+        const Conv& P = kE[l - 1];
+        const long n = (long)b * P.pc * P.cout;
+        hipLaunchKernelGGL(col2im_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->st, s->dC,
+                           s->ae[l - 1], b, E.cin, E.k, E.s, E.pc, E.valid, E.pin, P.sh, P.pc, P.valid,
+                           s->dF, s->dA);
+        CHK(hipGetLastError());
     }
     return CLIFFS_SUR_OK;
 }
 
-// loss and dP for the batch in act[0] / yb
-int loss_grad(cliffs_sur* s, int b, float* loss) {
+// loss (into the device word dloss), dP (dF, dA) and dR for the batch in
+// ax / yb / yr; nothing waits for the host
+int loss_grad(cliffs_sur* s, int b, float wmap, float wrun, float* dloss) {
     const long n = (long)b * kOut;
     const int blocks = (int)((n + 255) / 256);
-    hipLaunchKernelGGL(mse_kernel, dim3(blocks), dim3(256), 0, s->st, s->pred, s->yb, n, s->dF, s->dA,
-                       s->part);
+    hipLaunchKernelGGL(mse_kernel, dim3(blocks), dim3(256), 0, s->st, s->pred, s->yb, n, wmap, s->dF,
+                       s->dA, s->part);
     CHK(hipGetLastError());
-    CHK(hipMemcpyAsync(s->hpart, s->part, sizeof(float) * blocks, hipMemcpyDeviceToHost, s->st));
-    CHK(hipStreamSynchronize(s->st));
-    double sum = 0;
-    for (int i = 0; i < blocks; i++) sum += s->hpart[i];
-    *loss = (float)(sum / (double)n);
+    hipLaunchKernelGGL(runup_loss_kernel, dim3(1), dim3(256), 0, s->st, s->rpred, s->yr, b, wrun, s->dR,
+                       s->dRb, s->part + blocks);
+    CHK(hipGetLastError());
+    hipLaunchKernelGGL(loss_final_kernel, dim3(1), dim3(256), 0, s->st, (const float*)s->part, blocks, n,
+                       wmap, b, dloss);
+    CHK(hipGetLastError());
     return CLIFFS_SUR_OK;
 }
 
-int upload_batch(cliffs_sur* s, int b, const float* x, const float* y) {
-    std::vector<float> tmp;
+int upload_batch(cliffs_sur* s, int b, const float* x, const float* y, const float* yr) {
     CHK(hipMemcpyAsync(s->dF, x, sizeof(float) * (size_t)b * kIn, hipMemcpyHostToDevice, s->st));
     hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)(((long)b * kIn + 255) / 256)), dim3(256), 0,
-                       s->st, s->dF, s->act[0], (long)b * kIn);
+                       s->st, s->dF, s->ax, (long)b * kIn);
     CHK(hipGetLastError());
     if (y) CHK(hipMemcpyAsync(s->yb, y, sizeof(float) * (size_t)b * kOut, hipMemcpyHostToDevice, s->st));
+    if (yr) CHK(hipMemcpyAsync(s->yr, yr, sizeof(float) * (size_t)b, hipMemcpyHostToDevice, s->st));
     return CLIFFS_SUR_OK;
 }
 
@@ -380,11 +686,14 @@ void free_sur(cliffs_sur* s) {
     if (!s) return;
     (void)hipSetDevice(s->dev);
     if (s->st) (void)hipStreamSynchronize(s->st);
-    for (void* p : {(void*)s->w, (void*)s->g, (void*)s->m, (void*)s->v, (void*)s->wb, (void*)s->pred,
-                    (void*)s->yb, (void*)s->dF, (void*)s->dA, (void*)s->dB, (void*)s->part,
-                    (void*)s->didx, (void*)s->dx, (void*)s->dy})
+    for (void* p : {(void*)s->w, (void*)s->g, (void*)s->m, (void*)s->v, (void*)s->wb, (void*)s->ax,
+                    (void*)s->feat, (void*)s->arg, (void*)s->pred, (void*)s->rpred, (void*)s->yb,
+                    (void*)s->yr, (void*)s->dF, (void*)s->dA, (void*)s->dB, (void*)s->dC, (void*)s->dH,
+                    (void*)s->dHb, (void*)s->dR, (void*)s->dRb, (void*)s->part, (void*)s->kpart, (void*)s->didx, (void*)s->dloss,
+                    (void*)s->dx, (void*)s->dy, (void*)s->dyr, (void*)s->ground})
         (void)hipFree(p);
-    for (auto p : s->act) (void)hipFree(p);
+    for (auto p : s->ae) (void)hipFree(p);
+    for (auto p : s->ad) (void)hipFree(p);
     if (s->hpart) (void)hipHostFree(s->hpart);
     if (s->st) (void)hipStreamDestroy(s->st);
     delete s;
@@ -400,6 +709,11 @@ int adam_all(cliffs_sur* s, float lr) {
     return CLIFFS_SUR_OK;
 }
 
+template <class T>
+bool dalloc(T** p, size_t n) {
+    return hipMalloc((void**)p, sizeof(T) * n) == hipSuccess;
+}
+
 }  // namespace
 
 extern "C" {
@@ -410,12 +724,12 @@ int cliffs_sur_dims(int* gauges, int* samples, int* map, int* layers) {
     if (gauges) *gauges = kG;
     if (samples) *samples = kT;
     if (map) *map = kMap;
-    if (layers) *layers = 7;
+    if (layers) *layers = kLayers;
     return CLIFFS_SUR_OK;
 }
 
 int cliffs_sur_create(int device, int batch_max, unsigned seed, cliffs_sur** out) {
-    if (!out || batch_max < 1) return fail(CLIFFS_SUR_ERR_ARG, "bad arguments");
+    if (!out || batch_max < 1 || batch_max > 256) return fail(CLIFFS_SUR_ERR_ARG, "bad arguments");
     *out = nullptr;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
@@ -431,32 +745,49 @@ int cliffs_sur_create(int device, int batch_max, unsigned seed, cliffs_sur** out
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&s->st) != hipSuccess)
         return bail(fail(CLIFFS_SUR_ERR_HIP, "stream"));
     long np = 0;
-    for (int l = 0; l < 7; l++) {
+    std::vector<long> fan_in(kLayers), nw(kLayers);
+    for (int l = 0; l < kLayers; l++) {
+        const long in = l < 4 ? conv_k(l) : kD[l - 4].in, o = l < 4 ? kE[l].cout : kD[l - 4].out;
         s->woff[l] = np;
-        np += (long)kL[l].in * kL[l].out;
+        np += in * o;
         s->boff[l] = np;
-        np += kL[l].out;
+        np += o;
+        fan_in[l] = in;
+        nw[l] = in * o;
     }
     s->np = np;
-    long amax = 0;
-    for (int l = 0; l <= 7; l++) amax = std::max(amax, act_elems(l, batch_max));
-    bool ok = hipMalloc(&s->w, sizeof(float) * np) == hipSuccess &&
-              hipMalloc(&s->g, sizeof(float) * np) == hipSuccess &&
-              hipMalloc(&s->m, sizeof(float) * np) == hipSuccess &&
-              hipMalloc(&s->v, sizeof(float) * np) == hipSuccess &&
-              hipMalloc(&s->wb, sizeof(short) * np) == hipSuccess &&
-              hipMalloc(&s->pred, sizeof(float) * (size_t)batch_max * kOut) == hipSuccess &&
-              hipMalloc(&s->yb, sizeof(float) * (size_t)batch_max * kOut) == hipSuccess &&
-              hipMalloc(&s->dF, sizeof(float) * amax) == hipSuccess &&
-              hipMalloc(&s->dA, sizeof(short) * amax) == hipSuccess &&
-              hipMalloc(&s->dB, sizeof(short) * amax) == hipSuccess &&
-              hipMalloc(&s->part, sizeof(float) * (amax / 64 + 4096)) == hipSuccess &&
-              hipHostMalloc(&s->hpart, sizeof(float) * (amax / 64 + 4096)) == hipSuccess &&
-              hipMalloc(&s->didx, sizeof(int) * batch_max) == hipSuccess;
-    for (int l = 0; l <= 7 && ok; l++)
-        ok = hipMalloc(&s->act[l], sizeof(short) * act_elems(l, batch_max)) == hipSuccess;
+    const int B = batch_max;
+    // FP32 gradient scratch: the largest of the map output, the decoder's
+    // activations and the encoder's compute layouts
+    long fmax = (long)B * kOut, cmax = 0;
+    for (int l = 0; l < kND; l++) fmax = std::max(fmax, dense_out(l, B));
+    for (int l = 0; l < 4; l++) {
+        fmax = std::max(fmax, (long)B * kE[l].pc * kE[l].cout);
+        cmax = std::max(cmax, (long)B * kE[l].pc * conv_k(l));
+    }
+    bool ok = dalloc(&s->w, np) && dalloc(&s->g, np) && dalloc(&s->m, np) && dalloc(&s->v, np) &&
+              dalloc(&s->wb, np) && dalloc(&s->ax, (size_t)B * kIn + conv_k(0)) &&
+              dalloc(&s->feat, (size_t)B * kFeat) && dalloc(&s->arg, (size_t)B * kPoolC) &&
+              dalloc(&s->pred, (size_t)B * kOut) && dalloc(&s->rpred, (size_t)B) &&
+              dalloc(&s->yb, (size_t)B * kOut) && dalloc(&s->yr, (size_t)B) && dalloc(&s->dF, fmax) &&
+              dalloc(&s->dA, fmax) && dalloc(&s->dB, fmax) && dalloc(&s->dC, cmax) &&
+              dalloc(&s->dH, (size_t)B * kFeat) && dalloc(&s->dHb, (size_t)B * kFeat) &&
+              dalloc(&s->dR, (size_t)B) && dalloc(&s->dRb, (size_t)B) &&
+              dalloc(&s->part, fmax / 256 + 4096) && dalloc(&s->kpart, kSplitWords) &&
+              dalloc(&s->dloss, 1) && (s->nloss = 1) &&
+              hipHostMalloc(&s->hpart, sizeof(float) * (fmax / 256 + 4096)) == hipSuccess;
+    for (int l = 0; l < 4 && ok; l++) {
+        const size_t n = enc_elems(l, B) + enc_slack(l);
+        // padding rows and slack are read by overlapping windows: zero once,
+        // never written
+        ok = dalloc(&s->ae[l], n) && hipMemset(s->ae[l], 0, sizeof(short) * n) == hipSuccess;
+    }
+    if (ok) ok = hipMemset(s->ax, 0, sizeof(short) * ((size_t)B * kIn + conv_k(0))) == hipSuccess;
+    for (int l = 0; l < kND && ok; l++)
+        if (l != HR && l != T6) ok = dalloc(&s->ad[l], dense_out(l, B));
     if (!ok) return bail(fail(CLIFFS_SUR_ERR_NOMEM, "device allocation"));
-    // He-uniform weights, zero biases (host PRNG: splitmix64)
+    // He-uniform weights, zero biases (host PRNG: splitmix64); the two
+    // linear output layers start 10x smaller
     std::vector<float> hw(np, 0.f);
     unsigned long long z = 0x9E3779B97F4A7C15ull * (seed + 1);
     auto rnd = [&]() {
@@ -467,10 +798,10 @@ int cliffs_sur_create(int device, int batch_max, unsigned seed, cliffs_sur** out
         x ^= x >> 31;
         return (double)(x >> 11) * (1.0 / 9007199254740992.0);
     };
-    for (int l = 0; l < 7; l++) {
-        const double lim = std::sqrt(6.0 / kL[l].in) * (l == 6 ? 0.1 : 1.0);
-        for (long i = 0; i < (long)kL[l].in * kL[l].out; i++)
-            hw[s->woff[l] + i] = (float)((2.0 * rnd() - 1.0) * lim);
+    for (int l = 0; l < kLayers; l++) {
+        const bool last = l == 4 + HR || l == 4 + T6;
+        const double lim = std::sqrt(6.0 / (double)fan_in[l]) * (last ? 0.1 : 1.0);
+        for (long i = 0; i < nw[l]; i++) hw[s->woff[l] + i] = (float)((2.0 * rnd() - 1.0) * lim);
     }
     const int rc = cliffs_sur_set_params(s, hw.data());
     if (rc) return bail(rc);
@@ -511,76 +842,155 @@ int cliffs_sur_get_params(cliffs_sur* s, float* p) {
     return CLIFFS_SUR_OK;
 }
 
-int cliffs_sur_forward(cliffs_sur* s, int b, const float* x, float* y) {
-    if (!s || !x || !y || b < 1 || b > s->bmax) return fail(CLIFFS_SUR_ERR_ARG, "bad arguments");
+int cliffs_sur_forward(cliffs_sur* s, int b, const float* x, float* y, float* runup) {
+    if (!s || !x || (!y && !runup) || b < 1 || b > s->bmax) return fail(CLIFFS_SUR_ERR_ARG, "bad arguments");
     CHK(hipSetDevice(s->dev));
     int rc;
-    if ((rc = upload_batch(s, b, x, nullptr)) || (rc = forward(s, b))) return rc;
-    CHK(hipMemcpyAsync(y, s->pred, sizeof(float) * (size_t)b * kOut, hipMemcpyDeviceToHost, s->st));
+    if ((rc = upload_batch(s, b, x, nullptr, nullptr)) || (rc = forward(s, b))) return rc;
+    if (y) CHK(hipMemcpyAsync(y, s->pred, sizeof(float) * (size_t)b * kOut, hipMemcpyDeviceToHost, s->st));
+    if (runup) CHK(hipMemcpyAsync(runup, s->rpred, sizeof(float) * (size_t)b, hipMemcpyDeviceToHost, s->st));
     CHK(hipStreamSynchronize(s->st));
     return CLIFFS_SUR_OK;
 }
 
-int cliffs_sur_loss_grad(cliffs_sur* s, int b, const float* x, const float* y, float* loss,
-                         float* grads) {
-    if (!s || !x || !y || !loss || b < 1 || b > s->bmax) return fail(CLIFFS_SUR_ERR_ARG, "bad arguments");
-    CHK(hipSetDevice(s->dev));
-    int rc;
-    if ((rc = upload_batch(s, b, x, y)) || (rc = forward(s, b)) || (rc = loss_grad(s, b, loss)) ||
-        (rc = backward(s, b)))
-        return rc;
-    if (grads) CHK(hipMemcpy(grads, s->g, sizeof(float) * s->np, hipMemcpyDeviceToHost));
-    CHK(hipStreamSynchronize(s->st));
-    return CLIFFS_SUR_OK;
-}
-
-int cliffs_sur_train(cliffs_sur* s, long n, const float* x, const float* y, int b, int steps, float lr,
-                     unsigned seed, float* losses) {
-    if (!s || n < 1 || !x || !y || b < 1 || b > s->bmax || steps < 0)
+int cliffs_sur_loss_grad(cliffs_sur* s, int b, const float* x, const float* y, const float* yr, float wmap,
+                         float wrun, float* loss, float* grads) {
+    if (!s || !x || !y || !yr || !loss || b < 1 || b > s->bmax)
         return fail(CLIFFS_SUR_ERR_ARG, "bad arguments");
     CHK(hipSetDevice(s->dev));
+    int rc;
+    if ((rc = upload_batch(s, b, x, y, yr)) || (rc = forward(s, b)) ||
+        (rc = loss_grad(s, b, wmap, wrun, s->dloss)) || (rc = backward(s, b)))
+        return rc;
+    CHK(hipMemcpyAsync(s->hpart, s->dloss, sizeof(float), hipMemcpyDeviceToHost, s->st));
+    if (grads) CHK(hipMemcpyAsync(grads, s->g, sizeof(float) * s->np, hipMemcpyDeviceToHost, s->st));
+    CHK(hipStreamSynchronize(s->st));
+    *loss = s->hpart[0];
+    return CLIFFS_SUR_OK;
+}
+
+int cliffs_sur_set_data(cliffs_sur* s, long n, const float* x, const float* y, const float* yr) {
+    if (!s || n < 1 || !x || !y || !yr) return fail(CLIFFS_SUR_ERR_ARG, "bad arguments");
+    CHK(hipSetDevice(s->dev));
+    CHK(hipStreamSynchronize(s->st));
     if (n != s->ndata) {
         (void)hipFree(s->dx);
         (void)hipFree(s->dy);
-        s->dx = s->dy = nullptr;
-        if (hipMalloc(&s->dx, sizeof(float) * n * kIn) != hipSuccess ||
-            hipMalloc(&s->dy, sizeof(float) * n * kOut) != hipSuccess)
+        (void)hipFree(s->dyr);
+        s->dx = s->dy = s->dyr = nullptr;
+        s->ndata = 0;
+        if (!dalloc(&s->dx, (size_t)n * kIn) || !dalloc(&s->dy, (size_t)n * kOut) || !dalloc(&s->dyr, (size_t)n))
             return fail(CLIFFS_SUR_ERR_NOMEM, "training data allocation");
         s->ndata = n;
     }
     CHK(hipMemcpy(s->dx, x, sizeof(float) * n * kIn, hipMemcpyHostToDevice));
     CHK(hipMemcpy(s->dy, y, sizeof(float) * n * kOut, hipMemcpyHostToDevice));
+    CHK(hipMemcpy(s->dyr, yr, sizeof(float) * n, hipMemcpyHostToDevice));
+    return CLIFFS_SUR_OK;
+}
+
+int cliffs_sur_train(cliffs_sur* s, int b, int steps, float lr, float wmap, float wrun, unsigned seed,
+                     float* losses) {
+    if (!s || b < 1 || b > s->bmax || steps < 0) return fail(CLIFFS_SUR_ERR_ARG, "bad arguments");
+    if (s->ndata < 1) return fail(CLIFFS_SUR_ERR_ARG, "no training data (cliffs_sur_set_data)");
+    CHK(hipSetDevice(s->dev));
+    const long n = s->ndata;
+    if (steps == 0) return CLIFFS_SUR_OK;
+    // the batches of every step drawn beforehand (epochs of a shuffled
+    // order, Fisher-Yates) and copied once, and the losses kept on the
+    // device: the steps are queued without waiting for the host
     unsigned long long z = 0xD1B54A32D192ED03ull * (seed + 7);
-    std::vector<int> perm(n), idx(b);
+    std::vector<int> perm(n), idx((size_t)steps * b);
     for (long i = 0; i < n; i++) perm[i] = (int)i;
     long pos = n;
-    int rc;
-    for (int it = 0; it < steps; it++) {
-        for (int k = 0; k < b; k++) {  // epochs of a shuffled order (Fisher-Yates)
-            if (pos >= n) {
-                for (long i = n - 1; i > 0; i--) {
-                    z = z * 6364136223846793005ull + 1442695040888963407ull;
-                    const long j = (long)((z >> 33) % (unsigned long long)(i + 1));
-                    std::swap(perm[i], perm[j]);
-                }
-                pos = 0;
+    for (size_t k = 0; k < idx.size(); k++) {
+        if (pos >= n) {
+            for (long i = n - 1; i > 0; i--) {
+                z = z * 6364136223846793005ull + 1442695040888963407ull;
+                const long j = (long)((z >> 33) % (unsigned long long)(i + 1));
+                std::swap(perm[i], perm[j]);
             }
-            idx[k] = perm[pos++];
+            pos = 0;
         }
-        CHK(hipMemcpyAsync(s->didx, idx.data(), sizeof(int) * b, hipMemcpyHostToDevice, s->st));
-        hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)(((long)b * kIn + 255) / 256)), dim3(256), 0,
-                           s->st, s->dx, s->didx, b, (long)kIn, s->act[0], (float*)nullptr);
-        CHK(hipGetLastError());
-        hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)(((long)b * kOut + 255) / 256)), dim3(256), 0,
-                           s->st, s->dy, s->didx, b, (long)kOut, (short*)nullptr, s->yb);
-        CHK(hipGetLastError());
-        float loss = 0.f;
-        if ((rc = forward(s, b)) || (rc = loss_grad(s, b, &loss)) || (rc = backward(s, b)) ||
-            (rc = adam_all(s, lr)))
-            return rc;
-        if (losses) losses[it] = loss;
+        idx[k] = perm[pos++];
     }
     CHK(hipStreamSynchronize(s->st));
+    if ((long)idx.size() > s->nidx) {
+        (void)hipFree(s->didx);
+        s->didx = nullptr;
+        s->nidx = 0;
+        if (!dalloc(&s->didx, idx.size())) return fail(CLIFFS_SUR_ERR_NOMEM, "batch index allocation");
+        s->nidx = (long)idx.size();
+    }
+    if (steps > s->nloss) {
+        (void)hipFree(s->dloss);
+        s->dloss = nullptr;
+        s->nloss = 0;
+        if (!dalloc(&s->dloss, (size_t)steps)) return fail(CLIFFS_SUR_ERR_NOMEM, "loss allocation");
+        s->nloss = steps;
+    }
+    CHK(hipMemcpyAsync(s->didx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, s->st));
+    int rc;
+    for (int it = 0; it < steps; it++) {
+        const int* bi = s->didx + (size_t)it * b;
+        hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)(((long)b * kIn + 255) / 256)), dim3(256), 0,
+                           s->st, s->dx, bi, b, kIn, s->ax, (float*)nullptr);
+        CHK(hipGetLastError());
+        hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)(((long)b * kOut + 255) / 256)), dim3(256), 0,
+                           s->st, s->dy, bi, b, kOut, (short*)nullptr, s->yb);
+        CHK(hipGetLastError());
+        hipLaunchKernelGGL(gather_rows_kernel, dim3(1), dim3(256), 0, s->st, s->dyr, bi, b, 1L,
+                           (short*)nullptr, s->yr);
+        CHK(hipGetLastError());
+        if ((rc = forward(s, b)) || (rc = loss_grad(s, b, wmap, wrun, s->dloss + it)) ||
+            (rc = backward(s, b)) || (rc = adam_all(s, lr)))
+            return rc;
+    }
+    if (losses) CHK(hipMemcpyAsync(losses, s->dloss, sizeof(float) * steps, hipMemcpyDeviceToHost, s->st));
+    CHK(hipStreamSynchronize(s->st));
+    return CLIFFS_SUR_OK;
+}
+
+int cliffs_sur_set_ground(cliffs_sur* s, const float* ground) {
+    if (!s || !ground) return fail(CLIFFS_SUR_ERR_ARG, "NULL argument");
+    CHK(hipSetDevice(s->dev));
+    if (!s->ground && !dalloc(&s->ground, kMapPix)) return fail(CLIFFS_SUR_ERR_NOMEM, "ground allocation");
+    CHK(hipMemcpyAsync(s->ground, ground, sizeof(float) * kMapPix, hipMemcpyHostToDevice, s->st));
+    CHK(hipStreamSynchronize(s->st));
+    return CLIFFS_SUR_OK;
+}
+
+int cliffs_sur_predict(cliffs_sur* const* nets, int k, int b, const float* x, float* runups, float* maps) {
+    if (!nets || k < 1 || !x || b < 1 || (!runups && !maps)) return fail(CLIFFS_SUR_ERR_ARG, "bad arguments");
+    cliffs_sur* s0 = nets[0];
+    for (int i = 0; i < k; i++) {
+        if (!nets[i] || nets[i]->dev != s0->dev) return fail(CLIFFS_SUR_ERR_ARG, "networks on different devices");
+        if (b > nets[i]->bmax) return fail(CLIFFS_SUR_ERR_ARG, "batch larger than a network's batch_max");
+    }
+    if (maps && !s0->ground) return fail(CLIFFS_SUR_ERR_ARG, "no ground set (cliffs_sur_set_ground)");
+    CHK(hipSetDevice(s0->dev));
+    const long n = (long)b * kOut;
+    int rc;
+    for (int i = 0; i < k; i++) {
+        cliffs_sur* s = nets[i];
+        if ((rc = upload_batch(s, b, x, nullptr, nullptr)) || (rc = forward(s, b))) return rc;
+        if (runups)
+            CHK(hipMemcpyAsync(runups + (size_t)i * b, s->rpred, sizeof(float) * b, hipMemcpyDeviceToHost, s->st));
+        CHK(hipStreamSynchronize(s->st));
+        if (maps) {  // the first network's FP32 scratch holds the sum (its input staging is done)
+            hipLaunchKernelGGL(accumulate_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s0->st,
+                               (const float*)s->pred, n, i == 0 ? 1 : 0, s0->dF);
+            CHK(hipGetLastError());
+            CHK(hipStreamSynchronize(s0->st));
+        }
+    }
+    if (maps) {
+        hipLaunchKernelGGL(levels_to_maps_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s0->st,
+                           (const float*)s0->dF, b, 1.0f / (float)k, (const float*)s0->ground, s0->pred);
+        CHK(hipGetLastError());
+        CHK(hipMemcpyAsync(maps, s0->pred, sizeof(float) * n, hipMemcpyDeviceToHost, s0->st));
+        CHK(hipStreamSynchronize(s0->st));
+    }
     return CLIFFS_SUR_OK;
 }
 
@@ -599,9 +1009,8 @@ int cliffs_sur_gemm_bf16(int device, int M, int N, int K, const float* A, long s
         (void)hipFree(ba);
         (void)hipFree(bb);
     };
-    if (hipMalloc(&fa, sizeof(float) * na) != hipSuccess || hipMalloc(&fb, sizeof(float) * nb) != hipSuccess ||
-        hipMalloc(&fc, sizeof(float) * (size_t)M * N) != hipSuccess ||
-        hipMalloc(&ba, sizeof(short) * na) != hipSuccess || hipMalloc(&bb, sizeof(short) * nb) != hipSuccess) {
+    if (!dalloc(&fa, na) || !dalloc(&fb, nb) || !dalloc(&fc, (size_t)M * N) || !dalloc(&ba, na) ||
+        !dalloc(&bb, nb)) {
         cleanup();
         return fail(CLIFFS_SUR_ERR_NOMEM, "device allocation");
     }
@@ -610,16 +1019,7 @@ int cliffs_sur_gemm_bf16(int device, int M, int N, int K, const float* A, long s
     if (e == hipSuccess) {
         hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, nullptr, fa, ba, na);
         hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, nullptr, fb, bb, nb);
-        GemmArgs g{};
-        g.M = M;
-        g.N = N;
-        g.K = K;
-        g.A = ba;
-        g.sam = sam;
-        g.sak = sak;
-        g.B = bb;
-        g.sbk = sbk;
-        g.sbn = sbn;
+        GemmArgs g = plain_gemm(M, N, K, ba, sam, sak, bb, sbk, sbn);
         g.C = fc;
         g.ldc = N;
         e = gemm(g, nullptr);

@@ -1,15 +1,17 @@
 """Learned nest surrogate: Python side of include/cliffs_surrogate.h.
 
 The network (hand-written BF16 MFMA kernels, csrc/surrogate.hip) maps the
-surface-elevation series of 64 gauges on a nested grid's offshore boundary
-(256 samples) to the nested grid's 64 x 64 max-inundation map (per pixel the
-highest ground the water reached, so the map's maximum is the run-up).  This module
-defines the nested site, generates data with the solver (parents stepped as
-an ensemble with device gauge recording for the inputs, the coupled
-parent/child nested solve for the targets), permutes maps to and from the
-network's quadtree order, trains, and gates predictions: two independently
-trained networks must agree, otherwise the scenario is solved with the full
-nested solve (the fallback, and the answer its error is measured against).
+surface-elevation series of 64 gauges on a nested grid's wet boundary (512
+samples) to the nested grid's 512 x 512 max-inundation map (per pixel, one
+child cell: the ground height the water reached, 0 where it did not) and
+to its maximum run-up (the ground height of the highest flooded cell).
+This module defines the nested site, generates data with the solver
+(parents stepped as an ensemble with device gauge recording for the inputs,
+the coupled parent/child nested solve for the targets), permutes maps to
+and from the network's quadtree order, trains, and gates predictions:
+several independently trained networks must agree on the run-up, otherwise
+the scenario is solved with the full nested solve (the fallback, and the
+answer its error is measured against).
 """
 from __future__ import annotations
 
@@ -20,8 +22,9 @@ import numpy as np
 import cliffs_gpu
 import synth
 
-G, T, MAP = 64, 256, 64
+G, T, MAP = 64, 512, 512
 NOUT = MAP * MAP
+LEVELS = 6                   # 2 x 2 sub-pixel levels below the 8 x 8 base
 _fp = C.POINTER(C.c_float)
 
 _sur = None
@@ -41,12 +44,15 @@ def _lib():
         "cliffs_sur_nparams": ([H, C.POINTER(C.c_long)], C.c_int),
         "cliffs_sur_get_params": ([H, _fp], C.c_int),
         "cliffs_sur_set_params": ([H, _fp], C.c_int),
-        "cliffs_sur_forward": ([H, C.c_int, _fp, _fp], C.c_int),
-        "cliffs_sur_loss_grad": ([H, C.c_int, _fp, _fp, _fp, _fp], C.c_int),
-        "cliffs_sur_train": ([H, C.c_long, _fp, _fp, C.c_int, C.c_int, C.c_float, C.c_uint, _fp],
+        "cliffs_sur_forward": ([H, C.c_int, _fp, _fp, _fp], C.c_int),
+        "cliffs_sur_loss_grad": ([H, C.c_int, _fp, _fp, _fp, C.c_float, C.c_float, _fp, _fp], C.c_int),
+        "cliffs_sur_set_data": ([H, C.c_long, _fp, _fp, _fp], C.c_int),
+        "cliffs_sur_train": ([H, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_uint, _fp],
                              C.c_int),
         "cliffs_sur_gemm_bf16": ([C.c_int, C.c_int, C.c_int, C.c_int, _fp, C.c_long, C.c_long, _fp,
                                   C.c_long, C.c_long, _fp], C.c_int),
+        "cliffs_sur_set_ground": ([H, _fp], C.c_int),
+        "cliffs_sur_predict": ([C.POINTER(H), C.c_int, C.c_int, _fp, _fp, _fp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -78,13 +84,14 @@ def _p(a):
 
 # -- layout ------------------------------------------------------------------
 
-def quadtree_order():
-    """q[iy*64 + ix] = position of pixel (iy, ix) in the network's output
-    (8 x 8 base pixel, then three 2 x 2 sub-pixel levels)."""
-    iy, ix = np.meshgrid(np.arange(MAP), np.arange(MAP), indexing="ij")
-    p0 = (iy >> 3) * 8 + (ix >> 3)
-    s = [2 * ((iy >> (3 - k)) & 1) + ((ix >> (3 - k)) & 1) for k in (1, 2, 3)]
-    return (((p0 * 4 + s[0]) * 4 + s[1]) * 4 + s[2]).ravel()
+def quadtree_order(size=MAP, levels=LEVELS):
+    """q[iy*size + ix] = position of pixel (iy, ix) in the network's output:
+    the 8 x 8 base pixel, then `levels` 2 x 2 sub-pixel levels."""
+    iy, ix = np.meshgrid(np.arange(size), np.arange(size), indexing="ij")
+    q = (iy >> levels) * 8 + (ix >> levels)
+    for k in range(1, levels + 1):
+        q = q * 4 + 2 * ((iy >> (levels - k)) & 1) + ((ix >> (levels - k)) & 1)
+    return q.ravel()
 
 
 _Q = quadtree_order()
@@ -103,6 +110,9 @@ def from_quadtree(y):
 
 
 # -- the network ---------------------------------------------------------------
+
+W_MAP, W_RUNUP = 10.0, 1.0   # loss weights of the map and the run-up
+
 
 class Surrogate:
     def __init__(self, batch_max=64, seed=0, device=0):
@@ -134,32 +144,74 @@ class Surrogate:
         p = _f32(p, (self.nparams,))
         _chk(_lib().cliffs_sur_set_params(self._h, _p(p)))
 
-    def forward(self, x):
-        """x (B, 256, 64) -> (B, 4096) quadtree-ordered outputs."""
+    def forward(self, x, maps=True):
+        """x (B, 512, 64) -> (quadtree maps (B, NOUT) or None, run-ups (B,))."""
         x = _f32(x)
         B = x.shape[0]
-        out = np.empty((B, NOUT), dtype=np.float32)
+        out = np.empty((B, NOUT), dtype=np.float32) if maps else None
+        ru = np.empty(B, dtype=np.float32)
         for b0 in range(0, B, self.batch_max):
             b1 = min(B, b0 + self.batch_max)
             xb = np.ascontiguousarray(x[b0:b1])
-            yb = np.empty((b1 - b0, NOUT), dtype=np.float32)
-            _chk(_lib().cliffs_sur_forward(self._h, b1 - b0, _p(xb), _p(yb)))
-            out[b0:b1] = yb
-        return out
+            rb = np.empty(b1 - b0, dtype=np.float32)
+            if maps:
+                yb = np.empty((b1 - b0, NOUT), dtype=np.float32)
+                _chk(_lib().cliffs_sur_forward(self._h, b1 - b0, _p(xb), _p(yb), _p(rb)))
+                out[b0:b1] = yb
+            else:
+                _chk(_lib().cliffs_sur_forward(self._h, b1 - b0, _p(xb), None, _p(rb)))
+            ru[b0:b1] = rb
+        return out, ru
 
-    def loss_grad(self, x, y):
-        x, y = _f32(x), _f32(y)
+    def loss_grad(self, x, y, yr, w_map=W_MAP, w_runup=W_RUNUP):
+        x, y, yr = _f32(x), _f32(y), _f32(yr)
         loss = C.c_float()
         g = np.empty(self.nparams, dtype=np.float32)
-        _chk(_lib().cliffs_sur_loss_grad(self._h, x.shape[0], _p(x), _p(y), C.byref(loss), _p(g)))
+        _chk(_lib().cliffs_sur_loss_grad(self._h, x.shape[0], _p(x), _p(y), _p(yr), float(w_map),
+                                         float(w_runup), C.byref(loss), _p(g)))
         return loss.value, g
 
-    def train(self, x, y, batch=32, steps=1000, lr=1e-3, seed=0):
-        x, y = _f32(x), _f32(y)
+    def set_ground(self, ground):
+        """Ground heights (MAP x MAP, raster; inf off land) for predict_all's maps."""
+        g = _f32(np.where(np.isfinite(ground), ground, np.inf), (MAP, MAP))
+        _chk(_lib().cliffs_sur_set_ground(self._h, _p(g)))
+
+    def set_data(self, x, y, yr):
+        """The training set, kept on the device for train()."""
+        x, y, yr = _f32(x), _f32(y), _f32(yr)
+        _chk(_lib().cliffs_sur_set_data(self._h, x.shape[0], _p(x), _p(y), _p(yr)))
+
+    def train(self, batch=32, steps=1000, lr=1e-3, seed=0, w_map=W_MAP, w_runup=W_RUNUP):
         losses = np.empty(steps, dtype=np.float32)
-        _chk(_lib().cliffs_sur_train(self._h, x.shape[0], _p(x), _p(y), int(batch), int(steps),
-                                     float(lr), int(seed), _p(losses)))
+        _chk(_lib().cliffs_sur_train(self._h, int(batch), int(steps), float(lr), float(w_map),
+                                     float(w_runup), int(seed), _p(losses)))
         return losses
+
+
+def predict_all(nets, x, maps=True):
+    """The networks' answer for inputs x (cliffs_sur_predict): every
+    network's run-up (k, n) in m, clipped at 0, and, with maps, the
+    inundation maps (n, MAP, MAP, raster) of their mean flood levels against
+    the site's ground (built on the device)."""
+    x = _f32(x)
+    n, k = x.shape[0], len(nets)
+    bm = min(net.batch_max for net in nets)
+    if maps and not getattr(nets[0], "_ground", False):
+        nets[0].set_ground(site().ground_win)
+        nets[0]._ground = True
+    hs = (C.c_void_p * k)(*[net._h for net in nets])
+    ru = np.empty((k, n), dtype=np.float32)
+    out = np.empty((n, MAP, MAP), dtype=np.float32) if maps else None
+    for b0 in range(0, n, bm):
+        b1 = min(n, b0 + bm)
+        xb = np.ascontiguousarray(x[b0:b1])
+        rb = np.empty((k, b1 - b0), dtype=np.float32)
+        mb = np.empty((b1 - b0, MAP, MAP), dtype=np.float32) if maps else None
+        _chk(_lib().cliffs_sur_predict(hs, k, b1 - b0, _p(xb), _p(rb), _p(mb) if maps else None))
+        ru[:, b0:b1] = rb
+        if maps:
+            out[b0:b1] = mb
+    return np.maximum(ru.astype(np.float64), 0.0), out
 
 
 def gemm_bf16(A, B, device=0):
@@ -174,33 +226,41 @@ def gemm_bf16(A, B, device=0):
 
 # -- the nested site and its scenarios ------------------------------------------
 #
-# One site: a parent grid (192 x 192, 900 m, shelf and coastline of
-# synth.shelf_coast with terrain seed SITE_SEED) and a child grid at 5x
-# refinement (180 m, dt/5) over the coastal strip, fed one-way by the parent
-# (cliffs_nest_*).  Minimal flow depth and friction as the reference sets
-# them for its 30" and 10" Okushiri grids (paramsOK30s.txt, paramsOK10s.txt:
-# 1 m and 0.5 m, Manning n^2 0.000625), at which run-up is a well-posed
-# function of the source.  Scenarios differ only in the source.  The
-# network's input is the parent's surface elevation at 64 gauges along the
-# child's offshore (west) edge, every EVERY parent steps; its output is the
-# child's max-inundation map over the coastal window.
+# One site: a regional parent grid (128 x 128 cells of 1350 m, shelf and
+# coastline of synth.shelf_coast with terrain seed SITE_SEED) and a coastal
+# child grid at 15x refinement (90 m, 526 x 526 cells) over 35 x 35 parent
+# cells around the coast, fed one-way by the parent (cliffs_nest_*).  The
+# child steps CSTEPS times per parent step (dt/3: Courant number about 0.6
+# in its deepest water, 150 m).  Minimal flow depth and friction as the
+# reference sets them for its 30" and 10" Okushiri grids (paramsOK30s.txt,
+# paramsOK10s.txt: 1 m and 0.5 m, Manning n^2 0.000625).  Scenarios differ
+# only in the source.  The network's input is the parent's surface
+# elevation at 64 gauges along the child's wet boundary (its offshore edge
+# and the wet parts of its two lateral edges), every EVERY parent steps; its
+# output is the child's max-inundation map over the 512 x 512-cell window
+# that leaves 7 cells at each of the child's edges.
 
 SITE_SEED = 5
-PNX = PNY = 192
-PDX = 900.0
-REFINE = 5
-CI0, CI1 = 124, 164          # child spans parent columns CI0..CI1 (0-based)
-CJ0, CJ1 = 32, 161           # and parent rows CJ0..CJ1
-EVERY = 4                    # parent steps between gauge samples and feed records
+PNX = PNY = 128
+PDX = 1350.0
+REFINE = 15
+CSTEPS = 3                   # child steps per parent step
+CI0, CI1 = 81, 116           # child spans parent columns CI0..CI1 (0-based)
+CJ0, CJ1 = 46, 81            # and parent rows CJ0..CJ1
+EVERY = 2                    # parent steps between gauge samples and feed records
 CUKE = 1.0e-3                # arrival threshold of the feed (m)
-MAP_I0, MAP_W = 56, 128      # map window: child columns MAP_I0.. (2 per pixel),
-MAP_J0, MAP_H = 3, 640        # rows MAP_J0.. (10 per pixel), clear of the child's edges
+MAP_I0, MAP_J0 = 7, 7        # map window: child columns / rows MAP_I0.. / MAP_J0..
+WIN = MAP                    # window cells per side (one map pixel per cell)
+AMP = (0.5, 8.0)             # source amplitude (m)
+SRC_LEN = (20e3, 60e3)       # source length (m)
+SRC_X = (0.15, 0.5)          # source centre, fraction of the parent's width
+
 
 def plain(d):
     """Ground height (m) at distance d (m) inland: a coastal plain rising
-    0.1 m per km for 25 km, then 10 m per km.  On the child's 180 m cells
-    the plain rises 1.8 cm per cell, so the solver's run-up (the ground
-    height of the highest flooded cell) moves in steps of ~1 % of a metre."""
+    0.1 m per km for 25 km, then 10 m per km.  On the child's 90 m cells the
+    plain rises 9 mm per cell, so the solver's run-up (the ground height of
+    the highest flooded cell) moves in steps of under 1 % of a metre."""
     return np.where(d < 25e3, 1e-4 * d, 2.5 + 1e-2 * (d - 25e3))
 
 
@@ -209,19 +269,32 @@ C_GROUND = 0.5                     # and of the child
 CROUGH = synth.ENSEMBLE_CROUGH     # Manning n^2 of both
 
 X_SCALE = 2.0     # gauge elevation (m) -> network input
-Y_SCALE = 0.125   # inundation height (m) -> network output
 RUNUP_MIN = 1.0   # run-ups below this (m) are compared in absolute terms
 
 _site = None
+
+
+def _perimeter_gauges(wet):
+    """G parent cells (1-based I, J) evenly spaced along the child's wet
+    boundary: its south edge from the coast to the offshore corner, the
+    offshore (west) edge, the north edge back to the coast."""
+    cells = [(i, CJ0) for i in range(CI1, CI0, -1)] + [(CI0, j) for j in range(CJ0, CJ1)] + \
+            [(i, CJ1) for i in range(CI0, CI1 + 1)]
+    cells = [(i, j) for i, j in cells if wet[j, i]]
+    pick = np.round(np.linspace(0, len(cells) - 1, G)).astype(int)
+    I = np.array([cells[k][0] + 1 for k in pick], dtype=np.int32)
+    J = np.array([cells[k][1] + 1 for k in pick], dtype=np.int32)
+    return I, J
 
 
 class Site:
     """Terrain, child grid and nest maps of the site (built once)."""
 
     def __init__(self):
-        base = synth.shelf_coast(PNX, PNY, SITE_SEED, PDX, 4.0, 10e3, ground=P_GROUND,
+        base = synth.shelf_coast(PNX, PNY, SITE_SEED, PDX, AMP[1], SRC_LEN[0], ground=P_GROUND,
                                  crough=CROUGH, land=plain)
         self.dt = base.dt                           # CFL step for the largest source amplitude
+        self.dt_child = self.dt / CSTEPS
         self.dep, self.ground = base.dep, base.ground
         self.x, self.y = base.xcrd, base.ycrd
         self.xc = np.linspace(self.x[CI0], self.x[CI1], REFINE * (CI1 - CI0) + 1)
@@ -231,16 +304,14 @@ class Site:
         self.maps = cliffs_gpu.nest_maps(self.x, self.y, self.xc, self.yc)
         self.Xc, self.Yc = np.meshgrid(self.xc, self.yc)
         self.rows = None                            # tuned tile heights (parent, child)
-        # gauges: parent cells on the child's west edge (1-based)
-        self.gI = np.full(G, CI0 + 1, dtype=np.int32)
-        self.gJ = np.round(np.linspace(CJ0 + 1, CJ1 + 1, G)).astype(np.int32)
+        self.gI, self.gJ = _perimeter_gauges(self.dep > 5.0)
         # ground rise from one land cell to the next across the coast (median
         # over the window): the step in which the solver's run-up moves
         wg = _window(np.where(self.depc < 0, -self.depc, np.nan))
         self.land_step = float(np.nanmedian(np.abs(np.diff(wg, axis=1))))
-        # lowest land ground height per map pixel (inf where the pixel is all sea)
-        w = _window(np.where(self.depc < 0, -self.depc, np.inf))
-        self.pix_ground_min = w.reshape(MAP, MAP_H // MAP, MAP, MAP_W // MAP).min(axis=(1, 3))
+        # ground height (FP32, as the maps hold it) of the window's land cells,
+        # inf at sea
+        self.ground_win = _window(np.where(self.depc < 0, -self.depc, np.inf)).astype(np.float32)
 
 
 def site():
@@ -265,17 +336,17 @@ def _bilinear(f, x, y, xq, yq):
 
 
 def _window(a):
-    return a[MAP_J0:MAP_J0 + MAP_H, MAP_I0:MAP_I0 + MAP_W]
+    return a[MAP_J0:MAP_J0 + WIN, MAP_I0:MAP_I0 + WIN]
 
 
 def scenario(seed):
-    """The site with one source: amplitude 0.5-4 m, length 8-30 km, seeded
-    strike and place (x in 0.3-0.5 of the parent's width)."""
+    """The site with one source: amplitude AMP, length SRC_LEN, seeded
+    strike and place (x in SRC_X of the parent's width)."""
     S = site()
     rng = np.random.default_rng(50_000 + seed)
-    return synth.shelf_coast(PNX, PNY, SITE_SEED, PDX, float(rng.uniform(0.5, 4.0)),
-                             float(rng.uniform(8e3, 30e3)), steps=EVERY * T,
-                             source_seed=60_000 + seed, source_x=(0.3, 0.5), dt=S.dt,
+    return synth.shelf_coast(PNX, PNY, SITE_SEED, PDX, float(rng.uniform(*AMP)),
+                             float(rng.uniform(*SRC_LEN)), steps=EVERY * T,
+                             source_seed=60_000 + seed, source_x=SRC_X, dt=S.dt,
                              ground=P_GROUND, crough=CROUGH, land=plain)
 
 
@@ -287,14 +358,38 @@ def _parent(case, device):
     return g
 
 
+def flood_levels(maps):
+    """The network's map target: per row of the map window, the flood level
+    (the highest ground a flooded cell of that row has, 0 where none),
+    broadcast along the row -> (n, MAP, MAP).  The ground rises inland along
+    every row, so the map is recovered from its levels (map_from_levels)."""
+    m = np.asarray(maps, dtype=np.float32).reshape(-1, MAP, MAP)
+    return np.broadcast_to(m.max(axis=2, keepdims=True), m.shape).astype(np.float32)
+
+
+def map_from_levels(levels):
+    """Inundation-height maps (raster or quadtree order, like `levels`) from
+    flood-level fields: the ground height of every land cell at or below the
+    level, 0 elsewhere."""
+    lv = np.asarray(levels, dtype=np.float32)
+    g = site().ground_win if lv.shape[-1] == MAP else _ground_q()
+    return np.where(g <= lv, g, 0.0).astype(np.float32)
+
+
+def _ground_q():
+    S = site()
+    if getattr(S, "ground_q", None) is None:
+        S.ground_q = to_quadtree(S.ground_win)[0]
+    return S.ground_q
+
+
 def inundation_map(hmax, depc, celmin):
-    """Inundation height: the ground height (-dep) of land cells the water
-    reached (celerity maximum above celmin, as synth.max_runup counts them),
-    0 elsewhere; block maximum over 10 x 2 child cells of the coastal window
-    -> 64 x 64.  The map's maximum is the run-up over the window."""
+    """Inundation height over the map window, one pixel per child cell: the
+    ground height (-dep) of land cells the water reached (celerity maximum
+    above celmin, as synth.max_runup counts them), 0 elsewhere.  The map's
+    maximum is the run-up over the window."""
     e = np.where((hmax > celmin) & (depc < 0.0), -depc, 0.0)
-    e = _window(e)
-    return e.reshape(MAP, MAP_H // MAP, MAP, MAP_W // MAP).max(axis=(1, 3))
+    return _window(e).astype(np.float32)
 
 
 def runup_from_map(maps):
@@ -313,115 +408,174 @@ def _tuned_rows(grid):
     return rows
 
 
+class NestedSolve:
+    """The full nested solve of one scenario, parent and child on the device
+    (cliffs_nest_couple, cliffs_nest_run), EVERY * T parent steps; tile
+    heights tuned once per site.  advance(k) runs k record intervals."""
+
+    def __init__(self, case, device=0):
+        S = site()
+        self.case = case
+        self.g = _parent(case, device)
+        self.c = cliffs_gpu.Grid(S.depc, *S.cmetrics, dt=S.dt_child, ground=C_GROUND,
+                                 crough=case.crough, device=device)
+        z = np.zeros_like(S.depc)
+        src = case.meta["source"]
+        self.c.upload(synth.depth_to_celerity(S.depc + synth.dipole(S.Xc, S.Yc, S.depc, src), C_GROUND),
+                      z, z)
+        if S.rows is None:
+            S.rows = (_tuned_rows(self.g), _tuned_rows(self.c))
+        self.g.set_tile_rows(S.rows[0])
+        self.c.set_tile_rows(S.rows[1])
+        self.n = cliffs_gpu.Nest(self.g, S.maps)
+        self.n.couple(self.c, capacity=4)
+        self.iarv = False
+        self.done = 0
+
+    def advance(self, k):
+        S = site()
+        self.iarv = self.n.run(k, EVERY, S.dt, EVERY * CSTEPS, S.dt_child, CUKE, self.case.ground,
+                               self.iarv)
+        self.done += k
+
+    def hmax(self):
+        return self.c.download_max()[0]
+
+    def close(self):
+        self.n.close()
+        self.c.close()
+        self.g.close()
+
+
 def nested_hmax(case, device=0):
-    """The full nested solve of one scenario: parent and child on the device
-    (cliffs_nest_couple/record/child_step), EVERY * T parent steps, tile
-    heights tuned once per site.  -> (child hmax, child celmin)."""
-    S = site()
-    g = _parent(case, device)
-    c = cliffs_gpu.Grid(S.depc, *S.cmetrics, dt=case.dt / REFINE, ground=C_GROUND,
-                        crough=case.crough, device=device)
-    src = case.meta["source"]
-    z = np.zeros_like(S.depc)
-    c.upload(synth.depth_to_celerity(S.depc + synth.dipole(S.Xc, S.Yc, S.depc, src), C_GROUND), z, z)
-    if S.rows is None:
-        S.rows = (_tuned_rows(g), _tuned_rows(c))
-    g.set_tile_rows(S.rows[0])
-    c.set_tile_rows(S.rows[1])
-    n = cliffs_gpu.Nest(g, S.maps)
-    n.couple(c, capacity=4)
-    dtc, per = case.dt / REFINE, REFINE * EVERY      # child steps per record
-    trec, iarv, kc = [], False, 0
-    for k in range(T):
-        g.run(EVERY)
-        tp = (k + 1) * EVERY * case.dt
-        iarv, nrec = n.record(CUKE, case.ground, tp, iarv)
-        if nrec > len(trec):
-            trec.append(tp)
-        while kc + 1 < per * (len(trec) - 1):
-            kc += 1
-            j = kc // per
-            n.child_step(trec[j] + (kc - j * per) * dtc)
-    hmax, _ = c.download_max()
-    cm = c.celmin
-    n.close()
-    c.close()
-    g.close()
+    """The full nested solve of one scenario -> (child hmax, child celmin)."""
+    ns = NestedSolve(case, device)
+    ns.advance(T)
+    hmax, cm = ns.hmax(), ns.c.celmin
+    ns.close()
     return hmax, cm
 
 
 def nested_solve(case, device=0):
-    """-> (child inundation map (64, 64), exact max run-up over the map
+    """-> (child inundation map (512, 512), exact max run-up over the map
     window (m)) of the full nested solve (nested_hmax)."""
     S = site()
     hmax, cm = nested_hmax(case, device)
     return inundation_map(hmax, S.depc, cm), synth.max_runup(_window(hmax), _window(S.depc), cm)
 
 
+_pools = {}
+
+
 def parent_gauges(cases, device=0):
     """Parents of all `cases` stepped together (cliffs_ensemble_*), the
-    gauge series recorded on the device: -> (n, T, G) in m."""
+    gauge series recorded on the device: -> (n, T, G) in m.  The parents of
+    the site differ only in their initial state, so the grids and their
+    ensemble (tile shape tuned once, cliffs_ensemble_tune) are kept per
+    batch size and device, and each batch uploads its states and resets the
+    maxima."""
     S = site()
-    grids = [_parent(c, device) for c in cases]
-    ens = cliffs_gpu.Ensemble(grids)
-    ens.tune(3)
-    out = ens.run_gauges(EVERY, T, S.gI, S.gJ)
-    ens.close()
-    for g in grids:
-        g.close()
-    return out
+    key = (len(cases), device)
+    if key not in _pools:
+        grids = [_parent(c, device) for c in cases]
+        ens = cliffs_gpu.Ensemble(grids)
+        ens.tune(3)
+        _pools[key] = (grids, ens)
+    else:
+        grids, ens = _pools[key]
+        for g, c in zip(grids, cases):
+            g.upload(c.cel, c.xvel, c.yvel)
+            g.reset_max()
+    return ens.run_gauges(EVERY, T, S.gI, S.gJ)
 
 
-def dataset(seeds, batch=64, device=0):
-    """Network inputs (N, T, G), quadtree targets (N, 4096), maps and exact
-    run-ups, for the scenarios `seeds`."""
-    xs, maps, runups = [], [], []
+def release_parents():
+    """Free the kept parent grids and ensembles."""
+    for grids, ens in _pools.values():
+        ens.close()
+        for g in grids:
+            g.close()
+    _pools.clear()
+
+
+def dataset(seeds, batch=64, solvers=8, device=0, log=None):
+    """For the scenarios `seeds`: network inputs x (N, T, G), quadtree
+    level-field targets y (N, NOUT, m; flood_levels), exact run-ups ru (N,),
+    flooded cells area (N,) and, per scenario, the cells where the map
+    rebuilt from its own levels differs from the solver's map (rec).  The
+    parents of each batch are stepped as one ensemble, the nested solves run
+    `solvers` at a time from host threads (each on its own grids and
+    streams; the library's calls release the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
     seeds = list(seeds)
-    for b0 in range(0, len(seeds), batch):
+    n = len(seeds)
+    x = np.empty((n, T, G), dtype=np.float32)
+    yq = np.empty((n, NOUT), dtype=np.float32)
+    ru = np.empty(n)
+    area = np.empty(n, dtype=np.int64)
+    rec = np.empty(n, dtype=np.int64)
+    for b0 in range(0, n, batch):
         cases = [scenario(int(s)) for s in seeds[b0:b0 + batch]]
-        xs.append(parent_gauges(cases, device) * X_SCALE)
-        for c in cases:
-            m, r = nested_solve(c, device)
-            maps.append(m)
-            runups.append(r)
-    maps = np.array(maps, dtype=np.float32)
-    return (np.concatenate(xs).astype(np.float32), to_quadtree(maps * Y_SCALE), maps,
-            np.array(runups))
+        x[b0:b0 + len(cases)] = parent_gauges(cases, device) * X_SCALE
+        with ThreadPoolExecutor(max_workers=solvers) as ex:
+            out = list(ex.map(lambda c: nested_solve(c, device), cases))
+        for k, (m, r) in enumerate(out):
+            lv = flood_levels(m)
+            yq[b0 + k] = to_quadtree(lv)[0]
+            ru[b0 + k] = r
+            area[b0 + k] = int((m > 0).sum())
+            rec[b0 + k] = int((map_from_levels(lv)[0] != m).sum())
+        if log:
+            log(b0 + len(cases), n)
+    return dict(x=x, y=yq, ru=ru, area=area, rec=rec)
 
 
-def predict_maps(net, x):
-    return np.maximum(from_quadtree(net.forward(x)) / Y_SCALE, 0.0)
+def predict_runups(nets, x):
+    """Each network's run-up (m, clipped at 0) for inputs x: the gate's input."""
+    return list(predict_all(nets, x, maps=False)[0])
+
+
+def predict_levels(nets, x):
+    """The networks' mean flood-level field (quadtree order) for inputs x."""
+    acc = None
+    for net in nets:
+        m, _ = net.forward(x)
+        acc = m if acc is None else acc + m
+    return acc / len(nets)
 
 
 def answer(cases, nets, tol, device=0):
-    """Max-inundation maps of `cases` at the site: the parents stepped as one
-    ensemble give the gauge series; where the networks' run-ups agree within
-    tol (gate) the mean of their maps is the answer, elsewhere the full
-    nested solve runs.  -> (maps (n, 64, 64), answered by the surrogate (n,))."""
+    """Max-inundation maps and run-ups of `cases` at the site: the parents
+    stepped as one ensemble give the gauge series; where the networks'
+    run-ups agree within tol (gate) the answer is the mean of their run-ups
+    and the map of their mean flood levels (built on the device), elsewhere
+    the full nested solve runs.  -> (maps (n, 512, 512), run-ups (n,),
+    answered by the surrogate (n,))."""
     x = parent_gauges(cases, device) * X_SCALE
-    preds = [predict_maps(n, x) for n in nets]
-    ok = gate(*preds, tol=tol)
-    maps = np.mean(preds, axis=0).astype(np.float64)
+    rus, _ = predict_all(nets, x, maps=False)
+    ok = gate(list(rus), tol=tol)
+    runups = rus.mean(axis=0)
+    maps = np.zeros((len(cases), MAP, MAP), dtype=np.float32)
+    if ok.any():
+        maps[ok] = predict_all(nets, x[ok])[1]
     for k in np.flatnonzero(~ok):
-        maps[k] = nested_solve(cases[k], device)[0]
-    return maps, ok
+        maps[k], runups[k] = nested_solve(cases[k], device)
+    return maps, runups, ok
 
 
-def flooded_area(maps):
-    """Pixels with flooded land: inundation height at or above the pixel's
-    lowest land cell."""
-    S = site()
-    return (np.asarray(maps) >= S.pix_ground_min).sum(axis=(-2, -1))
+def flooded_cells(qlevels):
+    """Flooded land cells of the maps of quadtree-ordered flood-level fields."""
+    return (_ground_q() <= np.asarray(qlevels)).sum(axis=-1)
 
 
-def gate_score(*maps):
-    """Disagreement of networks' run-ups (map maxima): their spread (largest
-    minus smallest) relative to their mean, or to RUNUP_MIN where the mean
-    is smaller."""
-    r = np.stack([runup_from_map(m) for m in maps])
+def gate_score(runups):
+    """Disagreement of the networks' run-ups (list of (n,) arrays): their
+    spread (largest minus smallest) relative to their mean, or to RUNUP_MIN
+    where the mean is smaller."""
+    r = np.stack(runups)
     return (r.max(axis=0) - r.min(axis=0)) / np.maximum(r.mean(axis=0), RUNUP_MIN)
 
 
-def gate(*maps, tol):
+def gate(runups, tol):
     """Confidence gate: the networks' run-ups agree within tol."""
-    return gate_score(*maps) <= tol
+    return gate_score(runups) <= tol

@@ -1,21 +1,23 @@
 """Train and evaluate the learned nest surrogate on solver-generated scenarios.
 
-    python3 cliffs-src_amd/surrogate_study.py [--train 896] [--calib 128] [--test 128]
+    python3 cliffs-src_amd/surrogate_study.py [--train 4096] [--calib 512] [--test 512]
 
 Scenarios are sources at one nested site (surrogate.Site).  For each, the
 parents of all scenarios are stepped as an ensemble with gauge recording
 (the network's input) and the full nested solve (parent + coupled child) is
-run for the truth.  Several networks (--nets) with different seeds are trained on the
-MFMA kernels, with early stopping on the calibration split; the gate's
-tolerance is calibrated on the same split for each run-up error budget;
-everything is reported on the held-out test split.  A scenario is answered
-by the surrogate when the networks' run-ups agree (surrogate.gate), otherwise by
-the full nested solve.
+run for the truth.  Of the --train training runs, the last --calib are held
+back from the fit for early stopping and for calibrating the gate; the
+--test runs are held out from both.  Several networks (--nets) with
+different seeds are trained on the MFMA kernels.  A scenario is answered by
+the surrogate when the networks' run-ups agree (surrogate.gate), otherwise
+by the full nested solve; the gate's tolerance is calibrated per run-up
+error budget on the calibration runs and everything is reported on the
+held-out runs.
 
-Prints one JSON line: map error against the full nested solve (and against
-the training mean map, the method without the network), run-up error, and
-per run-up error budget the gate's acceptance and scenarios/hr with the
-surrogate against the nested solve alone, with the time of each leg.
+Prints one JSON line: run-up and map error against the full nested solve
+(and of the training mean, the method without the network), and per run-up
+error budget the gate's acceptance and scenarios/hr with the surrogate
+against the nested solve alone, with the time of each leg.
 """
 import argparse
 import json
@@ -29,116 +31,159 @@ sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 
 import surrogate as S  # noqa: E402
 
-RUNUP_MIN = S.RUNUP_MIN   # relative run-up errors are quoted for run-ups >= 1 m
-BUDGETS = (0.02, 0.05, 0.1, 0.2, 0.5)   # run-up error budgets (95th percentile)
+BUDGETS = (0.02, 0.05, 0.1, 0.2)   # run-up error budgets (95th percentile of the accepted)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def solve_split(name, seeds, batch, solvers):
-    """Inputs from the ensemble parents, truth from nested solves run
-    `solvers` at a time from host threads (each on its own grids and
-    streams; the library's calls release the interpreter lock).  The nested
-    solve's cost per scenario is the wall time of a batch over its size."""
-    from concurrent.futures import ThreadPoolExecutor
-    xs, maps, runups, t_par, t_nest = [], [], [], 0.0, []
-    seeds = list(seeds)
-    for b0 in range(0, len(seeds), batch):
-        cases = [S.scenario(int(s)) for s in seeds[b0:b0 + batch]]
-        t0 = time.perf_counter()
-        xs.append(S.parent_gauges(cases) * S.X_SCALE)
-        t_par += time.perf_counter() - t0
-        t0 = time.perf_counter()
-        with ThreadPoolExecutor(max_workers=solvers) as ex:
-            out = list(ex.map(S.nested_solve, cases))
-        t_nest += [(time.perf_counter() - t0) / len(cases)] * len(cases)
-        for m, r in out:
-            maps.append(m)
-            runups.append(r)
-        log(f"[study] {name}: {min(b0 + batch, len(seeds))}/{len(seeds)} scenarios, "
-            f"nested solve {np.mean(t_nest):.3f} s, parents {t_par / len(t_nest) * 1e3:.2f} ms per scenario")
-    maps = np.array(maps, dtype=np.float32)
-    return dict(x=np.concatenate(xs).astype(np.float32), y=S.to_quadtree(maps * S.Y_SCALE), maps=maps,
-                runup=np.array(runups), t_parent=t_par / len(seeds), t_nested=np.array(t_nest))
-
-
-def rmse(pred, maps):
-    return np.sqrt(((pred - maps) ** 2).mean(axis=(1, 2)))
-
-
 def runup_err(est, true):
     """Relative error for run-ups >= RUNUP_MIN, absolute / RUNUP_MIN below."""
-    return np.abs(est - true) / np.maximum(true, RUNUP_MIN)
+    return np.abs(est - true) / np.maximum(true, S.RUNUP_MIN)
 
 
-def train_early_stop(seed, tr, ca, steps, chunk, batch, lr):
+def make_split(name, seeds, ens, solvers):
+    t0 = time.perf_counter()
+    d = S.dataset(seeds, batch=ens, solvers=solvers,
+                  log=lambda k, n: log(f"[study] {name}: {k}/{n} scenarios, "
+                                       f"{time.perf_counter() - t0:.0f} s") if k % 512 == 0 or k == n else None)
+    d["t"] = (time.perf_counter() - t0) / len(seeds)
+    return d
+
+
+def train_early_stop(seed, fit, cal, steps, chunk, batch, lr, w_map, w_runup):
     net = S.Surrogate(batch_max=max(batch, 64), seed=seed)
+    net.set_data(fit["x"], fit["y"], fit["ru"])
     best, best_p, hist = np.inf, None, []
     for k in range(0, steps, chunk):
-        net.train(tr["x"], tr["y"], batch=batch, steps=chunk, lr=lr, seed=100 * seed + k)
-        e = float(rmse(S.predict_maps(net, ca["x"]), ca["maps"]).mean())
+        net.train(batch=batch, steps=chunk, lr=lr, seed=100 * seed + k, w_map=w_map, w_runup=w_runup)
+        [r] = S.predict_runups([net], cal["x"])
+        e = float(np.abs(r - cal["ru"]).mean())
         hist.append(round(e, 4))
         if e < best:
             best, best_p = e, net.params()
     net.set_params(best_p)
-    log(f"[study] network {seed}: calibration map RMSE by chunk {hist}")
+    log(f"[study] network {seed}: calibration run-up MAE (m) by chunk {hist}")
     return net, hist
+
+
+# GEMM FLOPs per sample of the network's forward pass (the layer table of
+# csrc/surrogate.hip: compute rows x 2 K N per layer; the encoder's compute
+# rows include the few per sample whose windows are dropped)
+FWD_GEMMS = [(128, 256, 128), (132, 640, 128), (66, 640, 256), (34, 1280, 256), (1, 512, 512),
+             (1, 512, 512), (1, 512, 1), (1, 512, 4096), (64, 64, 128), (256, 32, 64), (1024, 16, 32),
+             (4096, 8, 32), (16384, 8, 16), (65536, 4, 4)]
+FWD_FLOPS = sum(2 * r * k * n for r, k, n in FWD_GEMMS)
+
+
+def synthetic(steps, batch):
+    """The network's kernels alone, on random data of the stated shapes (no
+    solver): training steps and inference timed on the device's clock of the
+    host; one JSON line with achieved TFLOP/s (forward 3 x FWD_FLOPS per
+    sample in training: forward, weight and input gradients)."""
+    rng = np.random.default_rng(0)
+    n = 512
+    x = rng.normal(0, 0.5, (n, S.T, S.G)).astype(np.float32)
+    y = np.abs(rng.normal(0, 0.2, (n, S.NOUT))).astype(np.float32)
+    yr = rng.uniform(0, 1, n).astype(np.float32)
+    net = S.Surrogate(batch_max=max(batch, 64), seed=1)
+    net.set_data(x, y, yr)
+    net.train(batch=batch, steps=20, lr=5e-4, seed=1)                # warm-up
+    t0 = time.perf_counter()
+    net.train(batch=batch, steps=steps, lr=5e-4, seed=2)
+    t_step = (time.perf_counter() - t0) / steps
+    nets = [net]
+    S.predict_all(nets, x[:64], maps=False)
+    t0 = time.perf_counter()
+    S.predict_all(nets, x, maps=False)
+    t_gate = (time.perf_counter() - t0) / n
+    net.set_ground(np.where(np.arange(S.NOUT).reshape(S.MAP, S.MAP) % 3 == 0, 0.5, np.inf))
+    nets[0]._ground = True
+    t0 = time.perf_counter()
+    S.predict_all(nets, x)
+    t_map = (time.perf_counter() - t0) / n
+    net.close()
+    print(json.dumps({"batch": batch, "steps": steps, "train_step_ms": round(t_step * 1e3, 3),
+                      "train_tflops": round(3 * FWD_FLOPS * batch / t_step / 1e12, 3),
+                      "forward_gflop_per_sample": round(FWD_FLOPS / 1e9, 4),
+                      "runup_infer_ms_per_sample": round(t_gate * 1e3, 4),
+                      "map_infer_ms_per_sample": round(t_map * 1e3, 4)}), flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--train", type=int, default=896)
-    ap.add_argument("--calib", type=int, default=128)
-    ap.add_argument("--test", type=int, default=128)
-    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--train", type=int, default=4096)
+    ap.add_argument("--calib", type=int, default=512)
+    ap.add_argument("--test", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--chunk", type=int, default=250)
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--lr", type=float, default=5e-4)
+    ap.add_argument("--w-map", type=float, default=S.W_MAP)
+    ap.add_argument("--w-runup", type=float, default=S.W_RUNUP)
     ap.add_argument("--ens", type=int, default=64, help="parents per ensemble")
     ap.add_argument("--nets", type=int, default=4, help="networks trained from different seeds")
-    ap.add_argument("--solvers", type=int, default=4, help="nested solves run at a time")
+    ap.add_argument("--solvers", type=int, default=8, help="nested solves run at a time")
+    ap.add_argument("--save", default="", help="write per-scenario run-ups (npz) here")
+    ap.add_argument("--synthetic", type=int, default=0, metavar="STEPS",
+                    help="only time the network's kernels on random data for STEPS training steps")
     args = ap.parse_args()
+    if args.synthetic:
+        synthetic(args.synthetic, args.batch)
+        return
 
     t0 = time.perf_counter()
     S.nested_solve(S.scenario(0))        # tunes the site's tile heights once
-    tr = solve_split("train", range(args.train), args.ens, args.solvers)
-    ca = solve_split("calib", range(100_000, 100_000 + args.calib), args.ens, args.solvers)
-    te = solve_split("test", range(200_000, 200_000 + args.test), args.ens, args.solvers)
+    t1 = time.perf_counter()
+    S.nested_solve(S.scenario(1))
+    t_alone = time.perf_counter() - t1
+    tr = make_split("train", range(args.train), args.ens, args.solvers)
+    te = make_split("test", range(200_000, 200_000 + args.test), args.ens, args.solvers)
     t_gen = time.perf_counter() - t0
+    nfit = args.train - args.calib
+    fit = {k: tr[k][:nfit] for k in ("x", "y", "ru")}
+    cal = {k: tr[k][nfit:] for k in ("x", "y", "ru")}
 
     nets, hists = [], []
     t0 = time.perf_counter()
     for k in range(1, args.nets + 1):
-        net, h = train_early_stop(k, tr, ca, args.steps, args.chunk, args.batch, args.lr)
+        net, h = train_early_stop(k, fit, cal, args.steps, args.chunk, args.batch, args.lr, args.w_map,
+                                  args.w_runup)
         nets.append(net)
         hists.append(h)
     t_train = time.perf_counter() - t0
+    del fit
 
-    def infer(split):
-        t0 = time.perf_counter()
-        preds = [S.predict_maps(n, split["x"]) for n in nets]
-        t = (time.perf_counter() - t0) / len(split["x"])
-        return preds, t
+    # per-scenario cost of the surrogate path on the held-out scenarios: the
+    # parents stepped as ensembles with gauge recording, the networks'
+    # run-up heads for the gate, and for an accepted scenario the map from
+    # the networks' mean flood levels
+    cases = [S.scenario(s) for s in range(200_000, 200_000 + min(args.test, 256))]
+    t0 = time.perf_counter()
+    for b0 in range(0, len(cases), args.ens):
+        S.parent_gauges(cases[b0:b0 + args.ens])
+    t_parent = (time.perf_counter() - t0) / len(cases)
+    t0 = time.perf_counter()
+    rus = list(S.predict_all(nets, te["x"], maps=False)[0])
+    t_gate = (time.perf_counter() - t0) / len(te["x"])
+    t0 = time.perf_counter()
+    _, maps_p = S.predict_all(nets, te["x"])
+    t_map = (time.perf_counter() - t0) / len(te["x"])
+    rus_c = list(S.predict_all(nets, cal["x"], maps=False)[0])
+    t_nested = float(te["t"])        # nested solves, `solvers` at a time
 
-    # gate calibration per run-up error budget: the loosest tolerance whose
-    # accepted calibration scenarios keep the 95th percentile of the run-up
-    # error within the budget; then applied to the test split
-    preds_c, _ = infer(ca)
-    err_c = runup_err(S.runup_from_map(np.mean(preds_c, axis=0)), ca["runup"])
-    dis_c = S.gate_score(*preds_c)
-    preds, t_net = infer(te)
-    pred = np.mean(preds, axis=0)
-    dis_t = S.gate_score(*preds)
-    ru_pred = S.runup_from_map(pred)
-    err = runup_err(ru_pred, te["runup"])
-    t_nested = float(te["t_nested"].mean())
+    ru_c = np.mean(rus_c, axis=0)
+    err_c = runup_err(ru_c, cal["ru"])
+    dis_c = S.gate_score(rus_c)
+    ru_t = np.mean(rus, axis=0)
+    err = runup_err(ru_t, te["ru"])
+    dis_t = S.gate_score(rus)
 
     def gate_curve(e_cal, e_test, budgets):
         """Per budget: the loosest gate tolerance whose accepted calibration
-        scenarios keep the 95th percentile of the error e_cal within the
-        budget, applied to the test split."""
+        scenarios keep the 95th percentile of the error within the budget,
+        applied to the held-out runs."""
         curve = []
         for budget in budgets:
             tol = -1.0
@@ -147,53 +192,71 @@ def main():
                 if np.quantile(e_cal[ok], 0.95) <= budget:
                     tol = float(t)
             ok = dis_t <= tol
-            per_with = te["t_parent"] + t_net + (1.0 - ok.mean()) * t_nested
+            per_with = t_parent + t_gate + ok.mean() * t_map + (1.0 - ok.mean()) * t_nested
             curve.append({"budget": budget, "gate_tol": round(tol, 4), "accept": round(float(ok.mean()), 3),
                           "err_p95_accepted": round(float(np.quantile(e_test[ok], 0.95)), 4) if ok.any() else None,
+                          "err_max_accepted": round(float(e_test[ok].max()), 4) if ok.any() else None,
                           "speedup": round(t_nested / per_with, 2),
                           "scenarios_per_hr": round(3600.0 / per_with, 1)})
         return curve
 
     curve = gate_curve(err_c, err, BUDGETS)
-    # the solver's run-up is the ground height of its highest flooded cell, so
-    # it moves in steps of one land cell's rise; budgets in those steps
     cell = S.site().land_step
-    cells_c = np.abs(S.runup_from_map(np.mean(preds_c, axis=0)) - ca["runup"]) / cell
-    cells_t = np.abs(ru_pred - te["runup"]) / cell
-    curve_cells = gate_curve(cells_c, cells_t, (0.5, 1.0, 2.0))
-    ru_map = S.runup_from_map(te["maps"])            # the run-up reading alone, on the true map
-    err_map = runup_err(ru_map, te["runup"])
-    mean_map = tr["maps"].mean(axis=0)
-    r_net = rmse(pred, te["maps"])
-    r_mean = rmse(np.broadcast_to(mean_map, te["maps"].shape), te["maps"])
-    err_mean = runup_err(S.runup_from_map(mean_map[None]).repeat(len(pred)), te["runup"])
-    area_t, area_p = S.flooded_area(te["maps"]), S.flooded_area(pred)
+    cells_c = np.abs(ru_c - cal["ru"]) / cell
+    cells_t = np.abs(ru_t - te["ru"]) / cell
+    curve_cells = gate_curve(cells_c, cells_t, (1.0, 2.0, 4.0))
+    ru_mean = float(tr["ru"][:nfit].mean())
+    err_mean = runup_err(np.full_like(te["ru"], ru_mean), te["ru"])
+    # maps: the test maps rebuilt from their own levels (exact where the
+    # ground rises inland along every row), against those from the predicted
+    # levels and from the training mean level field
+    maps_t = S.map_from_levels(S.from_quadtree(te["y"]))
+    mean_lev = S.map_from_levels(S.from_quadtree(tr["y"][:nfit].mean(axis=0)))[0]
+    r_net = np.sqrt(((maps_p - maps_t) ** 2).mean(axis=(1, 2)))
+    r_mean = np.sqrt(((mean_lev[None] - maps_t) ** 2).mean(axis=(1, 2)))
+    area_t = te["area"]
+    area_p = (maps_p > 0).sum(axis=(1, 2))
+    area_m = int((mean_lev > 0).sum())
+    runup_map_p = maps_p.reshape(len(maps_p), -1).max(axis=1)
+    del maps_p, maps_t
 
     def q(a, p):
         return round(float(np.quantile(a, p)), 4) if len(a) else None
 
     out = {
-        "site": {"parent": [S.PNX, S.PNY, S.PDX], "refine": S.REFINE, "child_cells": int(S.site().depc.size),
-                 "parent_steps": S.EVERY * S.T, "gauges": S.G, "samples": S.T},
-        "networks": args.nets, "solvers": args.solvers, "train_scenarios": args.train, "calib_scenarios": args.calib, "test_scenarios": args.test,
-        "train_steps_max": args.steps, "calib_rmse_by_chunk": hists,
-        "nested_solve_s": round(t_nested, 4), "ensemble_parent_s": round(te["t_parent"], 5),
-        "surrogate_infer_s": round(t_net, 6),
+        "site": {"parent": [S.PNX, S.PNY, S.PDX], "refine": S.REFINE, "child_steps_per_parent_step": S.CSTEPS,
+                 "child_cells": int(S.site().depc.size), "parent_steps": S.EVERY * S.T, "gauges": S.G,
+                 "samples": S.T, "map": [S.MAP, S.MAP]},
+        "networks": args.nets, "params": nets[0].nparams, "solvers": args.solvers,
+        "train_runs": args.train, "fit_runs": nfit, "calib_runs": args.calib, "test_runs": args.test,
+        "train_steps_max": args.steps, "loss_weights": [args.w_map, args.w_runup],
+        "calib_runup_mae_by_chunk": hists,
+        "nested_solve_s": round(t_nested, 4), "nested_solve_alone_s": round(t_alone, 4),
+        "ensemble_parent_s": round(t_parent, 5), "gate_infer_s": round(t_gate, 6),
+        "map_infer_s": round(t_map, 6),
         "scenarios_per_hr_nested": round(3600.0 / t_nested, 1),
+        "ceiling_speedup": round(t_nested / (t_parent + t_gate + t_map), 2),
         "gate_curve": curve,
         "land_cell_rise_m": round(float(cell), 4), "gate_curve_land_cells": curve_cells,
-        "runup_within_one_cell_all": round(float((cells_t <= 1.0).mean()), 3),
-        "runup_median_m": round(float(np.median(te["runup"])), 3),
-        "map_rmse_m": round(float(r_net.mean()), 4),
-        "map_rmse_mean_map_m": round(float(r_mean.mean()), 4),
-        "runup_err_median_all": q(err, 0.5), "runup_err_p95_all": q(err, 0.95),
-        "runup_within_2pct_all": round(float((err <= 0.02).mean()), 3),
-        "runup_err_median_mean_map": q(err_mean, 0.5),
-        "runup_err_median_true_map": q(err_map, 0.5), "runup_err_p95_true_map": q(err_map, 0.95),
+        "runup_m_quantiles_test": [q(te["ru"], p) for p in (0.0, 0.05, 0.5, 0.95, 1.0)],
+        "runup_zero_share_test": round(float((te["ru"] == 0).mean()), 3),
+        "runup_err_median": q(err, 0.5), "runup_err_p95": q(err, 0.95),
+        "runup_within_2pct": round(float((err <= 0.02).mean()), 3),
+        "runup_within_one_cell": round(float((cells_t <= 1.0).mean()), 3),
+        "runup_err_median_mean": q(err_mean, 0.5), "runup_within_2pct_mean": round(float((err_mean <= 0.02).mean()), 3),
+        "runup_from_map_err_median": q(runup_err(runup_map_p, te["ru"]), 0.5),
+        "map_rmse_m": round(float(r_net.mean()), 5), "map_rmse_mean_map_m": round(float(r_mean.mean()), 5),
+        "map_cells_not_rebuilt_from_levels_median": q(te["rec"], 0.5),
+        "map_cells_not_rebuilt_from_levels_max": int(te["rec"].max()),
         "flooded_area_rel_err_median": q(np.abs(area_p - area_t)[area_t > 0] / area_t[area_t > 0], 0.5),
+        "flooded_area_rel_err_p95": q(np.abs(area_p - area_t)[area_t > 0] / area_t[area_t > 0], 0.95),
+        "flooded_area_rel_err_median_mean": q(np.abs(area_m - area_t)[area_t > 0] / area_t[area_t > 0], 0.5),
         "data_generation_s": round(t_gen, 1), "training_s": round(t_train, 1),
     }
     print(json.dumps(out), flush=True)
+    if args.save:
+        np.savez_compressed(args.save, ru_test=te["ru"], ru_pred=np.array(rus), ru_calib=cal["ru"],
+                            ru_pred_calib=np.array(rus_c), area_test=area_t, area_pred=area_p)
     for n in nets:
         n.close()
 

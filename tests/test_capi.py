@@ -112,3 +112,17 @@ def test_fortran_module_binds_every_header_function():
     with open(path) as f:
         bound = set(re.findall(r'bind\(C, name="(cliffs_\w+)"\)', f.read()))
     assert set(cliffs_gpu.header_symbols()) == bound
+
+
+def test_surrogate_header_exported_and_plain_c():
+    """include/cliffs_surrogate.h: plain C, every declared function exported."""
+    hdr = os.path.join(ROOT, "include", "cliffs_surrogate.h")
+    names = cliffs_gpu.header_symbols(hdr)
+    assert len(names) == 14 and "cliffs_sur_predict" in names and "cliffs_sur_set_data" in names
+    out = subprocess.run(["nm", "-D", "--defined-only", cliffs_gpu.LIB_PATH], check=True,
+                         capture_output=True, text=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    assert set(names) <= exported
+    src = '#include "%s"\nint main(void){return 0;}\n' % hdr
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-x", "c", "-", "-o", os.devnull],
+                   input=src, text=True, check=True)

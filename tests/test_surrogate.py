@@ -16,15 +16,16 @@ import surrogate_ref as R
 def _batch(rng, B):
     t = np.linspace(0, 1, S.T)[None, :, None]
     ph = rng.uniform(0, 2 * np.pi, (B, 1, S.G))
-    x = (np.sin(2 * np.pi * 3 * t + ph) * rng.uniform(0.2, 1.0, (B, 1, S.G))).astype(np.float32)
+    x = (np.sin(2 * np.pi * 5 * t + ph) * rng.uniform(0.2, 1.0, (B, 1, S.G))).astype(np.float32)
     y = rng.normal(0, 0.3, (B, S.NOUT)).astype(np.float32)
-    return x, y
+    yr = rng.normal(0, 1.0, B).astype(np.float32)
+    return x, y, yr
 
 
 def _params(seed=0):
     rng = np.random.default_rng(seed)
     ps = []
-    for _, i, n in R.LAYERS:
+    for i, n in R.SHAPES:
         ps += [rng.uniform(-1, 1, i * n) * np.sqrt(6.0 / i), rng.normal(0, 0.05, n)]
     return np.concatenate(ps).astype(np.float32)
 
@@ -41,7 +42,7 @@ def test_bf16_rounding_matches_torch():
 def _per_param(g_a, g_b):
     """(relative L2 error, cosine) of two gradient vectors, per weight and bias."""
     out, o = [], 0
-    for _, i, n in R.LAYERS:
+    for i, n in R.SHAPES:
         for k in (i * n, n):
             a, b = g_a[o:o + k].astype(np.float64), g_b[o:o + k].astype(np.float64)
             o += k
@@ -51,45 +52,65 @@ def _per_param(g_a, g_b):
 
 
 def test_reference_backprop_matches_autograd():
+    """The written-out backpropagation (convolution windows, time pool,
+    col2im, the two heads meeting in the trunk) equals torch autograd of the
+    same network without rounding; with the GPU's BF16 rounding points it
+    stays close (BF16 operands differ from FP32 by ~0.4 % each, over 14
+    layers)."""
     rng = np.random.default_rng(1)
-    x, y = _batch(rng, 2)
+    x, y, yr = _batch(rng, 2)
     p = _params(1)
-    l0, g0 = R.loss_grad_bf16(p, x, y, rounding=False)
-    l2, g2 = R.loss_grad_autograd(p, x, y)
+    l0, g0 = R.loss_grad_bf16(p, x, y, yr, 10.0, 1.0, rounding=False)
+    l2, g2 = R.loss_grad_autograd(p, x, y, yr, 10.0, 1.0)
     assert abs(l0 - l2) <= 1e-5 * l2
-    assert all(err <= 1e-4 for err, _ in _per_param(g0, g2))      # the written-out backprop
-    l1, g1 = R.loss_grad_bf16(p, x, y)                               # with BF16 operands
+    assert all(err <= 1e-5 for err, _ in _per_param(g0, g2))
+    l1, g1 = R.loss_grad_bf16(p, x, y, yr, 10.0, 1.0)
     assert abs(l1 - l2) <= 0.02 * l2
-    assert all(err <= 0.1 and cos >= 0.995 for err, cos in _per_param(g1, g2))
+    assert all(err <= 0.2 and cos >= 0.98 for err, cos in _per_param(g1, g2))
+
+
+def test_reference_planted_errors_fail_the_autograd_check():
+    """The autograd check catches plausible mistakes planted in the
+    restatement's backprop: the pool's maximum routed one time row late,
+    col2im summing the window gradients one input row off."""
+    rng = np.random.default_rng(2)
+    x, y, yr = _batch(rng, 2)
+    p = _params(2)
+    _, g2 = R.loss_grad_autograd(p, x, y, yr, 10.0, 1.0)
+    for plant, layers in (("pool_row", range(0, 8)), ("col2im", range(0, 6))):
+        _, gb = R.loss_grad_bf16(p, x, y, yr, 10.0, 1.0, rounding=False, plant=plant)
+        errs = [e for e, _ in _per_param(gb, g2)]
+        assert max(errs[l] for l in layers) > 1e-2, plant          # encoder gradients off
+        assert max(errs[8:]) <= 1e-5, plant                       # trunk and heads untouched
 
 
 def test_quadtree_layout():
     q = S.quadtree_order().reshape(S.MAP, S.MAP)
     assert sorted(q.ravel()) == list(range(S.NOUT))
-    # each 2x2 block is 4 consecutive outputs, each 8x8 block 64 consecutive
-    for by in range(0, 64, 2):
-        for bx in range(0, 64, 2):
-            blk = q[by:by + 2, bx:bx + 2].ravel()
-            assert blk.min() % 4 == 0 and blk.max() - blk.min() == 3
-    for by in range(0, 64, 8):
-        for bx in range(0, 64, 8):
-            blk = q[by:by + 8, bx:bx + 8].ravel()
-            assert blk.min() % 64 == 0 and blk.max() - blk.min() == 63
-    maps = np.random.default_rng(2).normal(size=(3, 64, 64)).astype(np.float32)
+    # each aligned 2^k x 2^k block is 4^k consecutive outputs
+    for k in (1, 3, 6):
+        w = 1 << k
+        for by in range(0, S.MAP, 64 if k < 6 else w):
+            for bx in range(0, S.MAP, 64 if k < 6 else w):
+                blk = q[by:by + w, bx:bx + w].ravel()
+                assert blk.min() % (w * w) == 0 and blk.max() - blk.min() == w * w - 1
+    maps = np.random.default_rng(2).normal(size=(2, S.MAP, S.MAP)).astype(np.float32)
     assert np.array_equal(S.from_quadtree(S.to_quadtree(maps)), maps)
+    # a small map: levels 3 below the base give the 64 x 64 order
+    q64 = S.quadtree_order(64, 3).reshape(64, 64)
+    assert q64[0, 0] == 0 and q64[0, 1] == 1 and q64[1, 0] == 2 and q64[8, 0] == 8 * 64
 
 
 def test_gate():
-    a = np.ones((3, 64, 64))
-    b = a.copy()
-    b[1] *= 1.5
-    b[2] = 0
-    assert list(S.gate(a, b, tol=0.1)) == [True, False, False]
+    a = np.ones(3)
+    b = np.array([1.0, 1.5, 0.0])
+    assert list(S.gate([a, b], tol=0.1)) == [True, False, False]
     # more networks: the spread of their run-ups against the mean
-    c = a.copy()
-    c[0] *= 1.05
-    assert list(S.gate(a, a, c, tol=0.1)) == [True, True, True]
-    assert list(S.gate(a, a, c, tol=0.01)) == [False, True, True]
+    c = np.array([1.05, 1.0, 1.0])
+    assert list(S.gate([a, a, c], tol=0.1)) == [True, True, True]
+    assert list(S.gate([a, a, c], tol=0.01)) == [False, True, True]
+    # small run-ups: the spread against RUNUP_MIN
+    assert list(S.gate([np.array([0.1]), np.array([0.105])], tol=0.01)) == [True]
 
 
 @pytest.mark.gpu
@@ -107,31 +128,40 @@ def test_gemm_bf16_matches_reference(gpu_lib):
 @pytest.mark.gpu
 def test_loss_and_gradients_match_references(gpu_lib):
     rng = np.random.default_rng(4)
-    x, y = _batch(rng, 3)
+    x, y, yr = _batch(rng, 3)
     p = _params(2)
     net = S.Surrogate(batch_max=4, seed=0)
     net.set_params(p)
-    loss, g = net.loss_grad(x, y)
-    l_b, g_b = R.loss_grad_bf16(p, x, y)
-    l_a, g_a = R.loss_grad_autograd(p, x, y)
+    loss, g = net.loss_grad(x, y, yr, 10.0, 1.0)
+    l_b, g_b = R.loss_grad_bf16(p, x, y, yr, 10.0, 1.0)
+    l_a, g_a = R.loss_grad_autograd(p, x, y, yr, 10.0, 1.0)
     assert abs(loss - l_b) <= 1e-3 * l_b
-    for l, ((err_b, _), (err_a, cos_a)) in enumerate(zip(_per_param(g, g_b), _per_param(g, g_a))):
-        assert err_b <= 1e-2, (l, err_b)                 # same BF16 rounding points
-        assert err_a <= 0.1 and cos_a >= 0.995, (l, err_a, cos_a)   # FP32 autograd
-    out = net.forward(x)
-    assert out.shape == (3, S.NOUT)
+    errs = list(zip(_per_param(g, g_b), _per_param(g, g_a)))
+    print("relative error per tensor (BF16 restatement, autograd):",
+          [(round(eb, 4), round(ea, 3)) for (eb, _), (ea, _) in errs])
+    for l, ((err_b, _), (err_a, cos_a)) in enumerate(errs):
+        # same BF16 rounding points: the GPU and numpy sum in different
+        # orders, so an activation now and then rounds to the next BF16
+        # value; six decoder layers (M1..T6, tensors 14..27) compound that
+        assert err_b <= (1e-2 if l < 14 else 3e-2), (l, err_b)
+        assert err_a <= 0.2 and cos_a >= 0.98, (l, err_a, cos_a)   # FP32 autograd
+    m, r = net.forward(x)
+    assert m.shape == (3, S.NOUT) and r.shape == (3,)
     net.close()
 
 
 @pytest.mark.gpu
 def test_training_reduces_loss(gpu_lib):
     rng = np.random.default_rng(5)
-    x, _ = _batch(rng, 96)
-    # a learnable target: a fixed smooth map per input, scaled by the series' energy
+    x, _, _ = _batch(rng, 96)
+    # learnable targets: a fixed smooth map per input scaled by the series'
+    # energy, and the energy itself
     base = np.sin(np.linspace(0, 3 * np.pi, S.NOUT)).astype(np.float32)
-    y = (np.sqrt((x ** 2).mean(axis=(1, 2)))[:, None] * base[None, :]).astype(np.float32)
+    e = np.sqrt((x ** 2).mean(axis=(1, 2))).astype(np.float32)
+    y = (e[:, None] * base[None, :]).astype(np.float32)
     net = S.Surrogate(batch_max=32, seed=1)
-    losses = net.train(x, y, batch=32, steps=300, lr=1e-3, seed=3)
+    net.set_data(x, y, e)
+    losses = net.train(batch=32, steps=300, lr=1e-3, seed=3)
     assert np.isfinite(losses).all()
     assert losses[-20:].mean() < 0.1 * losses[:5].mean(), (losses[:5], losses[-5:])
     net.close()
@@ -139,11 +169,18 @@ def test_training_reduces_loss(gpu_lib):
 
 @pytest.mark.gpu
 def test_solver_data(gpu_lib):
-    x, yq, maps, runups = S.dataset([0, 1])
-    assert x.shape == (2, S.T, S.G) and yq.shape == (2, S.NOUT) and maps.shape == (2, 64, 64)
-    assert np.isfinite(x).all() and (maps >= 0).all() and (runups >= 0).all()
+    d = S.dataset([0, 1])
+    x, yq, ru = d["x"], d["y"], d["ru"]
+    levels = S.from_quadtree(yq)
+    assert x.shape == (2, S.T, S.G) and yq.shape == (2, S.NOUT) and levels.shape == (2, S.MAP, S.MAP)
+    assert np.isfinite(x).all() and (levels >= 0).all() and (ru > 0).all()
     assert np.abs(x).max() > 0.0
-    assert np.array_equal(S.from_quadtree(yq) / S.Y_SCALE, maps)
+    # the highest flood level is the run-up; the maps rebuilt from the levels
+    # keep the flooded cells
+    assert np.array_equal(levels.reshape(2, -1).max(axis=1), ru.astype(np.float32))
+    maps = S.map_from_levels(levels)
+    assert ((maps > 0).sum(axis=(1, 2)) >= d["area"] - d["rec"]).all()
+    assert (d["rec"] <= 0.02 * d["area"]).all(), (d["rec"], d["area"])
     # the ensemble's gauge series are the parent's own (cliffs_run_gauges)
     st = S.site()
     case = S.scenario(1)
@@ -153,38 +190,59 @@ def test_solver_data(gpu_lib):
     alone = g.run_gauges(S.EVERY, S.T, st.gI, st.gJ)
     g.close()
     assert np.array_equal(alone * np.float32(S.X_SCALE), x[1])
+    # the kept parent grids, re-uploaded for the next batch, give the same
+    again = S.parent_gauges([S.scenario(0), S.scenario(1)]) * np.float32(S.X_SCALE)
+    assert np.array_equal(again, x)
 
 
 @pytest.mark.gpu
 def test_trained_network_gain(gpu_lib):
     """What the trained network gains over the method without it, on
-    scenarios it has not seen: its maps are closer to the full nested solve
-    than the training mean map (the best answer without a network), and
-    ensemble parents plus inference cost a small fraction of the nested
+    scenarios it has not seen: its run-ups are closer to the full nested
+    solve's than the training mean run-up (the best answer without a
+    network), its maps closer than the training mean map, and ensemble
+    parents plus the gate's inference cost a small fraction of the nested
     solve."""
     import time
-    tr = S.dataset(range(64))
+    d = S.dataset(range(704))
+    fit = slice(0, 640)
+    cal = slice(640, 704)
     cases = [S.scenario(s) for s in range(300_000, 300_032)]
     t0 = time.perf_counter()
-    x = S.parent_gauges(cases) * S.X_SCALE
+    xt = S.parent_gauges(cases) * S.X_SCALE
     t_par = (time.perf_counter() - t0) / len(cases)
     t0 = time.perf_counter()
     truth = [S.nested_solve(c) for c in cases]
     t_nest = (time.perf_counter() - t0) / len(cases)
     maps = np.array([m for m, _ in truth])
-    net = S.Surrogate(batch_max=64, seed=1)
-    losses = net.train(tr[0], tr[1], batch=32, steps=600, lr=5e-4, seed=2)
-    assert np.isfinite(losses).all()
+    rt = np.array([r for _, r in truth])
+    net = S.Surrogate(batch_max=32, seed=1)
+    net.set_data(d["x"][fit], d["y"][fit], d["ru"][fit])
+    best, best_p = np.inf, None
+    for k in range(8):                       # early stopping on held-back runs
+        losses = net.train(batch=32, steps=250, lr=5e-4, seed=2 + k)
+        assert np.isfinite(losses).all()
+        [r] = S.predict_runups([net], d["x"][cal])
+        e = float(np.abs(r - d["ru"][cal]).mean())
+        if e < best:
+            best, best_p = e, net.params()
+    net.set_params(best_p)
     t0 = time.perf_counter()
-    pred = S.predict_maps(net, x)
+    [pr] = S.predict_runups([net], xt)
     t_inf = (time.perf_counter() - t0) / len(cases)
+    _, pm = S.predict_all([net], xt)
     net.close()
-    r_net = float(np.sqrt(((pred - maps) ** 2).mean()))
-    r_mean = float(np.sqrt(((tr[2].mean(axis=0)[None] - maps) ** 2).mean()))
-    print(f"map RMSE {r_net:.3f} m vs mean map {r_mean:.3f} m; per scenario: parents {t_par * 1e3:.1f} ms, "
-          f"inference {t_inf * 1e3:.2f} ms, nested solve {t_nest * 1e3:.0f} ms")
+    mean_map = S.map_from_levels(S.from_quadtree(d["y"][fit].mean(axis=0)))[0]
+    e_net = float(np.abs(pr - rt).mean())
+    e_mean = float(np.abs(d["ru"][fit].mean() - rt).mean())
+    r_net = float(np.sqrt(((pm - maps) ** 2).mean()))
+    r_mean = float(np.sqrt(((mean_map[None] - maps) ** 2).mean()))
+    print(f"run-up error {e_net:.4f} m vs mean {e_mean:.4f} m; map RMSE {r_net:.4f} m vs mean map "
+          f"{r_mean:.4f} m; per scenario: parents {t_par * 1e3:.1f} ms, gate inference {t_inf * 1e3:.2f} ms, "
+          f"nested solve {t_nest * 1e3:.0f} ms")
+    assert e_net < 0.6 * e_mean, (e_net, e_mean)
     assert r_net < 0.95 * r_mean, (r_net, r_mean)
-    assert t_par + t_inf < 0.2 * t_nest, (t_par, t_inf, t_nest)
+    assert t_par + t_inf < 0.1 * t_nest, (t_par, t_inf, t_nest)
 
 
 @pytest.mark.gpu
@@ -193,14 +251,18 @@ def test_gate_falls_back_to_nested_solve(gpu_lib):
     nested solve, bit for bit; an accepted one by the networks' mean."""
     cases = [S.scenario(s) for s in (400_000, 400_001)]
     nets = [S.Surrogate(batch_max=8, seed=k) for k in (1, 2)]
-    maps, ok = S.answer(cases, nets, tol=-1.0)         # a negative tolerance accepts nothing
+    maps, runups, ok = S.answer(cases, nets, tol=-1.0)      # a negative tolerance accepts nothing
     assert not ok.any()
-    for c, m in zip(cases, maps):
-        assert np.array_equal(m, S.nested_solve(c)[0])
-    maps, ok = S.answer(cases, nets, tol=np.inf)
+    for c, m, r in zip(cases, maps, runups):
+        mm, rr = S.nested_solve(c)
+        assert np.array_equal(m, mm) and r == rr
+    maps, runups, ok = S.answer(cases, nets, tol=np.inf)
     assert ok.all()
-    x = S.parent_gauges(cases) * S.X_SCALE
-    want = 0.5 * (S.predict_maps(nets[0], x) + S.predict_maps(nets[1], x))
-    assert np.array_equal(maps, want)
+    x = S.parent_gauges(cases) * S.X_SCALE         # the kept parent grids, re-uploaded
+    rus, pm = S.predict_all(nets, x)
+    assert np.array_equal(maps, pm) and np.array_equal(runups, rus.mean(axis=0))
+    # the maps built on the device equal the host reconstruction of the
+    # networks' mean levels
+    assert np.array_equal(pm, S.map_from_levels(S.from_quadtree(S.predict_levels(nets, x))))
     for n in nets:
         n.close()
