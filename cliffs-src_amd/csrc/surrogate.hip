@@ -71,9 +71,13 @@ __device__ __forceinline__ float from_bf16(short s) {
 //   backward); accumulate: C += result.  Outputs FP32 C (ldc) and/or BF16 Cb.
 //   row(m) = m, or with rp > 0: t = m % rp, rows with t >= rv are dropped
 //   and the others stored at row (m / rp) * op + osh + t.
-// Tiles: 64 x 64 per 256-thread block, K in steps of 16 through LDS; each
-// wave computes a 32 x 32 quarter as 2 x 2 MFMA 16x16x16 tiles.
-constexpr int kTM = 64, kTN = 64, kTK = 16;
+// Tiles: 64 x 64 per 256-thread block, K in stages of 64 through LDS (four
+// MFMA k-steps per stage between two barriers: a stage's loads are the
+// latency a small product waits on); each wave computes a 32 x 32 quarter
+// as 2 x 2 MFMA 16x16x16 tiles.  A product with few output tiles is split
+// along K (blockIdx.z) into FP32 partials; ksum_kernel adds them in a fixed
+// order and applies the epilogue.
+constexpr int kTM = 64, kTN = 64, kTK = 64;   // stage depth kTK, or 16 for K <= 32
 
 struct GemmArgs {
     int M, N, K;
@@ -92,49 +96,90 @@ struct GemmArgs {
     float* part;          // ... into part[z][M][N] (no epilogue; 0: no split)
 };
 
+// the epilogue of output (m, n) with accumulated value v
+__device__ __forceinline__ void gemm_store(const GemmArgs& g, int m, int n, float v) {
+    long row = m;
+    if (g.rp > 0) {
+        const int tt = m % g.rp;
+        if (tt >= g.rv) return;
+        row = (long)(m / g.rp) * g.op + g.osh + tt;
+    }
+    const long o = row * g.ldc + n;
+    if (g.bias) v += g.bias[n];
+    if (g.relu) v = fmaxf(v, 0.f);
+    if (g.mask && !(from_bf16(g.mask[o]) > 0.f)) v = 0.f;
+    if (g.accumulate) v += g.C[o];
+    if (g.C) g.C[o] = v;
+    if (g.Cb) g.Cb[o] = to_bf16(v);
+}
+
+template <int TK>
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmArgs g) {
-    __shared__ short As[kTM][kTK + 4];   // [m][k]
-    __shared__ short Bs[kTN][kTK + 4];   // [n][k]
+    constexpr int kPer = TK / 4;         // elements of each operand per thread and stage
+    constexpr int kSh = TK == 64 ? 6 : 4;
+    __shared__ short As[kTM][TK + 8];    // [m][k]
+    __shared__ short Bs[kTN][TK + 8];    // [n][k]
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int m0 = blockIdx.y * kTM, n0 = blockIdx.x * kTN;
     const int wm = (w >> 1) * 32, wn = (w & 1) * 32;  // wave's quarter
     const int kbeg = g.part ? blockIdx.z * g.kchunk : 0;
     const int kend = g.part ? min(g.K, kbeg + g.kchunk) : g.K;
+    const bool ak = g.sak == 1 || g.sam != 1;   // A's unit stride along k
+    const bool bn = g.sbn == 1 || g.sbk != 1;   // B's unit stride along n
     f32x4 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    for (int k0 = kbeg; k0 < kend; k0 += kTK) {
-        // stage A (64 x 16) and B (16 x 64): 1024 elements each, 4 per thread
+    for (int k0 = kbeg; k0 < kend; k0 += TK) {
+        // stage A (64 x TK) and B (TK x 64), TK / 4 elements each per
+        // thread; consecutive lanes take the operand's unit-stride index
+        // (k of a row-major A, m of a transposed one; n of B, or k), so a
+        // wave's loads are contiguous
+        short ra[kPer], rb[kPer];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
+        for (int r = 0; r < kPer; r++) {
             const int e = t + 256 * r;
-            const int mm = e >> 4, kk = e & 15;      // A: row-major over k
+            const int mm = ak ? e >> kSh : e & 63, kk = ak ? e & (TK - 1) : e >> 6;
             const int m = m0 + mm, k = k0 + kk;
-            As[mm][kk] = (m < g.M && k < kend) ? g.A[(long)m * g.sam + (long)k * g.sak] : (short)0;
-            const int nn = e & 63, kb = e >> 6;      // B: consecutive n
+            ra[r] = (m < g.M && k < kend) ? g.A[(long)m * g.sam + (long)k * g.sak] : (short)0;
+            const int nn = bn ? e & 63 : e >> kSh, kb = bn ? e >> 6 : e & (TK - 1);
             const int n = n0 + nn, kB = k0 + kb;
-            Bs[nn][kb] = (n < g.N && kB < kend) ? g.B[(long)kB * g.sbk + (long)n * g.sbn] : (short)0;
+            rb[r] = (n < g.N && kB < kend) ? g.B[(long)kB * g.sbk + (long)n * g.sbn] : (short)0;
+        }
+#pragma unroll
+        for (int r = 0; r < kPer; r++) {
+            const int e = t + 256 * r;
+            if (ak)
+                As[e >> kSh][e & (TK - 1)] = ra[r];
+            else
+                As[e & 63][e >> 6] = ra[r];
+            if (bn)
+                Bs[e & 63][e >> 6] = rb[r];
+            else
+                Bs[e >> kSh][e & (TK - 1)] = rb[r];
         }
         __syncthreads();
         // MFMA 16x16x16 operand layout: lane l holds row (A) / column (B)
-        // l % 16 and k = 4 (l / 16) .. 4 (l / 16) + 3
+        // l % 16 and k = 4 (l / 16) .. 4 (l / 16) + 3 of each 16-k step
         const int r16 = lane & 15, kq = (lane >> 4) * 4;
-        bf16x4 a[2], b[2];
 #pragma unroll
-        for (int i = 0; i < 2; i++)
+        for (int ks = 0; ks < TK; ks += 16) {
+            bf16x4 a[2], b[2];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                a[i][q] = As[wm + 16 * i + r16][kq + q];
-                b[i][q] = Bs[wn + 16 * i + r16][kq + q];
-            }
+            for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int i = 0; i < 2; i++)
+                for (int q = 0; q < 4; q++) {
+                    a[i][q] = As[wm + 16 * i + r16][ks + kq + q];
+                    b[i][q] = Bs[wn + 16 * i + r16][ks + kq + q];
+                }
 #pragma unroll
-            for (int j = 0; j < 2; j++)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
         __syncthreads();
     }
     // result layout: lane l holds column l % 16, rows 4 (l / 16) + q
@@ -148,63 +193,70 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmArgs g) {
                 const int m = m0 + wm + 16 * i + rq + q;
                 const int n = n0 + wn + 16 * j + c16;
                 if (m >= g.M || n >= g.N) continue;
-                if (g.part) {
+                if (g.part)
                     g.part[((long)blockIdx.z * g.M + m) * g.N + n] = acc[i][j][q];
-                    continue;
-                }
-                long row = m;
-                if (g.rp > 0) {
-                    const int tt = m % g.rp;
-                    if (tt >= g.rv) continue;
-                    row = (long)(m / g.rp) * g.op + g.osh + tt;
-                }
-                float v = acc[i][j][q];
-                const long o = row * g.ldc + n;
-                if (g.bias) v += g.bias[n];
-                if (g.relu) v = fmaxf(v, 0.f);
-                if (g.mask && !(from_bf16(g.mask[o]) > 0.f)) v = 0.f;
-                if (g.accumulate) v += g.C[o];
-                if (g.C) g.C[o] = v;
-                if (g.Cb) g.Cb[o] = to_bf16(v);
+                else
+                    gemm_store(g, m, n, acc[i][j][q]);
             }
 }
 
-// sum of the split-K partials in a fixed order: C = sum_z part[z] (+= C)
-__global__ void ksum_kernel(const float* part, int splits, int M, int N, float* C, long ldc, int accumulate) {
+// the split-K partials added in a fixed order, then the epilogue: a thread
+// per output, or for few outputs a block per output (threads over the
+// splits, then a tree)
+__global__ void ksum_kernel(const float* part, int splits, const GemmArgs g) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long)M * N) return;
+    if (i >= (long)g.M * g.N) return;
     float v = 0.f;
-    for (int z = 0; z < splits; z++) v += part[(long)z * M * N + i];
-    const long o = (i / N) * ldc + i % N;
-    if (accumulate) v += C[o];
-    C[o] = v;
+    for (int z = 0; z < splits; z++) v += part[(long)z * g.M * g.N + i];
+    gemm_store(g, (int)(i / g.N), (int)(i % g.N), v);
+}
+__global__ void ksum_block_kernel(const float* part, int splits, const GemmArgs g) {
+    __shared__ float red[256];
+    const long i = blockIdx.x, mn = (long)g.M * g.N;
+    float v = 0.f;
+    for (int z = threadIdx.x; z < splits; z += blockDim.x) v += part[(long)z * mn + i];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) gemm_store(g, (int)(i / g.N), (int)(i % g.N), red[0]);
 }
 
 constexpr long kSplitWords = 1L << 22;   // split-K partials (FP32 words)
 
-// A product whose few output tiles would leave most of the device idle over
-// a long K (the weight gradients of the decoder: 4 x 4 outputs summed over
-// 2M rows) is split along K when a partial buffer is given and the
-// epilogue is a plain FP32 store.
+// A product whose few output tiles would leave most of the device idle
+// while each block walks a long K (the weight gradients: 4 x 4 outputs
+// summed over 2M rows; the trunk: 32 x 512 outputs over K = 512) is split
+// along K when a partial buffer is given, to about 256 k per block.
 hipError_t gemm(GemmArgs g, hipStream_t st, float* part = nullptr) {
     const dim3 grid((g.N + kTN - 1) / kTN, (g.M + kTM - 1) / kTM);
     const long tiles = (long)grid.x * grid.y;
     int splits = 1;
-    if (part && g.C && !g.Cb && !g.bias && !g.relu && !g.mask && g.rp == 0 && tiles < 256 && g.K >= 4096) {
-        splits = (int)std::min<long>({(g.K + 2047) / 2048, 1024 / tiles + 1, kSplitWords / ((long)g.M * g.N)});
+    if (part && tiles < 256 && g.K >= 1024) {
+        splits = (int)std::min<long>({(g.K + 255) / 256, 1024 / tiles + 1, kSplitWords / ((long)g.M * g.N)});
         splits = std::max(splits, 1);
     }
     if (splits > 1) {
         g.kchunk = (int)(((g.K + splits - 1) / splits + kTK - 1) / kTK * kTK);
         splits = (g.K + g.kchunk - 1) / g.kchunk;
+    }
+    if (splits > 1) {
         g.part = part;
-        hipLaunchKernelGGL(gemm_bf16_kernel, dim3(grid.x, grid.y, splits), dim3(256), 0, st, g);
+        hipLaunchKernelGGL(gemm_bf16_kernel<kTK>, dim3(grid.x, grid.y, splits), dim3(256), 0, st, g);
         const long n = (long)g.M * g.N;
-        hipLaunchKernelGGL(ksum_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (const float*)part,
-                           splits, g.M, g.N, g.C, g.ldc, g.accumulate);
+        if (n < 16384 && splits >= 32)
+            hipLaunchKernelGGL(ksum_block_kernel, dim3((unsigned)n), dim3(256), 0, st, (const float*)part, splits, g);
+        else
+            hipLaunchKernelGGL(ksum_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                               (const float*)part, splits, g);
     } else {
         g.part = nullptr;
-        hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, st, g);
+        if (g.K <= 32)
+            hipLaunchKernelGGL(gemm_bf16_kernel<16>, grid, dim3(256), 0, st, g);
+        else
+            hipLaunchKernelGGL(gemm_bf16_kernel<kTK>, grid, dim3(256), 0, st, g);
     }
     return hipGetLastError();
 }
@@ -225,12 +277,20 @@ __global__ void colsum_partial_kernel(const float* X, int M, int N, float* part)
         part[(long)blockIdx.y * N + n] = red[0][threadIdx.x] + red[1][threadIdx.x] +
                                          red[2][threadIdx.x] + red[3][threadIdx.x];
 }
+// one block per column: each thread sums a fixed stride of the chunks,
+// then a tree over the threads (a fixed order: deterministic)
 __global__ void colsum_final_kernel(const float* part, int chunks, int N, float* out) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+    __shared__ float red[256];
+    const int n = blockIdx.x;
     float s = 0.f;
-    for (int c = 0; c < chunks; c++) s += part[(long)c * N + n];
-    out[n] = s;
+    for (int c = threadIdx.x; c < chunks; c += blockDim.x) s += part[(long)c * N + n];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[n] = red[0];
 }
 
 __global__ void to_bf16_kernel(const float* x, short* y, long n) {
@@ -535,7 +595,7 @@ int forward(cliffs_sur* s, int b) {
         a.rv = E.valid;
         a.op = E.pout;
         a.osh = E.sh;
-        CHK(gemm(a, s->st));
+        CHK(gemm(a, s->st, s->kpart));
     }
     hipLaunchKernelGGL(pool_kernel, dim3((b * kPoolC + 255) / 256), dim3(256), 0, s->st, s->ae[3], b,
                        kPoolT, kPoolC, s->feat, s->arg);
@@ -556,7 +616,7 @@ int forward(cliffs_sur* s, int b) {
             a.C = s->pred;
         else
             a.Cb = s->ad[l];
-        CHK(gemm(a, s->st));
+        CHK(gemm(a, s->st, s->kpart));
     }
     return CLIFFS_SUR_OK;
 }
@@ -566,8 +626,7 @@ int colsum(cliffs_sur* s, const float* X, int M, int N, float* out) {
     hipLaunchKernelGGL(colsum_partial_kernel, dim3((N + 63) / 64, chunks), dim3(256), 0, s->st, X, M,
                        N, s->part);
     CHK(hipGetLastError());
-    hipLaunchKernelGGL(colsum_final_kernel, dim3((N + 255) / 256), dim3(256), 0, s->st, s->part,
-                       chunks, N, out);
+    hipLaunchKernelGGL(colsum_final_kernel, dim3(N), dim3(256), 0, s->st, s->part, chunks, N, out);
     CHK(hipGetLastError());
     return CLIFFS_SUR_OK;
 }
@@ -591,7 +650,7 @@ int dense_back(cliffs_sur* s, int j, int M, int in, int out, const short* X, con
     gx.ldc = in;
     gx.mask = mask;
     gx.accumulate = acc;
-    CHK(gemm(gx, s->st));
+    CHK(gemm(gx, s->st, s->kpart));
     return CLIFFS_SUR_OK;
 }
 
@@ -644,7 +703,7 @@ int backward(cliffs_sur* s, int b) {
         GemmArgs gc = plain_gemm(M, (int)K, E.cout, s->dA, E.cout, 1, s->wb + s->woff[l], 1, E.cout);
         gc.C = s->dC;
         gc.ldc = K;
-        CHK(gemm(gc, s->st));
+        CHK(gemm(gc, s->st, s->kpart));
         const Conv& P = kE[l - 1];
         const long n = (long)b * P.pc * P.cout;
         hipLaunchKernelGGL(col2im_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->st, s->dC,

@@ -544,16 +544,16 @@ def predict_levels(nets, x):
     return acc / len(nets)
 
 
-def answer(cases, nets, tol, device=0):
+def answer(cases, nets, tol, lo=-np.inf, hi=np.inf, device=0):
     """Max-inundation maps and run-ups of `cases` at the site: the parents
-    stepped as one ensemble give the gauge series; where the networks'
-    run-ups agree within tol (gate) the answer is the mean of their run-ups
-    and the map of their mean flood levels (built on the device), elsewhere
-    the full nested solve runs.  -> (maps (n, 512, 512), run-ups (n,),
-    answered by the surrogate (n,))."""
+    stepped as one ensemble give the gauge series; where the gate accepts
+    the networks' run-ups (agreement within tol, mean in [lo, hi]) the
+    answer is their mean run-up and the map of their mean flood levels
+    (built on the device), elsewhere the full nested solve runs.  -> (maps
+    (n, 512, 512), run-ups (n,), answered by the surrogate (n,))."""
     x = parent_gauges(cases, device) * X_SCALE
     rus, _ = predict_all(nets, x, maps=False)
-    ok = gate(list(rus), tol=tol)
+    ok = gate(list(rus), tol=tol, lo=lo, hi=hi)
     runups = rus.mean(axis=0)
     maps = np.zeros((len(cases), MAP, MAP), dtype=np.float32)
     if ok.any():
@@ -576,6 +576,35 @@ def gate_score(runups):
     return (r.max(axis=0) - r.min(axis=0)) / np.maximum(r.mean(axis=0), RUNUP_MIN)
 
 
-def gate(runups, tol):
-    """Confidence gate: the networks' run-ups agree within tol."""
-    return gate_score(runups) <= tol
+def gate(runups, tol, lo=-np.inf, hi=np.inf):
+    """Confidence gate: the networks' run-ups agree within tol and their
+    mean lies in [lo, hi] (m), the range where they were found reliable
+    (calibrate_gate)."""
+    m = np.mean(np.stack(runups), axis=0)
+    return (gate_score(runups) <= tol) & (m >= lo) & (m <= hi)
+
+
+def calibrate_gate(runups, true, budget, q=0.95):
+    """The gate (tol, lo, hi) that accepts the most of the calibration runs
+    whose networks' run-ups are `runups` (list of (n,)) while keeping the
+    q-quantile of the accepted run-up errors (relative, absolute / RUNUP_MIN
+    below RUNUP_MIN) within `budget`: lo and hi among quantiles of the
+    predicted run-ups, for each pair the loosest tolerance.  -> (tol, lo,
+    hi), or None when no gate meets the budget."""
+    r = np.stack(runups)
+    m = r.mean(axis=0)
+    err = np.abs(m - true) / np.maximum(true, RUNUP_MIN)
+    sc = gate_score(runups)
+    los = [-np.inf] + list(np.quantile(m, [0.01, 0.02, 0.03, 0.05, 0.075, 0.1]))
+    his = [np.inf] + list(np.quantile(m, [0.99, 0.98, 0.97, 0.95, 0.925, 0.9]))
+    best, best_n = None, 0
+    for lo in los:
+        for hi in his:
+            inr = (m >= lo) & (m <= hi)
+            for t in np.sort(sc[inr])[::-1]:
+                ok = inr & (sc <= t)
+                if np.quantile(err[ok], q) <= budget:
+                    if ok.sum() > best_n:
+                        best, best_n = (float(t), float(lo), float(hi)), int(ok.sum())
+                    break
+    return best

@@ -6,8 +6,8 @@ Scenarios are sources at one nested site (surrogate.Site).  For each, the
 parents of all scenarios are stepped as an ensemble with gauge recording
 (the network's input) and the full nested solve (parent + coupled child) is
 run for the truth.  Of the --train training runs, the last --calib are held
-back from the fit for early stopping and for calibrating the gate; the
---test runs are held out from both.  Several networks (--nets) with
+back from the fit for calibrating the gate and the --stop before them for
+early stopping; the --test runs are held out from all of it.  Several networks (--nets) with
 different seeds are trained on the MFMA kernels.  A scenario is answered by
 the surrogate when the networks' run-ups agree (surrogate.gate), otherwise
 by the full nested solve; the gate's tolerance is calibrated per run-up
@@ -31,7 +31,7 @@ sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 
 import surrogate as S  # noqa: E402
 
-BUDGETS = (0.02, 0.05, 0.1, 0.2)   # run-up error budgets (95th percentile of the accepted)
+BUDGETS = (0.015, 0.02, 0.05, 0.1)   # run-up error budgets (95th percentile of the accepted)
 
 
 def log(*a):
@@ -53,18 +53,23 @@ def make_split(name, seeds, ens, solvers):
 
 
 def train_early_stop(seed, fit, cal, steps, chunk, batch, lr, w_map, w_runup):
+    """Adam with the learning rate on a cosine from lr down to lr / 20 over
+    the steps (constant within each chunk); the parameters of the chunk
+    with the smallest run-up error on the runs `cal` are kept."""
     net = S.Surrogate(batch_max=max(batch, 64), seed=seed)
     net.set_data(fit["x"], fit["y"], fit["ru"])
     best, best_p, hist = np.inf, None, []
     for k in range(0, steps, chunk):
-        net.train(batch=batch, steps=chunk, lr=lr, seed=100 * seed + k, w_map=w_map, w_runup=w_runup)
+        lr_k = lr / 20 + 0.5 * (lr - lr / 20) * (1.0 + np.cos(np.pi * k / steps))
+        net.train(batch=batch, steps=chunk, lr=float(lr_k), seed=100 * seed + k, w_map=w_map,
+                  w_runup=w_runup)
         [r] = S.predict_runups([net], cal["x"])
         e = float(np.abs(r - cal["ru"]).mean())
         hist.append(round(e, 4))
         if e < best:
             best, best_p = e, net.params()
     net.set_params(best_p)
-    log(f"[study] network {seed}: calibration run-up MAE (m) by chunk {hist}")
+    log(f"[study] network {seed}: early-stopping run-up MAE (m) by chunk {hist}")
     return net, hist
 
 
@@ -115,6 +120,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--train", type=int, default=4096)
     ap.add_argument("--calib", type=int, default=512)
+    ap.add_argument("--stop", type=int, default=256, help="training runs for early stopping")
     ap.add_argument("--test", type=int, default=512)
     ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--chunk", type=int, default=250)
@@ -123,7 +129,7 @@ def main():
     ap.add_argument("--w-map", type=float, default=S.W_MAP)
     ap.add_argument("--w-runup", type=float, default=S.W_RUNUP)
     ap.add_argument("--ens", type=int, default=64, help="parents per ensemble")
-    ap.add_argument("--nets", type=int, default=4, help="networks trained from different seeds")
+    ap.add_argument("--nets", type=int, default=6, help="networks trained from different seeds")
     ap.add_argument("--solvers", type=int, default=8, help="nested solves run at a time")
     ap.add_argument("--save", default="", help="write per-scenario run-ups (npz) here")
     ap.add_argument("--synthetic", type=int, default=0, metavar="STEPS",
@@ -141,14 +147,15 @@ def main():
     tr = make_split("train", range(args.train), args.ens, args.solvers)
     te = make_split("test", range(200_000, 200_000 + args.test), args.ens, args.solvers)
     t_gen = time.perf_counter() - t0
-    nfit = args.train - args.calib
+    nfit = args.train - args.calib - args.stop
     fit = {k: tr[k][:nfit] for k in ("x", "y", "ru")}
-    cal = {k: tr[k][nfit:] for k in ("x", "y", "ru")}
+    stop = {k: tr[k][nfit:nfit + args.stop] for k in ("x", "y", "ru")}
+    cal = {k: tr[k][nfit + args.stop:] for k in ("x", "y", "ru")}
 
     nets, hists = [], []
     t0 = time.perf_counter()
     for k in range(1, args.nets + 1):
-        net, h = train_early_stop(k, fit, cal, args.steps, args.chunk, args.batch, args.lr, args.w_map,
+        net, h = train_early_stop(k, fit, stop, args.steps, args.chunk, args.batch, args.lr, args.w_map,
                                   args.w_runup)
         nets.append(net)
         hists.append(h)
@@ -180,31 +187,48 @@ def main():
     err = runup_err(ru_t, te["ru"])
     dis_t = S.gate_score(rus)
 
-    def gate_curve(e_cal, e_test, budgets):
-        """Per budget: the loosest gate tolerance whose accepted calibration
-        scenarios keep the 95th percentile of the error within the budget,
-        applied to the held-out runs."""
+    def gate_curve(budgets, cells=False):
+        """Per budget: the gate calibrated on the calibration runs
+        (surrogate.calibrate_gate: the most accepted with the 95th
+        percentile of their run-up error within the budget), applied to the
+        held-out runs."""
         curve = []
         for budget in budgets:
-            tol = -1.0
-            for t in np.sort(dis_c):
-                ok = dis_c <= t
-                if np.quantile(e_cal[ok], 0.95) <= budget:
-                    tol = float(t)
-            ok = dis_t <= tol
+            if cells:   # budget in land cells: relative budget of the calibration median
+                b_rel = budget * cell / max(float(np.median(cal["ru"])), S.RUNUP_MIN)
+            else:
+                b_rel = budget
+            g = S.calibrate_gate(rus_c, cal["ru"], b_rel)
+            ok = S.gate(rus, *g) if g else np.zeros(len(err), dtype=bool)
+            e_test = cells_t if cells else err
             per_with = t_parent + t_gate + ok.mean() * t_map + (1.0 - ok.mean()) * t_nested
-            curve.append({"budget": budget, "gate_tol": round(tol, 4), "accept": round(float(ok.mean()), 3),
+            curve.append({"budget": budget, "gate": [round(v, 4) for v in g] if g else None,
+                          "accept": round(float(ok.mean()), 3),
                           "err_p95_accepted": round(float(np.quantile(e_test[ok], 0.95)), 4) if ok.any() else None,
                           "err_max_accepted": round(float(e_test[ok].max()), 4) if ok.any() else None,
                           "speedup": round(t_nested / per_with, 2),
                           "scenarios_per_hr": round(3600.0 / per_with, 1)})
         return curve
 
-    curve = gate_curve(err_c, err, BUDGETS)
     cell = S.site().land_step
-    cells_c = np.abs(ru_c - cal["ru"]) / cell
     cells_t = np.abs(ru_t - te["ru"]) / cell
-    curve_cells = gate_curve(cells_c, cells_t, (1.0, 2.0, 4.0))
+    curve = gate_curve(BUDGETS)
+    # the operating point for a 2 % target: the gate calibrated at 90 % of it
+    # (a margin for the calibration runs being a sample)
+    target = gate_curve((0.9 * 0.02,))[0]
+    curve_cells = gate_curve((1.0, 2.0, 4.0), cells=True)
+    # the gate on agreement alone (no run-up range), for comparison
+    curve_spread = []
+    for budget in BUDGETS:
+        tol = -1.0
+        for t in np.sort(dis_c):
+            if np.quantile(err_c[dis_c <= t], 0.95) <= budget:
+                tol = float(t)
+        ok = dis_t <= tol
+        per_with = t_parent + t_gate + ok.mean() * t_map + (1.0 - ok.mean()) * t_nested
+        curve_spread.append({"budget": budget, "gate_tol": round(tol, 4), "accept": round(float(ok.mean()), 3),
+                             "err_p95_accepted": round(float(np.quantile(err[ok], 0.95)), 4) if ok.any() else None,
+                             "speedup": round(t_nested / per_with, 2)})
     ru_mean = float(tr["ru"][:nfit].mean())
     err_mean = runup_err(np.full_like(te["ru"], ru_mean), te["ru"])
     # maps: the test maps rebuilt from their own levels (exact where the
@@ -228,15 +252,16 @@ def main():
                  "child_cells": int(S.site().depc.size), "parent_steps": S.EVERY * S.T, "gauges": S.G,
                  "samples": S.T, "map": [S.MAP, S.MAP]},
         "networks": args.nets, "params": nets[0].nparams, "solvers": args.solvers,
-        "train_runs": args.train, "fit_runs": nfit, "calib_runs": args.calib, "test_runs": args.test,
+        "train_runs": args.train, "fit_runs": nfit, "stop_runs": args.stop, "calib_runs": args.calib,
+        "test_runs": args.test,
         "train_steps_max": args.steps, "loss_weights": [args.w_map, args.w_runup],
-        "calib_runup_mae_by_chunk": hists,
+        "stop_runup_mae_by_chunk": hists,
         "nested_solve_s": round(t_nested, 4), "nested_solve_alone_s": round(t_alone, 4),
         "ensemble_parent_s": round(t_parent, 5), "gate_infer_s": round(t_gate, 6),
         "map_infer_s": round(t_map, 6),
         "scenarios_per_hr_nested": round(3600.0 / t_nested, 1),
         "ceiling_speedup": round(t_nested / (t_parent + t_gate + t_map), 2),
-        "gate_curve": curve,
+        "at_2pct_target": target, "gate_curve": curve, "gate_curve_agreement_only": curve_spread,
         "land_cell_rise_m": round(float(cell), 4), "gate_curve_land_cells": curve_cells,
         "runup_m_quantiles_test": [q(te["ru"], p) for p in (0.0, 0.05, 0.5, 0.95, 1.0)],
         "runup_zero_share_test": round(float((te["ru"] == 0).mean()), 3),

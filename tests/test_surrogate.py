@@ -111,6 +111,28 @@ def test_gate():
     assert list(S.gate([a, a, c], tol=0.01)) == [False, True, True]
     # small run-ups: the spread against RUNUP_MIN
     assert list(S.gate([np.array([0.1]), np.array([0.105])], tol=0.01)) == [True]
+    # the run-up range: means outside [lo, hi] are rejected however close
+    r = np.array([0.4, 0.8, 1.3])
+    assert list(S.gate([r, r], tol=0.1, lo=0.5, hi=1.2)) == [False, True, False]
+
+
+def test_calibrate_gate():
+    """calibrate_gate keeps the 95th percentile of the accepted errors within
+    the budget on its own runs, and uses the run-up range where the errors
+    sit at the ends of the distribution."""
+    rng = np.random.default_rng(7)
+    n = 600
+    true = rng.uniform(0.5, 1.2, n)
+    bias = np.where(true < 0.57, 0.05, 0.0)                 # unreliable at the low 10 %
+    noise = rng.normal(0, 0.004, (4, n)) * (1 + 3 * (rng.random(n) < 0.05))
+    runups = list(true + bias + noise)
+    err = np.abs(np.mean(runups, axis=0) - true) / np.maximum(true, S.RUNUP_MIN)
+    tol, lo, hi = S.calibrate_gate(runups, true, 0.02)
+    ok = S.gate(runups, tol, lo, hi)
+    assert np.quantile(err[ok], 0.95) <= 0.02
+    assert np.quantile(err, 0.95) > 0.02                      # the budget needs a gate
+    assert lo > 0.55 and ok.mean() > 0.8, (lo, ok.mean())
+    assert S.calibrate_gate(runups, true, 1e-6) is None
 
 
 @pytest.mark.gpu
