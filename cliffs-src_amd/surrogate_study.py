@@ -52,12 +52,28 @@ def make_split(name, seeds, ens, solvers):
     return d
 
 
-def train_early_stop(seed, fit, cal, steps, chunk, batch, lr, w_map, w_runup):
+def oversample(ru, rmax, width=0.05):
+    """Indices of the fitted runs with each one repeated in inverse
+    proportion to how many runs share its run-up bin (bins of `width` m),
+    at most rmax times: the ends of the run-up distribution, where a network
+    regresses toward the mean, weigh more in the fit."""
+    b = np.floor(ru / width).astype(int)
+    _, inv, cnt = np.unique(b, return_inverse=True, return_counts=True)
+    rep = np.clip(np.round(np.median(cnt[inv]) / cnt[inv]), 1, rmax).astype(int)
+    return np.repeat(np.arange(len(ru)), rep)
+
+
+def train_early_stop(seed, fit, cal, steps, chunk, batch, lr, w_map, w_runup, rmax=1):
     """Adam with the learning rate on a cosine from lr down to lr / 20 over
     the steps (constant within each chunk); the parameters of the chunk
-    with the smallest run-up error on the runs `cal` are kept."""
+    with the smallest run-up error on the runs `cal` are kept.  rmax > 1:
+    the fitted runs oversampled by their run-up (oversample)."""
     net = S.Surrogate(batch_max=max(batch, 64), seed=seed)
-    net.set_data(fit["x"], fit["y"], fit["ru"])
+    if rmax > 1:
+        idx = oversample(fit["ru"], rmax)
+        net.set_data(fit["x"][idx], fit["y"][idx], fit["ru"][idx])
+    else:
+        net.set_data(fit["x"], fit["y"], fit["ru"])
     best, best_p, hist = np.inf, None, []
     for k in range(0, steps, chunk):
         lr_k = lr / 20 + 0.5 * (lr - lr / 20) * (1.0 + np.cos(np.pi * k / steps))
@@ -121,6 +137,8 @@ def main():
     ap.add_argument("--train", type=int, default=4096)
     ap.add_argument("--calib", type=int, default=512)
     ap.add_argument("--stop", type=int, default=256, help="training runs for early stopping")
+    ap.add_argument("--oversample", type=int, default=1, metavar="R",
+                    help="repeat fitted runs of sparse run-up bins up to R times")
     ap.add_argument("--test", type=int, default=512)
     ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--chunk", type=int, default=250)
@@ -156,7 +174,7 @@ def main():
     t0 = time.perf_counter()
     for k in range(1, args.nets + 1):
         net, h = train_early_stop(k, fit, stop, args.steps, args.chunk, args.batch, args.lr, args.w_map,
-                                  args.w_runup)
+                                  args.w_runup, args.oversample)
         nets.append(net)
         hists.append(h)
     t_train = time.perf_counter() - t0
@@ -254,7 +272,7 @@ def main():
         "networks": args.nets, "params": nets[0].nparams, "solvers": args.solvers,
         "train_runs": args.train, "fit_runs": nfit, "stop_runs": args.stop, "calib_runs": args.calib,
         "test_runs": args.test,
-        "train_steps_max": args.steps, "loss_weights": [args.w_map, args.w_runup],
+        "train_steps_max": args.steps, "loss_weights": [args.w_map, args.w_runup], "oversample": args.oversample,
         "stop_runup_mae_by_chunk": hists,
         "nested_solve_s": round(t_nested, 4), "nested_solve_alone_s": round(t_alone, 4),
         "ensemble_parent_s": round(t_parent, 5), "gate_infer_s": round(t_gate, 6),
