@@ -550,7 +550,7 @@ struct cliffs_sur {
     short* dRb = nullptr;
     float* part = nullptr;               // reduction partials
     float* kpart = nullptr;              // split-K partials (kSplitWords)
-    float* hpart = nullptr;              // pinned
+    float* hpart = nullptr;              // pinned: a loss read back by cliffs_sur_loss_grad
     int* didx = nullptr;                 // batch rows of every training step
     long nidx = 0;
     float* dloss = nullptr;              // per-step losses
@@ -832,9 +832,9 @@ int cliffs_sur_create(int device, int batch_max, unsigned seed, cliffs_sur** out
               dalloc(&s->dA, fmax) && dalloc(&s->dB, fmax) && dalloc(&s->dC, cmax) &&
               dalloc(&s->dH, (size_t)B * kFeat) && dalloc(&s->dHb, (size_t)B * kFeat) &&
               dalloc(&s->dR, (size_t)B) && dalloc(&s->dRb, (size_t)B) &&
-              dalloc(&s->part, fmax / 256 + 4096) && dalloc(&s->kpart, kSplitWords) &&
-              dalloc(&s->dloss, 1) && (s->nloss = 1) &&
-              hipHostMalloc(&s->hpart, sizeof(float) * (fmax / 256 + 4096)) == hipSuccess;
+              dalloc(&s->part, fmax / 256 + 4096) && dalloc(&s->kpart, kSplitWords) && dalloc(&s->dloss, 1) &&
+              hipHostMalloc(&s->hpart, sizeof(float) * 16) == hipSuccess;
+    if (ok) s->nloss = 1;
     for (int l = 0; l < 4 && ok; l++) {
         const size_t n = enc_elems(l, B) + enc_slack(l);
         // padding rows and slack are read by overlapping windows: zero once,

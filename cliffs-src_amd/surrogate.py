@@ -120,6 +120,7 @@ class Surrogate:
         _chk(_lib().cliffs_sur_create(int(device), int(batch_max), int(seed), C.byref(h)))
         self._h = h
         self.batch_max = batch_max
+        self.has_ground = False
         n = C.c_long()
         _chk(_lib().cliffs_sur_nparams(h, C.byref(n)))
         self.nparams = n.value
@@ -175,6 +176,7 @@ class Surrogate:
         """Ground heights (MAP x MAP, raster; inf off land) for predict_all's maps."""
         g = _f32(np.where(np.isfinite(ground), ground, np.inf), (MAP, MAP))
         _chk(_lib().cliffs_sur_set_ground(self._h, _p(g)))
+        self.has_ground = True
 
     def set_data(self, x, y, yr):
         """The training set, kept on the device for train()."""
@@ -196,9 +198,8 @@ def predict_all(nets, x, maps=True):
     x = _f32(x)
     n, k = x.shape[0], len(nets)
     bm = min(net.batch_max for net in nets)
-    if maps and not getattr(nets[0], "_ground", False):
+    if maps and not nets[0].has_ground:
         nets[0].set_ground(site().ground_win)
-        nets[0]._ground = True
     hs = (C.c_void_p * k)(*[net._h for net in nets])
     ru = np.empty((k, n), dtype=np.float32)
     out = np.empty((n, MAP, MAP), dtype=np.float32) if maps else None

@@ -120,7 +120,6 @@ def synthetic(steps, batch):
     S.predict_all(nets, x, maps=False)
     t_gate = (time.perf_counter() - t0) / n
     net.set_ground(np.where(np.arange(S.NOUT).reshape(S.MAP, S.MAP) % 3 == 0, 0.5, np.inf))
-    nets[0]._ground = True
     t0 = time.perf_counter()
     S.predict_all(nets, x)
     t_map = (time.perf_counter() - t0) / n
