@@ -200,6 +200,65 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmArgs g) {
             }
 }
 
+// Tall, narrow products (N <= 16, many rows: the decoder's last layers, 2 M
+// rows x 4 outputs over K = 4): 256 x 16 output tiles, each wave 64 rows as
+// four MFMA 16x16x16 tiles, K in stages of 16.
+__global__ __launch_bounds__(256) void gemm_bf16_tall_kernel(const GemmArgs g) {
+    __shared__ short As[256][16 + 8];    // [m][k]
+    __shared__ short Bs[16][16 + 8];     // [n][k]
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int m0 = blockIdx.y * 256, n0 = blockIdx.x * 16;
+    const bool ak = g.sak == 1 || g.sam != 1;
+    const bool bn = g.sbn == 1 || g.sbk != 1;
+    f32x4 acc[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < g.K; k0 += 16) {
+        short ra[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int e = t + 256 * r;
+            const int mm = ak ? e >> 4 : e & 255, kk = ak ? e & 15 : e >> 8;
+            const int m = m0 + mm, k = k0 + kk;
+            ra[r] = (m < g.M && k < g.K) ? g.A[(long)m * g.sam + (long)k * g.sak] : (short)0;
+        }
+        const int nn = bn ? t & 15 : t >> 4, kb = bn ? t >> 4 : t & 15;
+        const int n = n0 + nn, kB = k0 + kb;
+        const short rbv = (n < g.N && kB < g.K) ? g.B[(long)kB * g.sbk + (long)n * g.sbn] : (short)0;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int e = t + 256 * r;
+            if (ak)
+                As[e >> 4][e & 15] = ra[r];
+            else
+                As[e & 255][e >> 8] = ra[r];
+        }
+        Bs[nn][kb] = rbv;
+        __syncthreads();
+        const int r16 = lane & 15, kq = (lane >> 4) * 4;
+        bf16x4 b;
+#pragma unroll
+        for (int q = 0; q < 4; q++) b[q] = Bs[r16][kq + q];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            bf16x4 a;
+#pragma unroll
+            for (int q = 0; q < 4; q++) a[q] = As[64 * w + 16 * i + r16][kq + q];
+            acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    const int c16 = lane & 15, rq = (lane >> 4) * 4;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int m = m0 + 64 * w + 16 * i + rq + q;
+            const int n = n0 + c16;
+            if (m < g.M && n < g.N) gemm_store(g, m, n, acc[i][q]);
+        }
+}
+
 // the split-K partials added in a fixed order, then the epilogue: a thread
 // per output, or for few outputs a block per output (threads over the
 // splits, then a tree)
@@ -235,6 +294,8 @@ hipError_t gemm(GemmArgs g, hipStream_t st, float* part = nullptr) {
     const long tiles = (long)grid.x * grid.y;
     int splits = 1;
     if (part && tiles < 256 && g.K >= 1024) {
+        // about one resident round of blocks: more splits only queue more
+        // latency-bound stages
         splits = (int)std::min<long>({(g.K + 255) / 256, 1024 / tiles + 1, kSplitWords / ((long)g.M * g.N)});
         splits = std::max(splits, 1);
     }
@@ -251,6 +312,9 @@ hipError_t gemm(GemmArgs g, hipStream_t st, float* part = nullptr) {
         else
             hipLaunchKernelGGL(ksum_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
                                (const float*)part, splits, g);
+    } else if (g.N <= 16 && g.M >= 1024) {
+        g.part = nullptr;
+        hipLaunchKernelGGL(gemm_bf16_tall_kernel, dim3((g.N + 15) / 16, (g.M + 255) / 256), dim3(256), 0, st, g);
     } else {
         g.part = nullptr;
         if (g.K <= 32)

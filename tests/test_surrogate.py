@@ -138,7 +138,10 @@ def test_calibrate_gate():
 @pytest.mark.gpu
 def test_gemm_bf16_matches_reference(gpu_lib):
     rng = np.random.default_rng(3)
-    for M, N, K in [(70, 33, 45), (128, 512, 64), (1, 4096, 512), (300, 4, 16), (64, 64, 2048)]:
+    # the last two: tall, narrow products (N <= 16, M >= 1024) take the
+    # 256 x 16 tile kernel, over one and several K stages
+    for M, N, K in [(70, 33, 45), (128, 512, 64), (1, 4096, 512), (300, 4, 16), (64, 64, 2048),
+                    (5000, 4, 4), (3000, 13, 40)]:
         A = rng.normal(size=(M, K)).astype(np.float32)
         B = rng.normal(size=(K, N)).astype(np.float32)
         got = S.gemm_bf16(A, B)
