@@ -195,6 +195,7 @@ struct cliffs_nest {
     bool start_set = false;            // threshold given by cliffs_nest_child_start
     int start = 0;                     // 1-based start record, 0 while unknown
     int scanned = 0;                   // records found below the threshold
+    int waited = 0;                    // record the child stream last waited for
     long run_k = 0;                    // cliffs_nest_run: record intervals run so far
     long run_kc = 0;                   // and child steps taken
     bool counted = false;              // the parent's use count includes this nest
@@ -1691,6 +1692,7 @@ int cliffs_nest_couple(cliffs_nest* n, cliffs_handle* child, int capacity) {
     n->scanned = 0;
     n->run_k = 0;
     n->run_kc = 0;
+    n->waited = 0;
     return CLIFFS_OK;
 }
 
@@ -1808,6 +1810,9 @@ int cliffs_nest_child_step(cliffs_nest* n, double t) {
             if (!n->ended) return fail(CLIFFS_ERR_STATE, "child ahead of the parent feed");
             n->bndr = false;  // feed ended: edges 0 from now on
         } else {
+            // record irec-1 leaves the child's use: its slot is free for the
+            // parent once the child work enqueued so far has run
+            if (n->irec >= 2) HIPCHK(hipEventRecord(n->use_ev[(n->irec - 2) % n->cap], c->st));
             n->irec++;
         }
     }
@@ -1820,15 +1825,16 @@ int cliffs_nest_child_step(cliffs_nest* n, double t) {
         const int s0 = (n->irec - 2) % n->cap, s1 = (n->irec - 1) % n->cap;
         r0 = n->ring + (size_t)s0 * n->nuvq;
         r1 = n->ring + (size_t)s1 * n->nuvq;
-        HIPCHK(hipStreamWaitEvent(c->st, n->rec_ev[s1], 0));  // newest record needed is written
+        // the newest record needed is written (once per record: the parent
+        // writes records in order, so the older one is written too)
+        if (n->waited != n->irec) {
+            HIPCHK(hipStreamWaitEvent(c->st, n->rec_ev[s1], 0));
+            n->waited = n->irec;
+        }
     }
     hipError_t e = launch_child_edges(r0, r1, cc, 1.0 - cc, c->k.grav, n->nxc, n->nyc, zero,
                                       n->cedges, n->cedges + 6 * (size_t)n->nyc, c->st);
     if (e != hipSuccess) return hipfail(e, "child_edges");
-    if (!zero) {
-        HIPCHK(hipEventRecord(n->use_ev[(n->irec - 2) % n->cap], c->st));
-        HIPCHK(hipEventRecord(n->use_ev[(n->irec - 1) % n->cap], c->st));
-    }
     return do_step(c, n->cedges, n->cedges + 6 * (size_t)n->nyc);
 }
 
