@@ -9,7 +9,7 @@ ARCH=${CLIFFS_ARCH:-gfx950}
 FLAGS="-O3 -std=c++17 -fPIC --offload-arch=$ARCH -ffp-contract=off -Xclang -target-feature -Xclang -packed-fp32-ops -Wall"
 cd "$HERE/csrc"
 $HIPCC $FLAGS -c sweeps.hip -o "$HERE/csrc/sweeps.o"
-# the fused step kernel is bound by instruction issue at 168 VGPRs; the
+# the fused step kernel is bound by its x role's critical path and HBM; the
 # scheduler's AMDGPU register-pressure trackers give it a ~1 % faster schedule
 # (same arithmetic, bit-identical results)
 $HIPCC $FLAGS -mllvm -amdgpu-use-amdgpu-trackers -c step_fused.hip -o "$HERE/csrc/step_fused.o"

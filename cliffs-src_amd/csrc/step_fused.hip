@@ -147,8 +147,6 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared<N>&
     const int i0 = cs - 4 + N * lane;
     const int ld_lane = (xrole && i0 >= 0 && i0 < nx) ? 1 : 0;
     const XLane XL{i0, xlane_vmask<N>(nx, i0), &sh.xmt[N * lane]};
-    const int outmask = (xrole && lane >= 1 && lane <= 62) ? XL.vmask : 0;
-    int colwet_x = 0;
 
     // Y role: thread ty owns column cs+ty (window node ty+4)
     const int col = cs + ty;
@@ -157,7 +155,9 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared<N>&
     // max-wave of the column updated here (keepmax) or deferred (loadmax
     // only: the tile is flagged if the update would change a maximum)
     int keepmax = 0, loadmax = 0;
-    int colwet_f = 0;  // bit 0: a node of the column is wet after the step; bit 1: deferred change
+    // bit 0: a node of the column is wet after the step; bit 1: deferred change;
+    // bit 2: a node of the column (rows j0 .. j1-1) is wet after the x-sweep
+    int colwet_f = 0;
 
     if (xrole) {
         // ---- X role: the wave's next row is loaded one group ahead
@@ -204,7 +204,6 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared<N>&
                     stq<N>(&buf[w][1][N * lane], ocp);
                     stq<N>(&buf[w][2][N * lane], ou);
                     stq<N>(&buf[w][3][N * lane], ov);
-                    if (!kExact && q >= j0 && q < j1) colwet_x |= wet_bits<N>(s, ocp, outmask, p);
                 }
             }
             __syncthreads();
@@ -265,6 +264,9 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared<N>&
                             rin.v = buf[k][2][ty + 4];
                         }
                         const YOut f = pipe.advance(r, rin, rv, cc, ny, sh.ymt, sh.zt, yb, p);
+                        // the column's true y-sweep flag (ctr): row q after the x-sweep is
+                        // wet (the pipeline's land bit of its newest row; invalid rows are land)
+                        if (!kExact && q >= j0 && q < j1 && !pipe.has(YPipe::kLandB)) colwet_f |= 4;
                         if (r >= j0 && r < j1) {
                             bool wet = false;
                             if (col_ok) {
@@ -301,11 +303,7 @@ __device__ __forceinline__ void fused_tile(const FArgs& a, int tile, FShared<N>&
         a.rfl_out[(size_t)strip * ny + j0 + k] = any ? 1 : 0;
     }
     if (col_ok) a.cpr_out[(size_t)col * a.cstride + chunk] = colwet_f & 1;
-    if (!kExact && outmask && ld_lane && colwet_x) {
-#pragma unroll
-        for (int k = 0; k < N; k++)
-            if (bit(colwet_x, k)) a.ctr[i0 + k] = 1;
-    }
+    if (!kExact && col_ok && (colwet_f & 4)) a.ctr[col] = 1;
 }
 
 // Eight waves per block (two roles) for 248-column tiles, six for 120-column
