@@ -15,5 +15,6 @@ $HIPCC $FLAGS -c sweeps.hip -o "$HERE/csrc/sweeps.o"
 $HIPCC $FLAGS -mllvm -amdgpu-use-amdgpu-trackers -c step_fused.hip -o "$HERE/csrc/step_fused.o"
 $HIPCC $FLAGS -c nest.hip -o "$HERE/csrc/nest.o"
 $HIPCC $FLAGS -c surrogate.hip -o "$HERE/csrc/surrogate.o"
+$HIPCC $FLAGS -c fp64.hip -o "$HERE/csrc/fp64.o"
 $HIPCC $FLAGS -c capi.cpp -o "$HERE/csrc/capi.o"
-$HIPCC $FLAGS -shared -o "$HERE/libcliffs_gpu.so" "$HERE/csrc/sweeps.o" "$HERE/csrc/step_fused.o" "$HERE/csrc/nest.o" "$HERE/csrc/surrogate.o" "$HERE/csrc/capi.o"
+$HIPCC $FLAGS -shared -o "$HERE/libcliffs_gpu.so" "$HERE/csrc/sweeps.o" "$HERE/csrc/step_fused.o" "$HERE/csrc/nest.o" "$HERE/csrc/surrogate.o" "$HERE/csrc/fp64.o" "$HERE/csrc/capi.o"

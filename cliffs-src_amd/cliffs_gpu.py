@@ -104,6 +104,15 @@ def load_library():
         "cliffs_nest_child_start": ([H, C.c_double, _dp], C.c_int),
         "cliffs_nest_run": ([H, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double,
                              C.c_double, C.c_double, _ip], C.c_int),
+        # FP64 line mode (include/cliffs_fp64.h)
+        "cliffs64_create": ([C.POINTER(_Desc), C.POINTER(H)], C.c_int),
+        "cliffs64_destroy": ([H], C.c_int),
+        "cliffs64_upload_state": ([H, _dp, _dp, _dp], C.c_int),
+        "cliffs64_download_state": ([H, _dp, _dp, _dp], C.c_int),
+        "cliffs64_download_max": ([H, _dp, _dp], C.c_int),
+        "cliffs64_sweep": ([H, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int], C.c_int),
+        "cliffs64_maxwave": ([H], C.c_int),
+        "cliffs64_run": ([H, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -292,6 +301,71 @@ class Grid:
         s = C.c_void_p()
         _chk(load_library().cliffs_stream(self._h, C.byref(s)))
         return s.value or 0
+
+
+class Grid64:
+    """A grid in FP64 line mode (include/cliffs_fp64.h): the reference's
+    state and line procedure in double precision on the GPU, equal to the
+    reference's FP64 arithmetic bit for bit; one thread per line, so much
+    slower than Grid."""
+
+    def __init__(self, dep, s1=None, s2=None, zeta=None, *, dt, ground, crough=0.0, device=0):
+        lib = load_library()
+        dep = _f64(dep)
+        ny, nx = dep.shape
+        self.nx, self.ny = nx, ny
+        s1a = _f64(s1) if s1 is not None and nx > 2 else None
+        s2a = _f64(s2) if s2 is not None and ny > 2 else None
+        za = _f64(zeta) if zeta is not None and ny > 2 else None
+        d = _Desc(nx, ny, _p(dep), _p(s1a), _p(s2a), _p(za), float(dt), float(ground),
+                  float(crough), int(device))
+        h = C.c_void_p()
+        _chk(lib.cliffs64_create(C.byref(d), C.byref(h)))
+        self._h = h
+        self.mx = max(ny if nx > 2 else 1, nx if ny > 2 else 1)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().cliffs64_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload(self, cel, xvel, yvel):
+        shp = (self.ny, self.nx)
+        c, u, v = _f64(cel, shp), _f64(xvel, shp), _f64(yvel, shp)
+        _chk(load_library().cliffs64_upload_state(self._h, _p(c), _p(u), _p(v)))
+
+    def download(self):
+        out = [np.empty((self.ny, self.nx)) for _ in range(3)]
+        _chk(load_library().cliffs64_download_state(self._h, *[_p(a) for a in out]))
+        return tuple(out)
+
+    def download_max(self):
+        out = [np.empty((self.ny, self.nx)) for _ in range(2)]
+        _chk(load_library().cliffs64_download_max(self._h, *[_p(a) for a in out]))
+        return tuple(out)
+
+    def sweep(self, iu, first, last, edge1=None, edge2=None):
+        e1, e2 = Grid._edge(edge1, self.mx), Grid._edge(edge2, self.mx)
+        mx = next((e.shape[1] for e in (e1, e2) if e is not None), 0)
+        _chk(load_library().cliffs64_sweep(self._h, iu, first, last, _p(e1), _p(e2), mx))
+
+    def maxwave(self):
+        _chk(load_library().cliffs64_maxwave(self._h))
+
+    def run(self, nsteps: int):
+        _chk(load_library().cliffs64_run(self._h, int(nsteps)))
 
 
 class Ensemble:

@@ -88,6 +88,9 @@ void pinpoint(const double* xx, int nx, double xp, int* ip_out, double* cp) {
 
 }  // namespace
 
+// The other modules of the library report through cliffs_last_error too.
+int cliffs::set_error(int code, const char* msg) { return fail(code, msg); }
+
 struct cliffs_handle {
     int dev = 0;
     hipStream_t st = nullptr;

@@ -20,6 +20,9 @@
 
 namespace cliffs {
 
+// Records msg as cliffs_last_error() and returns code (capi.cpp).
+int set_error(int code, const char* msg);
+
 constexpr int kXStrip = 248;    // output nodes per x-sweep wave (window 256)
 constexpr int kXRows = 8;       // rows per x-sweep wave
 constexpr int kYRows = 128;     // output rows per y-sweep block

@@ -1,0 +1,586 @@
+// FP64 line mode (include/cliffs_fp64.h): the reference's line procedure
+// (cliffs.f:17-346) in double precision on the device, one thread per line.
+//
+// HBM layout.  The state (c, xvel, yvel), the maxima and the depth are
+// (nx, ny) arrays of doubles with x fastest, the reference's own order.
+// Each thread of a sweep owns one line; the line's work arrays (the
+// reference's per-line scratch: h, u, v, the invariants, their updates, the
+// ghost values per segment, the segment bounds and the land flags) live in
+// one HBM buffer with element i of thread t's array k at
+// (k * (n + 2) + i) * L + t (L threads in the launch), so a wave whose
+// threads step through their lines together touches consecutive addresses.
+//
+// Arithmetic.  The reference's expressions in the reference's order, in
+// double precision, compiled with FP contraction off (no fused
+// multiply-add is formed) and with IEEE division and square root: every
+// operation rounds as the reference's FP64 build rounds it.
+//
+// Parallelism.  The segment scan of a line (cliffs.f:65-111) is sequential
+// along the line, as in the reference; lines are independent, so a sweep
+// runs them all at once.  A step of an n x n grid therefore has n threads
+// per sweep: a fidelity mode for flow that FP32 state cannot follow, not a
+// fast path.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <new>
+#include <string>
+
+#include "../../include/cliffs_fp64.h"
+#include "cliffs_dev.h"
+
+namespace {
+
+constexpr double kG = (double)9.8f;            // g (global_modules.f:17, an FP32 literal)
+constexpr double kSqr2 = (double)1.41421356f;  // cliffs.f:34
+
+// work arrays of a line (1-based, n + 2 slots each)
+enum : int {
+    kH, kU, kV, kVV, kP, kQ, kPP, kQQ, kDX, kSCL, kDEP,
+    kPL1, kPL2, kPL3, kPL4, kPR1, kPR2, kPR3, kPR4,  // ghost (P, Q, v, d) per segment, left and right
+    kDoubles
+};
+enum : int { kLG, kRG, kFL, kInts };  // segment bounds; land (bit 0) and newland (bit 1) flags
+
+struct Line64 {
+    int nx, iu, first, nlines, n;
+    const double* dep;
+    double* cel;
+    double* xvel;
+    double* yvel;
+    const double* s1;
+    const double* s2;
+    const double* zeta;
+    const double* e1;  // edge arrays (mx, 3) column-major, or null (zeros)
+    const double* e2;
+    int mx;
+    double dt, ground, crough, celmin;
+    double* wd;  // kDoubles * (n + 2) * nlines
+    int* wi;     // kInts * (n + 2) * nlines
+};
+
+struct Work {
+    double* d;
+    int* k;
+    size_t np, L;
+    int t;
+    __device__ double& operator()(int a, int i) const { return d[((size_t)a * np + i) * L + t]; }
+    __device__ int& ix(int a, int i) const { return k[((size_t)a * np + i) * L + t]; }
+};
+
+__device__ __forceinline__ bool land_of(const Work& w, int i) { return (w.ix(kFL, i) & 1) != 0; }
+__device__ __forceinline__ bool newland_of(const Work& w, int i) { return (w.ix(kFL, i) & 2) != 0; }
+__device__ __forceinline__ void clear_newland(const Work& w, int i) { w.ix(kFL, i) &= 1; }
+
+__global__ __launch_bounds__(64) void line64_kernel(const Line64 a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nlines) return;
+    const int irw = a.first + t;  // 1-based line
+    const int n = a.n, nx = a.nx, iu = a.iu;
+    const double G = kG, celmin = a.celmin, ground = a.ground, dt = a.dt;
+    const Work w{a.wd, a.wi, (size_t)n + 2, (size_t)a.nlines, t};
+
+    // gather (cliffs.f:36-54): the line's depth, c, normal and tangential
+    // velocity, metric term and spacing; land flags (cliffs.f:58-63)
+    for (int i = 1; i <= n; i++) {
+        const size_t o = iu == 1 ? (size_t)nx * (irw - 1) + (i - 1) : (size_t)(irw - 1) + (size_t)nx * (i - 1);
+        const double h = a.cel[o];
+        w(kDEP, i) = a.dep[o];
+        w(kH, i) = h;
+        w(kU, i) = iu == 1 ? a.xvel[o] : a.yvel[o];
+        w(kV, i) = iu == 1 ? a.yvel[o] : a.xvel[o];
+        w(kSCL, i) = iu == 1 ? 0.0 : a.zeta[i - 1];
+        w.ix(kLG, i) = 0;
+        w.ix(kRG, i) = 0;
+        const int land = h <= celmin ? 1 : 0;
+        w.ix(kFL, i) = land | (land << 1);
+    }
+    for (int i = 1; i <= n - 1; i++)
+        w(kDX, i) = iu == 1 ? a.s1[(size_t)(nx - 1) * (irw - 1) + (i - 1)] : a.s2[i - 1];
+
+    // segment scan with run-up expansion (cliffs.f:65-111)
+    bool water = false;
+    int kseg = 0;
+    if (!land_of(w, 1)) {
+        water = true;
+        kseg = 1;
+        w.ix(kLG, 1) = 1;
+    }
+    const int k1 = kseg + 1;
+    for (int i = 2; i <= n; i++) {
+        if (water) {
+            if (land_of(w, i)) {
+                const double hp = w(kH, i - 1);
+                const double flood = (hp * hp) / G - w(kDEP, i - 1) + w(kDEP, i);
+                if ((flood > ground) && !land_of(w, i - 1) && (w(kU, i - 1) >= 0)) {
+                    // the wet node floods its dry neighbour (right-going run-up)
+                    w(kU, i) = w(kU, i - 1);
+                    w(kV, i) = w(kV, i - 1);
+                    w(kH, i) = celmin;
+                    clear_newland(w, i);
+                } else {
+                    water = false;
+                    w.ix(kRG, kseg) = i;
+                }
+            }
+        } else if (!land_of(w, i)) {
+            water = true;
+            kseg = kseg + 1;
+            const double hi = w(kH, i);
+            const double flood = (hi * hi) / G - w(kDEP, i) + w(kDEP, i - 1);
+            if ((flood > ground) && (w(kU, i) <= 0)) {
+                // left-going run-up onto the dry node before the segment
+                w.ix(kLG, kseg) = (i - 2 > 1) ? i - 2 : 1;
+                clear_newland(w, i - 1);
+                w(kU, i - 1) = w(kU, i);
+                w(kV, i - 1) = w(kV, i);
+                w(kH, i - 1) = celmin;
+            } else {
+                w.ix(kLG, kseg) = i - 1;
+            }
+            if (kseg > 1 && w.ix(kLG, kseg) < w.ix(kRG, kseg - 1)) {
+                // the new segment overlaps the previous one: merge them
+                w.ix(kLG, kseg) = 0;
+                kseg = kseg - 1;
+                w.ix(kRG, kseg) = 0;
+            }
+        }
+    }
+    if (kseg == 0) return;  // cliffs.f:113: the line is left untouched
+    if (w.ix(kRG, kseg) == 0) w.ix(kRG, kseg) = n;
+    int k2 = kseg - 1;
+    if (newland_of(w, w.ix(kRG, kseg))) k2 = kseg;
+
+    // nodes still dry are emptied; invariants inside the segments (cliffs.f:114-136)
+    for (int i = 1; i <= n; i++) {
+        if (newland_of(w, i)) {
+            w(kH, i) = 0;
+            w(kU, i) = 0;
+            w(kV, i) = 0;
+        }
+        w(kVV, i) = w(kV, i);
+        w(kP, i) = 0;
+        w(kQ, i) = 0;
+    }
+    for (int k = 1; k <= kseg; k++) {
+        const int i1 = w.ix(kLG, k) + 1, i2 = w.ix(kRG, k) - 1;
+        for (int i = i1; i <= i2; i++) {
+            const double u = w(kU, i), h = w(kH, i);
+            w(kP, i) = u + 2 * h;
+            w(kQ, i) = u - 2 * h;
+        }
+    }
+    // reflective ghosts at shorelines (cliffs.f:137-152)
+    for (int k = k1; k <= kseg; k++) {
+        const int j = w.ix(kLG, k) + 1;
+        w(kPL1, k) = -w(kQ, j);
+        w(kPL2, k) = -w(kP, j);
+        w(kPL3, k) = w(kV, j);
+        w(kPL4, k) = w(kDEP, j);
+    }
+    for (int k = 1; k <= k2; k++) {
+        const int j = w.ix(kRG, k) - 1;
+        w(kPR1, k) = -w(kQ, j);
+        w(kPR2, k) = -w(kP, j);
+        w(kPR3, k) = w(kV, j);
+        w(kPR4, k) = w(kDEP, j);
+    }
+    // the line's own end nodes (cliffs.f:153-164)
+    if (!newland_of(w, 1)) {
+        const double u = w(kU, 1), h = w(kH, 1);
+        w(kP, 1) = u + 2 * h;
+        w(kQ, 1) = u - 2 * h;
+        w(kPL1, 1) = w(kP, 1);
+        w(kPL2, 1) = w(kQ, 1);
+        w(kPL3, 1) = w(kV, 1);
+        w(kPL4, 1) = w(kDEP, 1);
+    }
+    if (!newland_of(w, n)) {
+        const double u = w(kU, n), h = w(kH, n);
+        w(kP, n) = u + 2 * h;
+        w(kQ, n) = u - 2 * h;
+        w(kPR1, kseg) = w(kP, n);
+        w(kPR2, kseg) = w(kQ, n);
+        w(kPR3, kseg) = w(kV, n);
+        w(kPR4, kseg) = w(kDEP, n);
+    }
+    for (int i = 1; i <= n; i++) {
+        w(kQQ, i) = w(kQ, i);
+        w(kPP, i) = w(kP, i);
+    }
+
+    // VTCS-2 integration of the segments' interior nodes (cliffs.f:168-253)
+    for (int k = 1; k <= kseg; k++) {
+        const int i1 = w.ix(kLG, k) + 1, i2 = w.ix(kRG, k) - 1;
+        for (int i = i1; i <= i2; i++) {
+            double p1, q1, v1, d1, p2, q2, v2, d2;
+            if (i == i1) {
+                p1 = w(kPL1, k), q1 = w(kPL2, k), v1 = w(kPL3, k), d1 = w(kPL4, k);
+            } else {
+                p1 = w(kP, i - 1), q1 = w(kQ, i - 1), v1 = w(kV, i - 1), d1 = w(kDEP, i - 1);
+            }
+            if (i == i2) {
+                p2 = w(kPR1, k), q2 = w(kPR2, k), v2 = w(kPR3, k), d2 = w(kPR4, k);
+            } else {
+                p2 = w(kP, i + 1), q2 = w(kQ, i + 1), v2 = w(kV, i + 1), d2 = w(kDEP, i + 1);
+            }
+            const double uj = w(kU, i), vj = w(kV, i), hc = w(kH, i);
+            const double hj = hc * hc / G;
+            // almost-Manning friction (cliffs.f:204)
+            const double cc = G * a.crough * uj * sqrt((uj * uj + vj * vj) / hj) / hj;
+            const double pj = w(kP, i), qj = w(kQ, i);
+            const double sphr = (pj - qj) * (pj + qj) * w(kSCL, i);
+            const double fm = w(kDX, i - 1), fp = w(kDX, i);
+            const double dtx = dt / (fm + fp);
+            const double dj = w(kDEP, i);
+            const double dpj = G * (d2 - dj), dpm = G * (dj - d1);
+            const bool wet3 = (dj > ground) && (d1 > ground) && (d2 > ground);
+
+            // P family
+            double ej = 3 * pj + qj, e2 = 3 * p2 + q2, e1 = 3 * p1 + q1;
+            double Qpj = (0.125 * (e2 + ej) * (p2 - pj) - dpj) / fp;
+            double Qjm = (0.125 * (ej + e1) * (pj - p1) - dpm) / fm;
+            double Qsum;
+            if ((e1 < 0) && (e2 > 0) && wet3)
+                Qsum = (0.25 * ej * (p2 - p1) - 2 * G * (sqrt(dj * d2) - sqrt(dj * d1))) / (fm + fp);
+            else
+                Qsum = (Qpj + Qjm) / 2;
+            Qsum = Qsum + 0.25 * ej * dtx * (Qjm - Qpj);
+            w(kPP, i) = pj - dt * (Qsum + cc - sphr);
+
+            // Q family
+            ej = 3 * qj + pj, e2 = 3 * q2 + p2, e1 = 3 * q1 + p1;
+            Qpj = (0.125 * (e2 + ej) * (q2 - qj) - dpj) / fp;
+            Qjm = (0.125 * (ej + e1) * (qj - q1) - dpm) / fm;
+            if ((e1 < 0) && (e2 > 0) && wet3)
+                Qsum = (0.25 * ej * (q2 - q1) - 2 * G * (sqrt(dj * d2) - sqrt(dj * d1))) / (fm + fp);
+            else
+                Qsum = (Qpj + Qjm) / 2;
+            Qsum = Qsum + 0.25 * ej * dtx * (Qjm - Qpj);
+            w(kQQ, i) = qj - dt * (Qsum + cc + sphr);
+
+            // tangential velocity (cliffs.f:249-250)
+            Qsum = (v2 - vj) / fp - (vj - v1) / fm;
+            w(kVV, i) = vj - uj * dtx * (v2 - v1 - Qsum * (uj * dt + fp - fm));
+        }
+    }
+    // c and u from the new invariants (cliffs.f:255-262)
+    for (int k = 1; k <= kseg; k++) {
+        const int i1 = w.ix(kLG, k) + 1, i2 = w.ix(kRG, k) - 1;
+        for (int i = i1; i <= i2; i++) {
+            const double pp = w(kPP, i), qq = w(kQQ, i);
+            w(kH, i) = (pp - qq) / 4;
+            w(kU, i) = (pp + qq) / 2;
+        }
+    }
+    // shoreline surface-angle limiter on newly flooded end nodes (cliffs.f:264-285)
+    for (int k = k1; k <= kseg; k++) {
+        const int i = w.ix(kLG, k) + 1, j = i + 1;
+        if (land_of(w, i)) {
+            if (kSqr2 * w(kH, i) > w(kH, j)) w(kH, i) = w(kH, j) / kSqr2;
+            if (w(kU, i) < w(kU, j)) w(kU, i) = w(kU, j);
+        }
+    }
+    for (int k = 1; k <= k2; k++) {
+        const int i = w.ix(kRG, k) - 1, j = i - 1;
+        if (land_of(w, i)) {
+            if (kSqr2 * w(kH, i) > w(kH, j)) w(kH, i) = w(kH, j) / kSqr2;
+            if (w(kU, i) > w(kU, j)) w(kU, i) = w(kU, j);
+        }
+    }
+
+    // open or forced edges at the line's ends (cliffs.f:287-329)
+    {
+        const int mx = a.mx;
+        auto edge = [&](const double* e, int col) {
+            return e ? e[(size_t)(irw - 1) + (size_t)mx * col] : 0.0;
+        };
+        const double b1n = edge(a.e1, iu - 1), b1t = edge(a.e1, 3 - iu - 1), b1h = edge(a.e1, 2);
+        const double b2n = edge(a.e2, iu - 1), b2t = edge(a.e2, 3 - iu - 1), b2h = edge(a.e2, 2);
+        double flood = b1h + w(kDEP, 1);
+        if (!newland_of(w, 1) && !newland_of(w, 2) && (flood > ground)) {
+            const double pj = w(kP, 1), qj = w(kQ, 1), p2 = w(kP, 2), q2 = w(kQ, 2);
+            const double sphr = (pj - qj) * (pj + qj) * w(kSCL, 1);
+            const double Qpj = (3 * (q2 + qj) + p2 + pj) * (q2 - qj) / 8 - G * (w(kDEP, 2) - w(kDEP, 1));
+            const double qq = qj - dt * (Qpj / w(kDX, 1) + sphr);
+            const double pp = b1n + 2 * sqrt(G * flood);
+            const double u1 = w(kU, 1);
+            w(kVV, 1) = u1 < 0 ? w(kV, 1) - dt * (w(kV, 2) - w(kV, 1)) * u1 / w(kDX, 1) : b1t;
+            w(kH, 1) = (pp - qq) / 4;
+            w(kU, 1) = (pp + qq) / 2;
+        }
+        flood = w(kDEP, n) + b2h;
+        if (!newland_of(w, n - 1) && !newland_of(w, n) && (flood > ground)) {
+            const double pj = w(kP, n), qj = w(kQ, n), p1 = w(kP, n - 1), q1 = w(kQ, n - 1);
+            const double sphr = (pj - qj) * (pj + qj) * w(kSCL, n);
+            const double Qjm = (3 * (p1 + pj) + q1 + qj) * (pj - p1) / 8 - G * (w(kDEP, n) - w(kDEP, n - 1));
+            const double pp = pj - dt * (Qjm / w(kDX, n - 1) - sphr);
+            const double qq = b2n - 2 * sqrt(G * flood);
+            const double un = w(kU, n);
+            w(kVV, n) = un > 0 ? w(kV, n) - dt * (w(kV, n) - w(kV, n - 1)) * un / w(kDX, n - 1) : b2t;
+            w(kH, n) = (pp - qq) / 4;
+            w(kU, n) = (pp + qq) / 2;
+        }
+    }
+
+    // scatter (cliffs.f:331-343)
+    for (int i = 1; i <= n; i++) {
+        const size_t o = iu == 1 ? (size_t)nx * (irw - 1) + (i - 1) : (size_t)(irw - 1) + (size_t)nx * (i - 1);
+        a.cel[o] = w(kH, i);
+        if (iu == 1) {
+            a.xvel[o] = w(kU, i);
+            a.yvel[o] = w(kVV, i);
+        } else {
+            a.xvel[o] = w(kVV, i);
+            a.yvel[o] = w(kU, i);
+        }
+    }
+}
+
+// running maxima (cliffs_main.f:347-355)
+__global__ void maxwave64_kernel(const double* __restrict__ cel, const double* __restrict__ xv,
+                                 const double* __restrict__ yv, double* __restrict__ hm,
+                                 double* __restrict__ vm, size_t m) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m) return;
+    double aa = cel[c];
+    if (aa > hm[c]) hm[c] = aa;
+    aa = xv[c] * xv[c] + yv[c] * yv[c];
+    if (aa > vm[c]) vm[c] = aa;
+}
+
+int fail64(int code, const std::string& msg) { return cliffs::set_error(code, msg.c_str()); }
+
+#define HIPCHK64(call)                                                                      \
+    do {                                                                                    \
+        const hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess) return fail64(CLIFFS_ERR_HIP, std::string(#call) + ": " +     \
+                                                                hipGetErrorString(e_));      \
+    } while (0)
+
+}  // namespace
+
+struct cliffs64_handle {
+    int dev = 0, nx = 0, ny = 0, mx = 0;
+    double dt = 0, ground = 0, crough = 0, celmin = 0;
+    bool xrun = false, yrun = false;
+    int ii1 = 1, ii2 = 1, jj1 = 1, jj2 = 1;  // line ranges, 1-based (cliffs_main.f:140-164)
+    bool state = false;
+    double *dep = nullptr, *cel = nullptr, *xvel = nullptr, *yvel = nullptr;
+    double *hmax = nullptr, *velmax = nullptr;
+    double *s1 = nullptr, *s2 = nullptr, *zeta = nullptr;
+    double* edges = nullptr;  // edge1 (mx, 3) then edge2 (mx, 3)
+    double* wd = nullptr;
+    int* wi = nullptr;
+    hipStream_t st = nullptr;
+};
+
+namespace {
+
+void free64(cliffs64_handle* h) {
+    if (!h) return;
+    (void)hipSetDevice(h->dev);
+    if (h->st) (void)hipStreamSynchronize(h->st);
+    for (double* p : {h->dep, h->cel, h->xvel, h->yvel, h->hmax, h->velmax, h->s1, h->s2, h->zeta, h->edges,
+                      h->wd})
+        if (p) (void)hipFree(p);
+    if (h->wi) (void)hipFree(h->wi);
+    if (h->st) (void)hipStreamDestroy(h->st);
+    delete h;
+}
+
+int sweep64(cliffs64_handle* h, int iu, int first, int last, const double* e1, const double* e2, int mx) {
+    const int n = iu == 1 ? h->nx : h->ny;
+    Line64 a;
+    a.nx = h->nx;
+    a.iu = iu;
+    a.first = first;
+    a.nlines = last - first + 1;
+    a.n = n;
+    a.dep = h->dep;
+    a.cel = h->cel;
+    a.xvel = h->xvel;
+    a.yvel = h->yvel;
+    a.s1 = h->s1;
+    a.s2 = h->s2;
+    a.zeta = h->zeta;
+    a.e1 = e1;
+    a.e2 = e2;
+    a.mx = mx;
+    a.dt = h->dt;
+    a.ground = h->ground;
+    a.crough = h->crough;
+    a.celmin = h->celmin;
+    a.wd = h->wd;
+    a.wi = h->wi;
+    hipLaunchKernelGGL(line64_kernel, dim3((a.nlines + 63) / 64), dim3(64), 0, h->st, a);
+    HIPCHK64(hipGetLastError());
+    return CLIFFS_OK;
+}
+
+int maxwave64(cliffs64_handle* h) {
+    const size_t m = (size_t)h->nx * h->ny;
+    hipLaunchKernelGGL(maxwave64_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, h->st, h->cel, h->xvel,
+                       h->yvel, h->hmax, h->velmax, m);
+    HIPCHK64(hipGetLastError());
+    return CLIFFS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cliffs64_create(const cliffs_grid_desc* d, cliffs64_handle** out) {
+    if (!d || !out) return fail64(CLIFFS_ERR_ARG, "NULL descriptor or output pointer");
+    *out = nullptr;
+    if (d->nx < 1 || d->ny < 1 || !d->dep) return fail64(CLIFFS_ERR_ARG, "grid must be at least 1x1 with a depth array");
+    if (!(d->ground > 0.0)) return fail64(CLIFFS_ERR_ARG, "ground must be > 0 (c_min = sqrt(g ground))");
+    if (!(d->dt > 0.0)) return fail64(CLIFFS_ERR_ARG, "dt must be > 0");
+    if (d->nx > 2 && !d->s1) return fail64(CLIFFS_ERR_ARG, "s1 required when nx > 2");
+    if (d->ny > 2 && (!d->s2 || !d->zeta)) return fail64(CLIFFS_ERR_ARG, "s2 and zeta required when ny > 2");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail64(CLIFFS_ERR_NODEVICE, "no HIP device");
+    if (d->device < 0 || d->device >= ndev) return fail64(CLIFFS_ERR_NODEVICE, "device index out of range");
+
+    cliffs64_handle* h = new (std::nothrow) cliffs64_handle;
+    if (!h) return fail64(CLIFFS_ERR_NOMEM, "host allocation");
+    const int nx = d->nx, ny = d->ny;
+    h->dev = d->device;
+    h->nx = nx;
+    h->ny = ny;
+    h->dt = d->dt;
+    h->ground = d->ground;
+    h->crough = d->crough;
+    h->celmin = std::sqrt(kG * d->ground);  // cliffs_main.f:227
+    h->xrun = nx > 2;
+    h->yrun = ny > 2;
+    h->ii1 = h->xrun ? 2 : 1;
+    h->ii2 = h->xrun ? nx - 1 : 1;
+    h->jj1 = h->yrun ? 2 : 1;
+    h->jj2 = h->yrun ? ny - 1 : 1;
+    h->mx = std::max(h->xrun ? ny : 1, h->yrun ? nx : 1);
+
+    auto bail = [&](int rc) {
+        free64(h);
+        return rc;
+    };
+    if (hipSetDevice(h->dev) != hipSuccess) return bail(fail64(CLIFFS_ERR_HIP, "hipSetDevice"));
+    if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail64(CLIFFS_ERR_HIP, "stream creation"));
+    const size_t m = (size_t)nx * ny;
+    // work arrays for the longer of the two sweeps' (nodes + 2) x lines
+    const size_t wn = std::max((size_t)(nx + 2) * ny, (size_t)(ny + 2) * nx);
+    const size_t n_s1 = nx > 1 ? (size_t)(nx - 1) * ny : 1, n_s2 = ny > 1 ? (size_t)(ny - 1) : 1;
+    bool ok = hipMalloc(&h->dep, m * 8) == hipSuccess && hipMalloc(&h->cel, m * 8) == hipSuccess &&
+              hipMalloc(&h->xvel, m * 8) == hipSuccess && hipMalloc(&h->yvel, m * 8) == hipSuccess &&
+              hipMalloc(&h->hmax, m * 8) == hipSuccess && hipMalloc(&h->velmax, m * 8) == hipSuccess &&
+              hipMalloc(&h->s1, n_s1 * 8) == hipSuccess && hipMalloc(&h->s2, n_s2 * 8) == hipSuccess &&
+              hipMalloc(&h->zeta, (size_t)ny * 8) == hipSuccess &&
+              hipMalloc(&h->edges, (size_t)6 * h->mx * 8) == hipSuccess &&
+              hipMalloc(&h->wd, (size_t)kDoubles * wn * 8) == hipSuccess &&
+              hipMalloc(&h->wi, (size_t)kInts * wn * 4) == hipSuccess;
+    if (!ok) return bail(fail64(CLIFFS_ERR_NOMEM, "device allocation"));
+    ok = hipMemcpyAsync(h->dep, d->dep, m * 8, hipMemcpyHostToDevice, h->st) == hipSuccess &&
+         (nx <= 2 || hipMemcpyAsync(h->s1, d->s1, n_s1 * 8, hipMemcpyHostToDevice, h->st) == hipSuccess) &&
+         (ny <= 2 || (hipMemcpyAsync(h->s2, d->s2, n_s2 * 8, hipMemcpyHostToDevice, h->st) == hipSuccess &&
+                      hipMemcpyAsync(h->zeta, d->zeta, (size_t)ny * 8, hipMemcpyHostToDevice, h->st) ==
+                          hipSuccess)) &&
+         hipMemsetAsync(h->cel, 0, m * 8, h->st) == hipSuccess &&
+         hipMemsetAsync(h->xvel, 0, m * 8, h->st) == hipSuccess &&
+         hipMemsetAsync(h->yvel, 0, m * 8, h->st) == hipSuccess &&
+         hipMemsetAsync(h->hmax, 0, m * 8, h->st) == hipSuccess &&
+         hipMemsetAsync(h->velmax, 0, m * 8, h->st) == hipSuccess &&
+         hipStreamSynchronize(h->st) == hipSuccess;
+    if (!ok) return bail(fail64(CLIFFS_ERR_HIP, "grid upload"));
+    *out = h;
+    return CLIFFS_OK;
+}
+
+int cliffs64_destroy(cliffs64_handle* h) {
+    free64(h);
+    return CLIFFS_OK;
+}
+
+int cliffs64_upload_state(cliffs64_handle* h, const double* cel, const double* xvel, const double* yvel) {
+    if (!h || !cel || !xvel || !yvel) return fail64(CLIFFS_ERR_ARG, "NULL argument");
+    HIPCHK64(hipSetDevice(h->dev));
+    const size_t b = (size_t)h->nx * h->ny * 8;
+    HIPCHK64(hipMemcpyAsync(h->cel, cel, b, hipMemcpyHostToDevice, h->st));
+    HIPCHK64(hipMemcpyAsync(h->xvel, xvel, b, hipMemcpyHostToDevice, h->st));
+    HIPCHK64(hipMemcpyAsync(h->yvel, yvel, b, hipMemcpyHostToDevice, h->st));
+    HIPCHK64(hipStreamSynchronize(h->st));
+    h->state = true;
+    return CLIFFS_OK;
+}
+
+int cliffs64_download_state(cliffs64_handle* h, double* cel, double* xvel, double* yvel) {
+    if (!h || !cel || !xvel || !yvel) return fail64(CLIFFS_ERR_ARG, "NULL argument");
+    HIPCHK64(hipSetDevice(h->dev));
+    const size_t b = (size_t)h->nx * h->ny * 8;
+    HIPCHK64(hipMemcpyAsync(cel, h->cel, b, hipMemcpyDeviceToHost, h->st));
+    HIPCHK64(hipMemcpyAsync(xvel, h->xvel, b, hipMemcpyDeviceToHost, h->st));
+    HIPCHK64(hipMemcpyAsync(yvel, h->yvel, b, hipMemcpyDeviceToHost, h->st));
+    HIPCHK64(hipStreamSynchronize(h->st));
+    return CLIFFS_OK;
+}
+
+int cliffs64_download_max(cliffs64_handle* h, double* hmax, double* velmax) {
+    if (!h || !hmax || !velmax) return fail64(CLIFFS_ERR_ARG, "NULL argument");
+    HIPCHK64(hipSetDevice(h->dev));
+    const size_t b = (size_t)h->nx * h->ny * 8;
+    HIPCHK64(hipMemcpyAsync(hmax, h->hmax, b, hipMemcpyDeviceToHost, h->st));
+    HIPCHK64(hipMemcpyAsync(velmax, h->velmax, b, hipMemcpyDeviceToHost, h->st));
+    HIPCHK64(hipStreamSynchronize(h->st));
+    return CLIFFS_OK;
+}
+
+int cliffs64_sweep(cliffs64_handle* h, int iu, int first, int last, const double* edge1, const double* edge2,
+                   int mx) {
+    if (!h) return fail64(CLIFFS_ERR_ARG, "NULL handle");
+    if (!h->state) return fail64(CLIFFS_ERR_STATE, "no state uploaded");
+    if (iu != 1 && iu != 2) return fail64(CLIFFS_ERR_ARG, "iu must be 1 or 2");
+    const int nl = iu == 1 ? h->ny : h->nx;
+    const int n = iu == 1 ? h->nx : h->ny;
+    if (first < 1 || last > nl || first > last) return fail64(CLIFFS_ERR_ARG, "line range out of grid");
+    if (n < 3) return fail64(CLIFFS_ERR_ARG, "line length must be >= 3");
+    if ((edge1 || edge2) && mx < last) return fail64(CLIFFS_ERR_ARG, "edge arrays shorter than the lines swept");
+    HIPCHK64(hipSetDevice(h->dev));
+    const double *e1 = nullptr, *e2 = nullptr;
+    if (edge1 || edge2) {
+        if (mx > h->mx) return fail64(CLIFFS_ERR_ARG, "edge arrays longer than the grid's mx");
+        const size_t b = (size_t)3 * mx * 8;
+        if (edge1) HIPCHK64(hipMemcpyAsync(h->edges, edge1, b, hipMemcpyHostToDevice, h->st));
+        if (edge2) HIPCHK64(hipMemcpyAsync(h->edges + (size_t)3 * h->mx, edge2, b, hipMemcpyHostToDevice, h->st));
+        e1 = edge1 ? h->edges : nullptr;
+        e2 = edge2 ? h->edges + (size_t)3 * h->mx : nullptr;
+    }
+    int rc = sweep64(h, iu, first, last, e1, e2, mx);
+    if (rc) return rc;
+    // the edge staging buffer is reused by the next call
+    if (e1 || e2) HIPCHK64(hipStreamSynchronize(h->st));
+    return CLIFFS_OK;
+}
+
+int cliffs64_maxwave(cliffs64_handle* h) {
+    if (!h) return fail64(CLIFFS_ERR_ARG, "NULL handle");
+    if (!h->state) return fail64(CLIFFS_ERR_STATE, "no state uploaded");
+    HIPCHK64(hipSetDevice(h->dev));
+    return maxwave64(h);
+}
+
+int cliffs64_run(cliffs64_handle* h, int nsteps) {
+    if (!h || nsteps < 0) return fail64(CLIFFS_ERR_ARG, "bad run arguments");
+    if (!h->state) return fail64(CLIFFS_ERR_STATE, "no state uploaded");
+    HIPCHK64(hipSetDevice(h->dev));
+    for (int s = 0; s < nsteps; s++) {
+        int rc;
+        if (h->xrun && (rc = sweep64(h, 1, h->jj1, h->jj2, nullptr, nullptr, h->mx))) return rc;
+        if (h->yrun && (rc = sweep64(h, 2, h->ii1, h->ii2, nullptr, nullptr, h->mx))) return rc;
+        if ((rc = maxwave64(h))) return rc;
+    }
+    HIPCHK64(hipStreamSynchronize(h->st));
+    return CLIFFS_OK;
+}
+
+}  // extern "C"

@@ -5,9 +5,12 @@ cells of full-size scenarios of the batched-ensemble workload
 
     OMP_NUM_THREADS=8 python tests/golden/make_ensemble_runup.py [seed ...]
 
-The GPU test test_ensemble_runup_parity compares the device against these
-values; an oracle run of one scenario takes 1-3 minutes on 8 cores, too long
-to repeat inside a GPU test."""
+A key with the suffix "f" (e.g. 131f) is the same scenario frictionless with
+a 1 cm minimal flow depth, the regime where FP32 state and FP64 part.
+
+The GPU tests test_ensemble_runup_parity and test_fp64_full_size_scenario
+compare the device against these values; an oracle run of one scenario
+takes 1-3 minutes on 8 cores, too long to repeat inside a GPU test."""
 import json
 import os
 import sys
@@ -26,8 +29,9 @@ OUT = os.path.join(HERE, "ensemble_runup.json")
 SEEDS = [0, 1, 3, 5, 131]
 
 
-def run(seed, steps=4000):
-    case = synth.ensemble_scenario(seed, steps)
+def run(seed, steps=4000, frictionless=False):
+    kw = dict(ground=0.01, crough=0.0) if frictionless else {}
+    case = synth.ensemble_scenario(seed, steps, **kw)
     s1, s2, z = case.metrics()
     orc = oracle.Oracle(case.dep, s1, s2, z, dt=case.dt, ground=case.ground, crough=case.crough)
     orc.set_state(case.cel, case.xvel, case.yvel)
@@ -46,9 +50,10 @@ def run(seed, steps=4000):
 
 
 if __name__ == "__main__":
-    seeds = [int(x) for x in sys.argv[1:]] or SEEDS
+    keys = sys.argv[1:] or [str(s) for s in SEEDS]
     have = json.load(open(OUT)) if os.path.exists(OUT) else {}
-    for s in seeds:
-        have[str(s)] = run(s)
-        print(json.dumps(have[str(s)]), flush=True)
-        json.dump(dict(sorted(have.items(), key=lambda kv: int(kv[0]))), open(OUT, "w"), indent=1)
+    for key in keys:
+        have[key] = run(int(key.rstrip("f")), frictionless=key.endswith("f"))
+        print(json.dumps(have[key]), flush=True)
+        json.dump(dict(sorted(have.items(), key=lambda kv: (int(kv[0].rstrip("f")), kv[0]))),
+                  open(OUT, "w"), indent=1)

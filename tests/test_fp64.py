@@ -1,0 +1,176 @@
+"""FP64 line mode (include/cliffs_fp64.h): the reference's state and line
+procedure in double precision on the GPU, against the FP64 oracle.
+
+The bar is bit-identity of state and maxima: both restate cliffs.f in the
+reference's operation order with IEEE double arithmetic and no fused
+multiply-adds, and the oracle is itself bit-identical to the reference's
+own cliffs.f (tests/test_oracle.py).  Cases: run-up over a shelf coast
+with friction, frictionless films running up steep land, the 1-D beach
+(x-sweep only), forced edges, and one of the full-size ensemble scenarios
+whose FP32 trajectory parts from FP64 (its oracle run-up is a committed
+fixture).  CPU tests: the header is plain C and the library exports it.
+"""
+import ctypes
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import cliffs_gpu
+import oracle as O
+import synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HDR = os.path.join(ROOT, "include", "cliffs_fp64.h")
+NAMES = ("cel", "xvel", "yvel", "hmax", "velmax")
+
+
+def _symbols():
+    code = re.sub(r"/\*.*?\*/", "", open(HDR).read(), flags=re.S)
+    return sorted(set(re.findall(r"^\s*int\s+(cliffs64_\w+)\s*\(", code, re.M)))
+
+
+def test_fp64_header_plain_c_and_exported():
+    src = '#include "%s"\nint main(void){return 0;}\n' % HDR
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-x", "c", "-", "-o", os.devnull],
+                   input=src, text=True, check=True)
+    names = _symbols()
+    assert len(names) == 8, names
+    lib = ctypes.CDLL(cliffs_gpu.LIB_PATH)
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def _pair(gpu_lib, case):
+    s1, s2, zeta = case.metrics()
+    orc = O.Oracle(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground, crough=case.crough)
+    orc.set_state(case.cel, case.xvel, case.yvel)
+    g = gpu_lib.Grid64(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground, crough=case.crough)
+    g.upload(case.cel, case.xvel, case.yvel)
+    return orc, g
+
+
+def _same(orc, g):
+    want = orc.get_state()
+    got = tuple(g.download()) + tuple(g.download_max())
+    for x, y, name in zip(got, want, NAMES):
+        assert x.shape == y.shape
+        bad = int((x.view(np.int64) != y.view(np.int64)).sum())
+        assert bad == 0, f"{name}: {bad} cells differ (max abs {float(np.abs(x - y).max()):.3e})"
+
+
+def _flooded(g, orc, case):
+    hm = g.download_max()[0]
+    return int(((hm > orc.celmin) & (case.dep < 0)).sum())
+
+
+@pytest.mark.gpu
+def test_fp64_shelf_coast_runup_with_friction(gpu_lib):
+    """Run-up with almost-Manning friction and a 10 cm minimal flow depth."""
+    case = synth.shelf_coast(320, 240, 7, 400.0, 6.0, 60e3, steps=600, ground=0.1, crough=0.0025)
+    orc, g = _pair(gpu_lib, case)
+    orc.steps(600)
+    g.run(600)
+    _same(orc, g)
+    # the run floods land, so the expansion and limiter paths ran
+    assert _flooded(g, orc, case) > 50
+    g.close()
+
+
+@pytest.mark.gpu
+def test_fp64_coastal_fast_thin_films(gpu_lib):
+    """A frictionless coastal grid whose films run up steep land at up to
+    ~100 m/s (a 1 cm minimal flow depth)."""
+    case = synth.coastal(nx=220, ny=170, seed=5)
+    orc, g = _pair(gpu_lib, case)
+    orc.steps(400)
+    g.run(400)
+    _same(orc, g)
+    assert _flooded(g, orc, case) > 100
+    g.close()
+
+
+@pytest.mark.gpu
+def test_fp64_shelf_coast_thin_fast_flow(gpu_lib):
+    """A small shelf coast with a 6 m dipole, frictionless, 1 cm minimal flow
+    depth: the regime where FP32 state and FP64 part after ~1,000 steps."""
+    case = synth.shelf_coast(320, 240, 7, 400.0, 6.0, 60e3, steps=600, ground=0.01, crough=0.0)
+    orc, g = _pair(gpu_lib, case)
+    orc.steps(600)
+    g.run(600)
+    _same(orc, g)
+    assert _flooded(g, orc, case) > 50
+    g.close()
+
+
+@pytest.mark.gpu
+def test_fp64_beach_1d(gpu_lib):
+    case = synth.beach_1d(seed=3, n=1200)
+    orc, g = _pair(gpu_lib, case)
+    orc.steps(800)
+    g.run(800)
+    _same(orc, g)
+    g.close()
+
+
+@pytest.mark.gpu
+def test_fp64_forced_edges_and_line_ranges(gpu_lib):
+    """Sweeps of chosen line ranges with forced edge values (elevation,
+    normal and tangential velocity at both ends of each line)."""
+    case = synth.coastal(nx=160, ny=130, seed=9)
+    orc, g = _pair(gpu_lib, case)
+    ny, nx = case.dep.shape
+    rng = np.random.default_rng(4)
+    for k in range(30):
+        e1 = rng.normal(0, 0.05, (g.mx, 3))
+        e2 = rng.normal(0, 0.05, (g.mx, 3))
+        orc.set_edges(e1, e2)
+        orc.sweep(1, 2, ny - 1)
+        g.sweep(1, 2, ny - 1, e1, e2)
+        orc.sweep(2, 3 + k % 5, nx - 1 - k % 7)
+        g.sweep(2, 3 + k % 5, nx - 1 - k % 7, e1, e2)
+        orc.maxwave()
+        g.maxwave()
+    _same(orc, g)
+    g.close()
+
+
+@pytest.mark.gpu
+def test_fp64_rejects_bad_arguments(gpu_lib):
+    case = synth.coastal(nx=40, ny=30, seed=1)
+    s1, s2, zeta = case.metrics()
+    g = gpu_lib.Grid64(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground)
+    with pytest.raises(gpu_lib.CliffsError):
+        g.run(1)                      # no state uploaded
+    g.upload(case.cel, case.xvel, case.yvel)
+    for bad in ((3, 2, 5), (1, 0, 5), (1, 5, 4), (2, 2, 41)):
+        with pytest.raises(gpu_lib.CliffsError):
+            g.sweep(*bad)
+    g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("key", ["131f"])
+def test_fp64_full_size_scenario(gpu_lib, key):
+    """Ensemble scenario 131 at full size (1331 x 916, 4000 steps),
+    frictionless with a 1 cm minimal flow depth: there the FP32 state's
+    trajectory leaves FP64's after ~1,250 steps and ends with another
+    run-up; the FP64 line mode gives the FP64 oracle's run-up and flooded
+    area exactly (tests/golden/ensemble_runup.json)."""
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ensemble_runup.json")))[key]
+    case = synth.ensemble_scenario(int(key.rstrip("f")), ground=fx["ground"], crough=fx["crough"])
+    assert list(case.dep.shape) == fx["shape"]
+    s1, s2, zeta = case.metrics()
+    g = gpu_lib.Grid64(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground, crough=case.crough)
+    g.upload(case.cel, case.xvel, case.yvel)
+    for k in range(8):
+        g.run(500)
+        print(f"scenario {key}, FP64 line mode: {500 * (k + 1)} steps", flush=True)
+    hm = g.download_max()[0]
+    cm = float(np.sqrt(np.float64(np.float32(9.8)) * case.ground))
+    assert synth.max_runup(hm, case.dep, cm) == fx["max_runup_m"]
+    assert int(((hm > cm) & (case.dep < 0)).sum()) == fx["flooded_land_cells"]
+    g.close()
