@@ -306,8 +306,8 @@ class Grid:
 class Grid64:
     """A grid in FP64 line mode (include/cliffs_fp64.h): the reference's
     state and line procedure in double precision on the GPU, equal to the
-    reference's FP64 arithmetic bit for bit; one thread per line, so much
-    slower than Grid."""
+    reference's FP64 arithmetic bit for bit; one wave per line with a
+    sequential segment scan, so slower than Grid."""
 
     def __init__(self, dep, s1=None, s2=None, zeta=None, *, dt, ground, crough=0.0, device=0):
         lib = load_library()

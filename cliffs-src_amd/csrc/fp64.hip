@@ -1,14 +1,13 @@
 // FP64 line mode (include/cliffs_fp64.h): the reference's line procedure
-// (cliffs.f:17-346) in double precision on the device, one thread per line.
+// (cliffs.f:17-346) in double precision on the device, one wave per line.
 //
 // HBM layout.  The state (c, xvel, yvel), the maxima and the depth are
 // (nx, ny) arrays of doubles with x fastest, the reference's own order.
-// Each thread of a sweep owns one line; the line's work arrays (the
+// Each wave of a sweep owns one line; the line's work arrays (the
 // reference's per-line scratch: h, u, v, the invariants, their updates, the
-// ghost values per segment, the segment bounds and the land flags) live in
-// one HBM buffer with element i of thread t's array k at
-// (k * (n + 2) + i) * L + t (L threads in the launch), so a wave whose
-// threads step through their lines together touches consecutive addresses.
+// ghost values per segment, the segment bounds and the land flags) are
+// contiguous in one HBM buffer, so the wave's lanes, one node each, touch
+// consecutive addresses.
 //
 // Arithmetic.  The reference's expressions in the reference's order, in
 // double precision, compiled with FP contraction off (no fused
@@ -16,10 +15,13 @@
 // operation rounds as the reference's FP64 build rounds it.
 //
 // Parallelism.  The segment scan of a line (cliffs.f:65-111) is sequential
-// along the line, as in the reference; lines are independent, so a sweep
-// runs them all at once.  A step of an n x n grid therefore has n threads
-// per sweep: a fidelity mode for flow that FP32 state cannot follow, not a
-// fast path.
+// along the line, as in the reference, and runs on one lane; the other
+// passes (gather, invariants, ghosts, integration, reconstruction,
+// limiter, scatter) are independent per node or per segment and run on
+// the line's 64 lanes.  Lines are independent, so a sweep runs them all at
+// once.  It is a fidelity mode for flow that FP32 state cannot follow: the
+// scan and the FP64 traffic make it several times slower than the FP32
+// fused step.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -56,35 +58,50 @@ struct Line64 {
     const double* e2;
     int mx;
     double dt, ground, crough, celmin;
-    double* wd;  // kDoubles * (n + 2) * nlines
-    int* wi;     // kInts * (n + 2) * nlines
+    double* wd;  // nlines * kDoubles * (n + 2)
+    int* wi;     // nlines * kInts * (n + 2)
 };
 
+// Work arrays of one line: array k, element i at (line * kDoubles + k) * np + i
+// (np = n + 2), so the lanes of the line's wave touch consecutive addresses.
 struct Work {
     double* d;
     int* k;
-    size_t np, L;
-    int t;
-    __device__ double& operator()(int a, int i) const { return d[((size_t)a * np + i) * L + t]; }
-    __device__ int& ix(int a, int i) const { return k[((size_t)a * np + i) * L + t]; }
+    size_t np;
+    __device__ double& operator()(int a, int i) const { return d[(size_t)a * np + i]; }
+    __device__ int& ix(int a, int i) const { return k[(size_t)a * np + i]; }
 };
 
 __device__ __forceinline__ bool land_of(const Work& w, int i) { return (w.ix(kFL, i) & 1) != 0; }
 __device__ __forceinline__ bool newland_of(const Work& w, int i) { return (w.ix(kFL, i) & 2) != 0; }
 __device__ __forceinline__ void clear_newland(const Work& w, int i) { w.ix(kFL, i) &= 1; }
 
-__global__ __launch_bounds__(64) void line64_kernel(const Line64 a) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nlines) return;
-    const int irw = a.first + t;  // 1-based line
+// The lanes of a wave read what other lanes of it wrote to the work arrays
+// in the pass before (all of one work-group: one L1).
+__device__ __forceinline__ void pass_done() { __threadfence_block(); }
+
+constexpr int kLineWaves = 4;  // lines per block
+
+// One wave per line.  Lane 0 runs the segment scan (sequential along the
+// line, cliffs.f:65-111); every other pass is independent per node (or per
+// segment) and runs across the 64 lanes.
+__global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a) {
+    const int lane = threadIdx.x & 63;
+    const int t = blockIdx.x * kLineWaves + (threadIdx.x >> 6);  // line of this wave
+    if (t >= a.nlines) return;                                   // wave-uniform
+    const int irw = a.first + t;                                 // 1-based line
     const int n = a.n, nx = a.nx, iu = a.iu;
     const double G = kG, celmin = a.celmin, ground = a.ground, dt = a.dt;
-    const Work w{a.wd, a.wi, (size_t)n + 2, (size_t)a.nlines, t};
+    const size_t np = (size_t)n + 2;
+    const Work w{a.wd + (size_t)t * kDoubles * np, a.wi + (size_t)t * kInts * np, np};
+    auto at = [&](int i) {
+        return iu == 1 ? (size_t)nx * (irw - 1) + (i - 1) : (size_t)(irw - 1) + (size_t)nx * (i - 1);
+    };
 
     // gather (cliffs.f:36-54): the line's depth, c, normal and tangential
     // velocity, metric term and spacing; land flags (cliffs.f:58-63)
-    for (int i = 1; i <= n; i++) {
-        const size_t o = iu == 1 ? (size_t)nx * (irw - 1) + (i - 1) : (size_t)(irw - 1) + (size_t)nx * (i - 1);
+    for (int i = 1 + lane; i <= n; i += 64) {
+        const size_t o = at(i);
         const double h = a.cel[o];
         w(kDEP, i) = a.dep[o];
         w(kH, i) = h;
@@ -95,65 +112,73 @@ __global__ __launch_bounds__(64) void line64_kernel(const Line64 a) {
         w.ix(kRG, i) = 0;
         const int land = h <= celmin ? 1 : 0;
         w.ix(kFL, i) = land | (land << 1);
+        if (i <= n - 1) w(kDX, i) = iu == 1 ? a.s1[(size_t)(nx - 1) * (irw - 1) + (i - 1)] : a.s2[i - 1];
     }
-    for (int i = 1; i <= n - 1; i++)
-        w(kDX, i) = iu == 1 ? a.s1[(size_t)(nx - 1) * (irw - 1) + (i - 1)] : a.s2[i - 1];
+    pass_done();
 
-    // segment scan with run-up expansion (cliffs.f:65-111)
-    bool water = false;
-    int kseg = 0;
-    if (!land_of(w, 1)) {
-        water = true;
-        kseg = 1;
-        w.ix(kLG, 1) = 1;
-    }
-    const int k1 = kseg + 1;
-    for (int i = 2; i <= n; i++) {
-        if (water) {
-            if (land_of(w, i)) {
-                const double hp = w(kH, i - 1);
-                const double flood = (hp * hp) / G - w(kDEP, i - 1) + w(kDEP, i);
-                if ((flood > ground) && !land_of(w, i - 1) && (w(kU, i - 1) >= 0)) {
-                    // the wet node floods its dry neighbour (right-going run-up)
-                    w(kU, i) = w(kU, i - 1);
-                    w(kV, i) = w(kV, i - 1);
-                    w(kH, i) = celmin;
-                    clear_newland(w, i);
+    // segment scan with run-up expansion (cliffs.f:65-111), lane 0
+    int kseg = 0, k1 = 1, k2 = 0;
+    if (lane == 0) {
+        bool water = false;
+        if (!land_of(w, 1)) {
+            water = true;
+            kseg = 1;
+            w.ix(kLG, 1) = 1;
+        }
+        k1 = kseg + 1;
+        for (int i = 2; i <= n; i++) {
+            if (water) {
+                if (land_of(w, i)) {
+                    const double hp = w(kH, i - 1);
+                    const double flood = (hp * hp) / G - w(kDEP, i - 1) + w(kDEP, i);
+                    if ((flood > ground) && !land_of(w, i - 1) && (w(kU, i - 1) >= 0)) {
+                        // the wet node floods its dry neighbour (right-going run-up)
+                        w(kU, i) = w(kU, i - 1);
+                        w(kV, i) = w(kV, i - 1);
+                        w(kH, i) = celmin;
+                        clear_newland(w, i);
+                    } else {
+                        water = false;
+                        w.ix(kRG, kseg) = i;
+                    }
+                }
+            } else if (!land_of(w, i)) {
+                water = true;
+                kseg = kseg + 1;
+                const double hi = w(kH, i);
+                const double flood = (hi * hi) / G - w(kDEP, i) + w(kDEP, i - 1);
+                if ((flood > ground) && (w(kU, i) <= 0)) {
+                    // left-going run-up onto the dry node before the segment
+                    w.ix(kLG, kseg) = (i - 2 > 1) ? i - 2 : 1;
+                    clear_newland(w, i - 1);
+                    w(kU, i - 1) = w(kU, i);
+                    w(kV, i - 1) = w(kV, i);
+                    w(kH, i - 1) = celmin;
                 } else {
-                    water = false;
-                    w.ix(kRG, kseg) = i;
+                    w.ix(kLG, kseg) = i - 1;
+                }
+                if (kseg > 1 && w.ix(kLG, kseg) < w.ix(kRG, kseg - 1)) {
+                    // the new segment overlaps the previous one: merge them
+                    w.ix(kLG, kseg) = 0;
+                    kseg = kseg - 1;
+                    w.ix(kRG, kseg) = 0;
                 }
             }
-        } else if (!land_of(w, i)) {
-            water = true;
-            kseg = kseg + 1;
-            const double hi = w(kH, i);
-            const double flood = (hi * hi) / G - w(kDEP, i) + w(kDEP, i - 1);
-            if ((flood > ground) && (w(kU, i) <= 0)) {
-                // left-going run-up onto the dry node before the segment
-                w.ix(kLG, kseg) = (i - 2 > 1) ? i - 2 : 1;
-                clear_newland(w, i - 1);
-                w(kU, i - 1) = w(kU, i);
-                w(kV, i - 1) = w(kV, i);
-                w(kH, i - 1) = celmin;
-            } else {
-                w.ix(kLG, kseg) = i - 1;
-            }
-            if (kseg > 1 && w.ix(kLG, kseg) < w.ix(kRG, kseg - 1)) {
-                // the new segment overlaps the previous one: merge them
-                w.ix(kLG, kseg) = 0;
-                kseg = kseg - 1;
-                w.ix(kRG, kseg) = 0;
-            }
+        }
+        if (kseg > 0) {
+            if (w.ix(kRG, kseg) == 0) w.ix(kRG, kseg) = n;
+            k2 = kseg - 1;
+            if (newland_of(w, w.ix(kRG, kseg))) k2 = kseg;
         }
     }
+    kseg = __shfl(kseg, 0);
+    k1 = __shfl(k1, 0);
+    k2 = __shfl(k2, 0);
     if (kseg == 0) return;  // cliffs.f:113: the line is left untouched
-    if (w.ix(kRG, kseg) == 0) w.ix(kRG, kseg) = n;
-    int k2 = kseg - 1;
-    if (newland_of(w, w.ix(kRG, kseg))) k2 = kseg;
+    pass_done();
 
-    // nodes still dry are emptied; invariants inside the segments (cliffs.f:114-136)
-    for (int i = 1; i <= n; i++) {
+    // nodes still dry are emptied (cliffs.f:114-124); invariants cleared
+    for (int i = 1 + lane; i <= n; i += 64) {
         if (newland_of(w, i)) {
             w(kH, i) = 0;
             w(kU, i) = 0;
@@ -163,57 +188,65 @@ __global__ __launch_bounds__(64) void line64_kernel(const Line64 a) {
         w(kP, i) = 0;
         w(kQ, i) = 0;
     }
+    pass_done();
+    // Riemann invariants inside the segments (cliffs.f:125-136)
     for (int k = 1; k <= kseg; k++) {
         const int i1 = w.ix(kLG, k) + 1, i2 = w.ix(kRG, k) - 1;
-        for (int i = i1; i <= i2; i++) {
+        for (int i = i1 + lane; i <= i2; i += 64) {
             const double u = w(kU, i), h = w(kH, i);
             w(kP, i) = u + 2 * h;
             w(kQ, i) = u - 2 * h;
         }
     }
-    // reflective ghosts at shorelines (cliffs.f:137-152)
-    for (int k = k1; k <= kseg; k++) {
+    pass_done();
+    // reflective ghosts at shorelines (cliffs.f:137-152), one segment per lane
+    for (int k = k1 + lane; k <= kseg; k += 64) {
         const int j = w.ix(kLG, k) + 1;
         w(kPL1, k) = -w(kQ, j);
         w(kPL2, k) = -w(kP, j);
         w(kPL3, k) = w(kV, j);
         w(kPL4, k) = w(kDEP, j);
     }
-    for (int k = 1; k <= k2; k++) {
+    for (int k = 1 + lane; k <= k2; k += 64) {
         const int j = w.ix(kRG, k) - 1;
         w(kPR1, k) = -w(kQ, j);
         w(kPR2, k) = -w(kP, j);
         w(kPR3, k) = w(kV, j);
         w(kPR4, k) = w(kDEP, j);
     }
-    // the line's own end nodes (cliffs.f:153-164)
-    if (!newland_of(w, 1)) {
-        const double u = w(kU, 1), h = w(kH, 1);
-        w(kP, 1) = u + 2 * h;
-        w(kQ, 1) = u - 2 * h;
-        w(kPL1, 1) = w(kP, 1);
-        w(kPL2, 1) = w(kQ, 1);
-        w(kPL3, 1) = w(kV, 1);
-        w(kPL4, 1) = w(kDEP, 1);
+    pass_done();
+    // the line's own end nodes (cliffs.f:153-164), after the ghosts they may replace
+    if (lane == 0) {
+        if (!newland_of(w, 1)) {
+            const double u = w(kU, 1), h = w(kH, 1);
+            w(kP, 1) = u + 2 * h;
+            w(kQ, 1) = u - 2 * h;
+            w(kPL1, 1) = w(kP, 1);
+            w(kPL2, 1) = w(kQ, 1);
+            w(kPL3, 1) = w(kV, 1);
+            w(kPL4, 1) = w(kDEP, 1);
+        }
+        if (!newland_of(w, n)) {
+            const double u = w(kU, n), h = w(kH, n);
+            w(kP, n) = u + 2 * h;
+            w(kQ, n) = u - 2 * h;
+            w(kPR1, kseg) = w(kP, n);
+            w(kPR2, kseg) = w(kQ, n);
+            w(kPR3, kseg) = w(kV, n);
+            w(kPR4, kseg) = w(kDEP, n);
+        }
     }
-    if (!newland_of(w, n)) {
-        const double u = w(kU, n), h = w(kH, n);
-        w(kP, n) = u + 2 * h;
-        w(kQ, n) = u - 2 * h;
-        w(kPR1, kseg) = w(kP, n);
-        w(kPR2, kseg) = w(kQ, n);
-        w(kPR3, kseg) = w(kV, n);
-        w(kPR4, kseg) = w(kDEP, n);
-    }
-    for (int i = 1; i <= n; i++) {
+    pass_done();
+    for (int i = 1 + lane; i <= n; i += 64) {
         w(kQQ, i) = w(kQ, i);
         w(kPP, i) = w(kP, i);
     }
+    pass_done();
 
     // VTCS-2 integration of the segments' interior nodes (cliffs.f:168-253)
     for (int k = 1; k <= kseg; k++) {
         const int i1 = w.ix(kLG, k) + 1, i2 = w.ix(kRG, k) - 1;
-        for (int i = i1; i <= i2; i++) {
+        for (int i = i1 + lane; i <= i2; i += 64) {
             double p1, q1, v1, d1, p2, q2, v2, d2;
             if (i == i1) {
                 p1 = w(kPL1, k), q1 = w(kPL2, k), v1 = w(kPL3, k), d1 = w(kPL4, k);
@@ -265,33 +298,39 @@ __global__ __launch_bounds__(64) void line64_kernel(const Line64 a) {
             w(kVV, i) = vj - uj * dtx * (v2 - v1 - Qsum * (uj * dt + fp - fm));
         }
     }
+    pass_done();
     // c and u from the new invariants (cliffs.f:255-262)
     for (int k = 1; k <= kseg; k++) {
         const int i1 = w.ix(kLG, k) + 1, i2 = w.ix(kRG, k) - 1;
-        for (int i = i1; i <= i2; i++) {
+        for (int i = i1 + lane; i <= i2; i += 64) {
             const double pp = w(kPP, i), qq = w(kQQ, i);
             w(kH, i) = (pp - qq) / 4;
             w(kU, i) = (pp + qq) / 2;
         }
     }
-    // shoreline surface-angle limiter on newly flooded end nodes (cliffs.f:264-285)
-    for (int k = k1; k <= kseg; k++) {
+    pass_done();
+    // shoreline surface-angle limiter on newly flooded end nodes
+    // (cliffs.f:264-285): all left ends, then all right ends, one segment
+    // per lane (segments are disjoint, so the ends of one pass do not meet)
+    for (int k = k1 + lane; k <= kseg; k += 64) {
         const int i = w.ix(kLG, k) + 1, j = i + 1;
         if (land_of(w, i)) {
             if (kSqr2 * w(kH, i) > w(kH, j)) w(kH, i) = w(kH, j) / kSqr2;
             if (w(kU, i) < w(kU, j)) w(kU, i) = w(kU, j);
         }
     }
-    for (int k = 1; k <= k2; k++) {
+    pass_done();
+    for (int k = 1 + lane; k <= k2; k += 64) {
         const int i = w.ix(kRG, k) - 1, j = i - 1;
         if (land_of(w, i)) {
             if (kSqr2 * w(kH, i) > w(kH, j)) w(kH, i) = w(kH, j) / kSqr2;
             if (w(kU, i) > w(kU, j)) w(kU, i) = w(kU, j);
         }
     }
+    pass_done();
 
-    // open or forced edges at the line's ends (cliffs.f:287-329)
-    {
+    // open or forced edges at the line's ends (cliffs.f:287-329), lane 0
+    if (lane == 0) {
         const int mx = a.mx;
         auto edge = [&](const double* e, int col) {
             return e ? e[(size_t)(irw - 1) + (size_t)mx * col] : 0.0;
@@ -323,10 +362,11 @@ __global__ __launch_bounds__(64) void line64_kernel(const Line64 a) {
             w(kU, n) = (pp + qq) / 2;
         }
     }
+    pass_done();
 
     // scatter (cliffs.f:331-343)
-    for (int i = 1; i <= n; i++) {
-        const size_t o = iu == 1 ? (size_t)nx * (irw - 1) + (i - 1) : (size_t)(irw - 1) + (size_t)nx * (i - 1);
+    for (int i = 1 + lane; i <= n; i += 64) {
+        const size_t o = at(i);
         a.cel[o] = w(kH, i);
         if (iu == 1) {
             a.xvel[o] = w(kU, i);
@@ -414,7 +454,8 @@ int sweep64(cliffs64_handle* h, int iu, int first, int last, const double* e1, c
     a.celmin = h->celmin;
     a.wd = h->wd;
     a.wi = h->wi;
-    hipLaunchKernelGGL(line64_kernel, dim3((a.nlines + 63) / 64), dim3(64), 0, h->st, a);
+    hipLaunchKernelGGL(line64_kernel, dim3((a.nlines + kLineWaves - 1) / kLineWaves), dim3(64 * kLineWaves), 0,
+                       h->st, a);
     HIPCHK64(hipGetLastError());
     return CLIFFS_OK;
 }
