@@ -242,7 +242,32 @@ def cpu_baseline(case, steps):
             "kind": "port",
             "sample": f"{steps} steps of the {nx}x{ny} {name} grid after 1 warm-up step, "
                       f"FP64 C restatement of cliffs.f with OpenMP over lines, "
-                      f"{dt:.1f} s wall"}
+                      f"{dt:.1f} s wall"}, o.get_state()
+
+
+def fp64_bench(case, steps, want=None):
+    """The FP64 line mode (Grid64) on the same sample as the CPU baseline:
+    1 warm-up step, then `steps` timed steps; with `want` (the CPU
+    baseline's final state and maxima), whether they are bit-identical."""
+    import numpy as np
+
+    import cliffs_gpu
+    s1, s2, zeta = case.metrics()
+    g = cliffs_gpu.Grid64(case.dep, s1, s2, zeta, dt=case.dt, ground=case.ground)
+    g.upload(case.cel, case.xvel, case.yvel)
+    g.run(1)
+    t0 = time.perf_counter()
+    g.run(steps)
+    dt = time.perf_counter() - t0
+    got = tuple(g.download()) + tuple(g.download_max())
+    g.close()
+    ny, nx = case.dep.shape
+    same = None
+    if want is not None:
+        same = all(np.array_equal(a.view(np.int64), b.view(np.int64)) for a, b in zip(got, want))
+    return {"value": round(nx * ny * steps / dt / 1e9, 4), "unit": "Gcell-updates/s", "steps": steps,
+            "ms_per_step": round(dt / steps * 1e3, 3), "dtype": "f64",
+            "bit_identical_to_cpu_baseline": same}
 
 
 def copy_bandwidth(mib=1024, reps=5):
@@ -356,6 +381,8 @@ def main():
                          "scenario (0)")
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="skip the FP64 line mode's run of the CPU baseline's sample")
     ap.add_argument("--ensemble-scenarios", type=int, default=256,
                     help="scenarios of the batched-ensemble measurement (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=4000)
@@ -472,13 +499,19 @@ def main():
                                   0 if args.seed is None else args.seed, args.ensemble_workers)
         log(f"[bench] ensemble: {ensemble['scenarios_per_hr']} scenarios/hr, "
             f"{ensemble['gcell_updates_per_s']} Gcell-updates/s")
-    cpu = None
+    cpu, want = None, None
     if not args.no_cpu:
         log(f"[bench] CPU baseline: {args.cpu_steps} steps of the FP64 oracle")
-        cpu = cpu_baseline(case, args.cpu_steps)
+        cpu, want = cpu_baseline(case, args.cpu_steps)
         log(f"[bench] CPU {cpu['value']:.4f} Gcell-updates/s on {cpu['cores']} cores")
+    fp64 = None
+    if not args.no_fp64:
+        fp64 = fp64_bench(case, args.cpu_steps, want)
+        log(f"[bench] FP64 line mode: {fp64['value']} Gcell-updates/s, bit-identical to the CPU "
+            f"baseline: {fp64['bit_identical_to_cpu_baseline']}")
     out = bench_line(value, args.steps, args.warmup, wall, workload, nx, ny, case.dt, finite, roof,
                      ensemble, cpu)
+    out["fp64_line_mode"] = fp64
     print(json.dumps(out), flush=True)
 
 

@@ -39,6 +39,9 @@ module cliffs_gpu_mod
   public :: cliffs_ensemble_run_timed, cliffs_ensemble_tune, cliffs_ensemble_sync
   public :: cliffs_ensemble_run_gauges
   public :: cliffs_gpu_open, cliffs_error_message
+  ! FP64 line mode (include/cliffs_fp64.h)
+  public :: cliffs64_create, cliffs64_destroy, cliffs64_upload_state, cliffs64_download_state
+  public :: cliffs64_download_max, cliffs64_sweep, cliffs64_maxwave, cliffs64_run
 
   interface
     integer(c_int) function cliffs_create(desc, h) bind(C, name="cliffs_create")
@@ -270,6 +273,46 @@ module cliffs_gpu_mod
       integer(c_int), value :: every, nsamples, n
       integer(c_int), intent(in) :: ig(*), jg(*)
       real(c_float), intent(out) :: eta(*)
+    end function
+    ! FP64 line mode (include/cliffs_fp64.h): the reference's state and line
+    ! procedure in double precision on the device
+    integer(c_int) function cliffs64_create(desc, h) bind(C, name="cliffs64_create")
+      import :: c_int, c_ptr, cliffs_grid_desc
+      type(cliffs_grid_desc), intent(in) :: desc
+      type(c_ptr), intent(out) :: h
+    end function
+    integer(c_int) function cliffs64_destroy(h) bind(C, name="cliffs64_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function cliffs64_upload_state(h, cel, xvel, yvel) bind(C, name="cliffs64_upload_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: cel(*), xvel(*), yvel(*)
+    end function
+    integer(c_int) function cliffs64_download_state(h, cel, xvel, yvel) bind(C, name="cliffs64_download_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: cel(*), xvel(*), yvel(*)
+    end function
+    integer(c_int) function cliffs64_download_max(h, hmax, velmax) bind(C, name="cliffs64_download_max")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: hmax(*), velmax(*)
+    end function
+    integer(c_int) function cliffs64_sweep(h, iu, first, last, edge1, edge2, mx) bind(C, name="cliffs64_sweep")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, edge1, edge2
+      integer(c_int), value :: iu, first, last, mx
+    end function
+    integer(c_int) function cliffs64_maxwave(h) bind(C, name="cliffs64_maxwave")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function cliffs64_run(h, nsteps) bind(C, name="cliffs64_run")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: nsteps
     end function
   end interface
 

@@ -110,8 +110,14 @@ def test_fortran_module_binds_every_header_function():
     path = os.path.join(os.path.dirname(cliffs_gpu.HEADER), "..", "cliffs-src_amd", "fortran",
                         "cliffs_gpu_mod.f90")
     with open(path) as f:
-        bound = set(re.findall(r'bind\(C, name="(cliffs_\w+)"\)', f.read()))
+        txt = f.read()
+    bound = set(re.findall(r'bind\(C, name="(cliffs_\w+)"\)', txt))
     assert set(cliffs_gpu.header_symbols()) == bound
+    # and of include/cliffs_fp64.h (the FP64 line mode)
+    hdr64 = os.path.join(os.path.dirname(cliffs_gpu.HEADER), "cliffs_fp64.h")
+    code = re.sub(r"/\*.*?\*/", "", open(hdr64).read(), flags=re.S)
+    want64 = set(re.findall(r"^\s*int\s+(cliffs64_\w+)\s*\(", code, re.M))
+    assert len(want64) == 8 and want64 == set(re.findall(r'bind\(C, name="(cliffs64_\w+)"\)', txt))
 
 
 def test_surrogate_header_exported_and_plain_c():
