@@ -14,14 +14,14 @@
 // multiply-add is formed) and with IEEE division and square root: every
 // operation rounds as the reference's FP64 build rounds it.
 //
-// Parallelism.  The segment scan of a line (cliffs.f:65-111) is sequential
-// along the line, as in the reference, and runs on one lane; the other
-// passes (gather, invariants, ghosts, integration, reconstruction,
-// limiter, scatter) are independent per node or per segment and run on
-// the line's 64 lanes.  Lines are independent, so a sweep runs them all at
-// once.  It is a fidelity mode for flow that FP32 state cannot follow: the
-// scan and the FP64 traffic make it several times slower than the FP32
-// fused step.
+// Parallelism.  Every pass of a line runs on its wave's 64 lanes: the
+// segment scan (cliffs.f:65-111), sequential in the reference, as local
+// rules per node and a prefix count of the segment starts (see the kernel),
+// the other passes (gather, invariants, ghosts, integration,
+// reconstruction, limiter, scatter) per node or per segment.  Lines are
+// independent, so a sweep runs them all at once.  It is a fidelity mode for
+// flow that FP32 state cannot follow: the FP64 traffic and the passes over
+// the work arrays make it much slower than the FP32 fused step.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,7 +43,12 @@ enum : int {
     kPL1, kPL2, kPL3, kPL4, kPR1, kPR2, kPR3, kPR4,  // ghost (P, Q, v, d) per segment, left and right
     kDoubles
 };
-enum : int { kLG, kRG, kFL, kInts };  // segment bounds; land (bit 0) and newland (bit 1) flags
+enum : int { kLG, kRG, kFL, kInts };  // segment bounds (by segment); per-node flag bits below
+enum : int {
+    kLand = 1, kNewland = 2,                         // as loaded; still dry after the expansions
+    kR = 4, kW = 8, kS = 16, kL = 32, kM = 64, kE = 128,  // decisions of the segment scan
+    kEnd = 256,                                      // a segment ends here (not merged away)
+};
 
 struct Line64 {
     int nx, iu, first, nlines, n;
@@ -72,9 +77,8 @@ struct Work {
     __device__ int& ix(int a, int i) const { return k[(size_t)a * np + i]; }
 };
 
-__device__ __forceinline__ bool land_of(const Work& w, int i) { return (w.ix(kFL, i) & 1) != 0; }
-__device__ __forceinline__ bool newland_of(const Work& w, int i) { return (w.ix(kFL, i) & 2) != 0; }
-__device__ __forceinline__ void clear_newland(const Work& w, int i) { w.ix(kFL, i) &= 1; }
+__device__ __forceinline__ bool land_of(const Work& w, int i) { return (w.ix(kFL, i) & kLand) != 0; }
+__device__ __forceinline__ bool newland_of(const Work& w, int i) { return (w.ix(kFL, i) & kNewland) != 0; }
 
 // The lanes of a wave read what other lanes of it wrote to the work arrays
 // in the pass before (all of one work-group: one L1).
@@ -82,9 +86,7 @@ __device__ __forceinline__ void pass_done() { __threadfence_block(); }
 
 constexpr int kLineWaves = 4;  // lines per block
 
-// One wave per line.  Lane 0 runs the segment scan (sequential along the
-// line, cliffs.f:65-111); every other pass is independent per node (or per
-// segment) and runs across the 64 lanes.
+// One wave per line, one node (or segment) per lane in every pass.
 __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a) {
     const int lane = threadIdx.x & 63;
     const int t = blockIdx.x * kLineWaves + (threadIdx.x >> 6);  // line of this wave
@@ -116,65 +118,89 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
     }
     pass_done();
 
-    // segment scan with run-up expansion (cliffs.f:65-111), lane 0
-    int kseg = 0, k1 = 1, k2 = 0;
-    if (lane == 0) {
-        bool water = false;
-        if (!land_of(w, 1)) {
-            water = true;
-            kseg = 1;
-            w.ix(kLG, 1) = 1;
+    // Segment scan with run-up expansion (cliffs.f:65-111).  The reference
+    // walks the line once; every decision of that walk depends only on a
+    // node and its neighbours' original values, because a node that floods
+    // (run-up expansion) is land and its wet neighbour is never changed:
+    //   R(i)  right expansion: land i, wet i-1, flood from i-1, u(i-1) >= 0
+    //   W(i)  the walk is in water after i: !land(i) || R(i)
+    //   S(i)  a segment starts: wet i after land i-1 not expanded (i = 1: wet)
+    //   L(i)  left expansion at a start: flood from i, u(i) <= 0
+    //   M(i)  the start merges with the segment before (cliffs.f:103-107):
+    //         S, L and W(i-2), i.e. a single land node between the two
+    //   E(i)  the segment ends at land node i: W(i-1) and not R(i)
+    // so the lanes decide them node by node, and a prefix count of the
+    // starts numbers the segments.
+    for (int i = 1 + lane; i <= n; i += 64) {  // R, W
+        int fl = w.ix(kFL, i);
+        bool r = false;
+        if (i >= 2 && (fl & kLand) && !(w.ix(kFL, i - 1) & kLand)) {
+            const double hp = w(kH, i - 1);
+            const double flood = (hp * hp) / G - w(kDEP, i - 1) + w(kDEP, i);
+            r = (flood > ground) && (w(kU, i - 1) >= 0);
         }
-        k1 = kseg + 1;
-        for (int i = 2; i <= n; i++) {
-            if (water) {
-                if (land_of(w, i)) {
-                    const double hp = w(kH, i - 1);
-                    const double flood = (hp * hp) / G - w(kDEP, i - 1) + w(kDEP, i);
-                    if ((flood > ground) && !land_of(w, i - 1) && (w(kU, i - 1) >= 0)) {
-                        // the wet node floods its dry neighbour (right-going run-up)
-                        w(kU, i) = w(kU, i - 1);
-                        w(kV, i) = w(kV, i - 1);
-                        w(kH, i) = celmin;
-                        clear_newland(w, i);
-                    } else {
-                        water = false;
-                        w.ix(kRG, kseg) = i;
-                    }
-                }
-            } else if (!land_of(w, i)) {
-                water = true;
-                kseg = kseg + 1;
-                const double hi = w(kH, i);
-                const double flood = (hi * hi) / G - w(kDEP, i) + w(kDEP, i - 1);
-                if ((flood > ground) && (w(kU, i) <= 0)) {
-                    // left-going run-up onto the dry node before the segment
-                    w.ix(kLG, kseg) = (i - 2 > 1) ? i - 2 : 1;
-                    clear_newland(w, i - 1);
-                    w(kU, i - 1) = w(kU, i);
-                    w(kV, i - 1) = w(kV, i);
-                    w(kH, i - 1) = celmin;
-                } else {
-                    w.ix(kLG, kseg) = i - 1;
-                }
-                if (kseg > 1 && w.ix(kLG, kseg) < w.ix(kRG, kseg - 1)) {
-                    // the new segment overlaps the previous one: merge them
-                    w.ix(kLG, kseg) = 0;
-                    kseg = kseg - 1;
-                    w.ix(kRG, kseg) = 0;
-                }
-            }
-        }
-        if (kseg > 0) {
-            if (w.ix(kRG, kseg) == 0) w.ix(kRG, kseg) = n;
-            k2 = kseg - 1;
-            if (newland_of(w, w.ix(kRG, kseg))) k2 = kseg;
-        }
+        fl |= (r ? kR : 0) | ((!(fl & kLand) || r) ? kW : 0);
+        w.ix(kFL, i) = fl;
     }
-    kseg = __shfl(kseg, 0);
-    k1 = __shfl(k1, 0);
-    k2 = __shfl(k2, 0);
+    pass_done();
+    for (int i = 1 + lane; i <= n; i += 64) {  // S, L, M, E
+        int fl = w.ix(kFL, i);
+        const int fm1 = i >= 2 ? w.ix(kFL, i - 1) : 0;
+        const int fm2 = i >= 3 ? w.ix(kFL, i - 2) : 0;
+        const bool land = fl & kLand;
+        const bool st = i == 1 ? !land : (!land && (fm1 & kLand) && !(fm1 & kR));
+        bool le = false;
+        if (st && i >= 2) {
+            const double hi = w(kH, i);
+            const double flood = (hi * hi) / G - w(kDEP, i) + w(kDEP, i - 1);
+            le = (flood > ground) && (w(kU, i) <= 0);
+        }
+        const bool mg = st && le && i >= 3 && (fm2 & kW);
+        const bool en = i >= 2 && land && (fm1 & kW) && !(fl & kR);
+        fl |= (st ? kS : 0) | (le ? kL : 0) | (mg ? kM : 0) | (en ? kE : 0);
+        w.ix(kFL, i) = fl;
+    }
+    pass_done();
+    for (int i = 1 + lane; i <= n; i += 64) {  // expansions, final newland, ends not merged away
+        int fl = w.ix(kFL, i);
+        const int fp1 = i + 1 <= n ? w.ix(kFL, i + 1) : 0;
+        if (fl & kR) {  // the wet node i-1 floods node i (it is never changed itself)
+            w(kU, i) = w(kU, i - 1);
+            w(kV, i) = w(kV, i - 1);
+            w(kH, i) = celmin;
+            fl &= ~kNewland;
+        }
+        if (fp1 & kL) {  // the segment starting at i+1 floods node i
+            w(kU, i) = w(kU, i + 1);
+            w(kV, i) = w(kV, i + 1);
+            w(kH, i) = celmin;
+            fl &= ~kNewland;
+        }
+        if ((fl & kE) && !(fp1 & kM)) fl |= kEnd;
+        w.ix(kFL, i) = fl;
+    }
+    pass_done();
+    // segment numbers: starts not merged, counted along the line 64 nodes at a time
+    int kseg = 0;
+    for (int c = 0; c < n; c += 64) {
+        const int i = c + 1 + lane;
+        const int fl = i <= n ? w.ix(kFL, i) : 0;
+        const bool st = (fl & kS) && !(fl & kM);
+        const unsigned long long ms = __ballot(st);
+        const int below = __popcll(ms & ((1ull << lane) - 1ull));
+        if (st) w.ix(kLG, kseg + below + 1) = i == 1 ? 1 : ((fl & kL) ? (i - 2 > 1 ? i - 2 : 1) : i - 1);
+        if (fl & kEnd) w.ix(kRG, kseg + below + (st ? 1 : 0)) = i;
+        kseg += __popcll(ms);
+    }
     if (kseg == 0) return;  // cliffs.f:113: the line is left untouched
+    pass_done();
+    const int k1 = (w.ix(kFL, 1) & kLand) ? 1 : 2;
+    int rlast = w.ix(kRG, kseg);
+    if (rlast == 0) {  // the last segment runs to the line's end
+        rlast = n;
+        if (lane == 0) w.ix(kRG, kseg) = n;
+    }
+    const int k2 = (w.ix(kFL, rlast) & kNewland) ? kseg : kseg - 1;
     pass_done();
 
     // nodes still dry are emptied (cliffs.f:114-124); invariants cleared

@@ -139,6 +139,37 @@ def test_fp64_forced_edges_and_line_ranges(gpu_lib):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(8))
+def test_fp64_single_sweeps_random_wetdry(gpu_lib, seed):
+    """One x-sweep and one y-sweep over dense random wet/dry lines (every
+    run-up expansion, segment merge, ghost, limiter and edge branch of the
+    segment scan, on lines longer and shorter than a wave) with forced
+    edges, the metric term and friction on or off: bit-identical."""
+    from test_gpu_parity import _random_wetdry
+    rng = np.random.default_rng(300 + seed)
+    nx, ny = int(rng.integers(40, 300)), int(rng.integers(40, 200))
+    dep, cel, u, v, x, y = _random_wetdry(rng, nx, ny)
+    s1, s2, zeta = synth.metrics(x, y, True)
+    zeta = rng.normal(0, 1e-3, ny)
+    crough = float(rng.choice([0.0, 0.003]))
+    orc = O.Oracle(dep, s1, s2, zeta, dt=0.05, ground=0.01, crough=crough)
+    g = gpu_lib.Grid64(dep, s1, s2, zeta, dt=0.05, ground=0.01, crough=crough)
+    mx = max(nx, ny)
+    for iu, nl in ((1, ny), (2, nx)):
+        e1 = rng.normal(0, 0.2, (mx, 3))
+        e2 = rng.normal(0, 0.2, (mx, 3))
+        orc.set_state(cel, u, v)
+        orc.set_edges(e1, e2)
+        g.upload(cel, u, v)
+        orc.sweep(iu, 1, nl)
+        g.sweep(iu, 1, nl, e1, e2)
+        for a, b, name in zip(g.download(), orc.get_state()[:3], NAMES):
+            bad = int((a.view(np.int64) != b.view(np.int64)).sum())
+            assert bad == 0, f"sweep {iu}, {name}: {bad} cells differ"
+    g.close()
+
+
+@pytest.mark.gpu
 def test_fp64_rejects_bad_arguments(gpu_lib):
     case = synth.coastal(nx=40, ny=30, seed=1)
     s1, s2, zeta = case.metrics()
