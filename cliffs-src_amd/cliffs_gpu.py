@@ -113,6 +113,8 @@ def load_library():
         "cliffs64_sweep": ([H, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int], C.c_int),
         "cliffs64_maxwave": ([H], C.c_int),
         "cliffs64_run": ([H, C.c_int], C.c_int),
+        "cliffs64_gauges": ([H, C.c_int, _ip, _ip, _dp, _dp, _dp], C.c_int),
+        "cliffs64_health": ([H, _dp, C.POINTER(C.c_longlong)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -366,6 +368,21 @@ class Grid64:
 
     def run(self, nsteps: int):
         _chk(load_library().cliffs64_run(self._h, int(nsteps)))
+
+    def gauges(self, I, J):
+        """(eta, u, v) at 1-based gauge indices I, J (output_gages.f:18-29)."""
+        I = np.ascontiguousarray(I, dtype=np.int32)
+        J = np.ascontiguousarray(J, dtype=np.int32)
+        out = [np.empty(len(I)) for _ in range(3)]
+        _chk(load_library().cliffs64_gauges(self._h, len(I), I.ctypes.data_as(_ip), J.ctypes.data_as(_ip),
+                                            *[_p(a) for a in out]))
+        return tuple(out)
+
+    def health(self):
+        """(largest Courant number over wet cells, count of non-finite cells)."""
+        c, n = C.c_double(), C.c_longlong()
+        _chk(load_library().cliffs64_health(self._h, C.byref(c), C.byref(n)))
+        return c.value, n.value
 
 
 class Ensemble:

@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <new>
 #include <string>
 
@@ -416,6 +417,63 @@ __global__ void maxwave64_kernel(const double* __restrict__ cel, const double* _
     if (aa > vm[c]) vm[c] = aa;
 }
 
+// Gauge sampling (output_gages.f:18-29), one thread per gauge.
+__global__ void gauges64_kernel(const double* __restrict__ cel, const double* __restrict__ xv,
+                                const double* __restrict__ yv, const double* __restrict__ dep, int nx, int ny,
+                                double celmin, const int* __restrict__ ij, int n, double* __restrict__ out) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int i = ij[q] - 1, j = ij[n + q] - 1;
+    const size_t o = (size_t)j * nx + i;
+    const double c = cel[o];
+    out[q] = c > celmin ? c * c / kG - dep[o] : -1e32;
+    out[n + q] = xv[o];
+    out[2 * n + q] = yv[o];
+}
+
+// Largest Courant number (|u| + c) dt / dx over wet cells (dx the smaller
+// spacing next to the cell, along the directions swept) and the count of
+// non-finite cells; Courant numbers are >= 0, so their bit patterns order
+// as the values do.
+__global__ void health64_kernel(const double* __restrict__ cel, const double* __restrict__ xv,
+                                const double* __restrict__ yv, const double* __restrict__ s1,
+                                const double* __restrict__ s2, int nx, int ny, int xrun, int yrun, double dt,
+                                double celmin, unsigned long long* out) {
+    const size_t m = (size_t)nx * ny;
+    const size_t c0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double cmax = 0.0;
+    unsigned long long nbad = 0;
+    for (size_t cidx = c0; cidx < m; cidx += (size_t)gridDim.x * blockDim.x) {
+        const double c = cel[cidx], u = xv[cidx], v = yv[cidx];
+        if (!(isfinite(c) && isfinite(u) && isfinite(v))) {
+            nbad++;
+            continue;
+        }
+        if (c <= celmin) continue;
+        const int i = (int)(cidx % nx), j = (int)(cidx / nx);
+        if (xrun) {
+            double rdx = 0.0;
+            if (i >= 1) rdx = fmax(rdx, 1.0 / s1[(size_t)(nx - 1) * j + i - 1]);
+            if (i <= nx - 2) rdx = fmax(rdx, 1.0 / s1[(size_t)(nx - 1) * j + i]);
+            cmax = fmax(cmax, (fabs(u) + c) * dt * rdx);
+        }
+        if (yrun) {
+            double rdy = 0.0;
+            if (j >= 1) rdy = fmax(rdy, 1.0 / s2[j - 1]);
+            if (j <= ny - 2) rdy = fmax(rdy, 1.0 / s2[j]);
+            cmax = fmax(cmax, (fabs(v) + c) * dt * rdy);
+        }
+    }
+    for (int k = 32; k > 0; k >>= 1) {
+        cmax = fmax(cmax, __shfl_xor(cmax, k));
+        nbad += __shfl_xor(nbad, k);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&out[0], (unsigned long long)__double_as_longlong(cmax));
+        if (nbad) atomicAdd(&out[1], nbad);
+    }
+}
+
 int fail64(int code, const std::string& msg) { return cliffs::set_error(code, msg.c_str()); }
 
 #define HIPCHK64(call)                                                                      \
@@ -437,6 +495,9 @@ struct cliffs64_handle {
     double *hmax = nullptr, *velmax = nullptr;
     double *s1 = nullptr, *s2 = nullptr, *zeta = nullptr;
     double* edges = nullptr;  // edge1 (mx, 3) then edge2 (mx, 3)
+    unsigned long long* health = nullptr;  // cliffs64_health result: Courant bits, non-finite count
+    void* gbuf = nullptr;                   // cliffs64_gauges: samples, then indices
+    size_t gcap = 0;
     double* wd = nullptr;
     int* wi = nullptr;
     hipStream_t st = nullptr;
@@ -452,6 +513,8 @@ void free64(cliffs64_handle* h) {
                       h->wd})
         if (p) (void)hipFree(p);
     if (h->wi) (void)hipFree(h->wi);
+    if (h->health) (void)hipFree(h->health);
+    if (h->gbuf) (void)hipFree(h->gbuf);
     if (h->st) (void)hipStreamDestroy(h->st);
     delete h;
 }
@@ -546,7 +609,8 @@ int cliffs64_create(const cliffs_grid_desc* d, cliffs64_handle** out) {
               hipMalloc(&h->zeta, (size_t)ny * 8) == hipSuccess &&
               hipMalloc(&h->edges, (size_t)6 * h->mx * 8) == hipSuccess &&
               hipMalloc(&h->wd, (size_t)kDoubles * wn * 8) == hipSuccess &&
-              hipMalloc(&h->wi, (size_t)kInts * wn * 4) == hipSuccess;
+              hipMalloc(&h->wi, (size_t)kInts * wn * 4) == hipSuccess &&
+              hipMalloc(&h->health, 2 * sizeof(unsigned long long)) == hipSuccess;
     if (!ok) return bail(fail64(CLIFFS_ERR_NOMEM, "device allocation"));
     ok = hipMemcpyAsync(h->dep, d->dep, m * 8, hipMemcpyHostToDevice, h->st) == hipSuccess &&
          (nx <= 2 || hipMemcpyAsync(h->s1, d->s1, n_s1 * 8, hipMemcpyHostToDevice, h->st) == hipSuccess) &&
@@ -634,6 +698,59 @@ int cliffs64_maxwave(cliffs64_handle* h) {
     if (!h->state) return fail64(CLIFFS_ERR_STATE, "no state uploaded");
     HIPCHK64(hipSetDevice(h->dev));
     return maxwave64(h);
+}
+
+int cliffs64_gauges(cliffs64_handle* h, int n, const int* I, const int* J, double* eta, double* u, double* v) {
+    if (!h || n < 0 || (n > 0 && (!I || !J || !eta || !u || !v))) return fail64(CLIFFS_ERR_ARG, "bad gauge arguments");
+    if (!h->state) return fail64(CLIFFS_ERR_STATE, "no state uploaded");
+    if (n == 0) return CLIFFS_OK;
+    for (int q = 0; q < n; q++)
+        if (I[q] < 1 || I[q] > h->nx || J[q] < 1 || J[q] > h->ny) return fail64(CLIFFS_ERR_ARG, "gauge outside the grid");
+    HIPCHK64(hipSetDevice(h->dev));
+    // gauge indices and samples in one device buffer, kept for the next call
+    const size_t need = sizeof(double) * 3 * n + sizeof(int) * 2 * n;
+    if (need > h->gcap) {
+        if (h->gbuf) HIPCHK64(hipFree(h->gbuf));
+        h->gbuf = nullptr;
+        h->gcap = 0;
+        if (hipMalloc(&h->gbuf, need) != hipSuccess) {
+            h->gbuf = nullptr;
+            return fail64(CLIFFS_ERR_NOMEM, "gauge buffer");
+        }
+        h->gcap = need;
+    }
+    double* dout = static_cast<double*>(h->gbuf);
+    int* dij = reinterpret_cast<int*>(dout + 3 * (size_t)n);
+    HIPCHK64(hipMemcpyAsync(dij, I, sizeof(int) * n, hipMemcpyHostToDevice, h->st));
+    HIPCHK64(hipMemcpyAsync(dij + n, J, sizeof(int) * n, hipMemcpyHostToDevice, h->st));
+    hipLaunchKernelGGL(gauges64_kernel, dim3((n + 63) / 64), dim3(64), 0, h->st, h->cel, h->xvel, h->yvel, h->dep,
+                       h->nx, h->ny, h->celmin, dij, n, dout);
+    HIPCHK64(hipGetLastError());
+    HIPCHK64(hipMemcpyAsync(eta, dout, sizeof(double) * n, hipMemcpyDeviceToHost, h->st));
+    HIPCHK64(hipMemcpyAsync(u, dout + n, sizeof(double) * n, hipMemcpyDeviceToHost, h->st));
+    HIPCHK64(hipMemcpyAsync(v, dout + 2 * n, sizeof(double) * n, hipMemcpyDeviceToHost, h->st));
+    HIPCHK64(hipStreamSynchronize(h->st));
+    return CLIFFS_OK;
+}
+
+int cliffs64_health(cliffs64_handle* h, double* courant, long long* nonfinite) {
+    if (!h || !courant || !nonfinite) return fail64(CLIFFS_ERR_ARG, "NULL argument");
+    if (!h->state) return fail64(CLIFFS_ERR_STATE, "no state uploaded");
+    HIPCHK64(hipSetDevice(h->dev));
+    HIPCHK64(hipMemsetAsync(h->health, 0, 2 * sizeof(unsigned long long), h->st));
+    const size_t m = (size_t)h->nx * h->ny;
+    const unsigned blocks = (unsigned)std::min<size_t>((m + 255) / 256, 4096);
+    hipLaunchKernelGGL(health64_kernel, dim3(blocks), dim3(256), 0, h->st, h->cel, h->xvel, h->yvel, h->s1, h->s2,
+                       h->nx, h->ny, h->xrun ? 1 : 0, h->yrun ? 1 : 0, h->dt, h->celmin, h->health);
+    HIPCHK64(hipGetLastError());
+    unsigned long long r[2];
+    HIPCHK64(hipMemcpyAsync(r, h->health, sizeof(r), hipMemcpyDeviceToHost, h->st));
+    HIPCHK64(hipStreamSynchronize(h->st));
+    double c;
+    std::memcpy(&c, &r[0], sizeof(c));
+    *courant = c;
+    *nonfinite = (long long)r[1];
+    return CLIFFS_OK;
 }
 
 int cliffs64_run(cliffs64_handle* h, int nsteps) {

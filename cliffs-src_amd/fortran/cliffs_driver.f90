@@ -487,10 +487,11 @@ contains
     real(dp), intent(inout) :: pp(kx, ky)
     real(dp) :: x1, x2, y1, y2, xcc, ycc, q
     integer :: i, j, ii, jj
-    x1 = xq(1) - 0.1d-20
-    x2 = xq(nxq) + 0.1d-20
-    y1 = yq(1) - 0.1d-20
-    y2 = yq(nyq) + 0.1d-20
+    real(dp), parameter :: tiny_ = real(0.1e-20, dp)  ! an FP32 literal (apply_initial_conditions.f:12)
+    x1 = xq(1) - tiny_
+    x2 = xq(nxq) + tiny_
+    y1 = yq(1) - tiny_
+    y2 = yq(nyq) + tiny_
     if (nxq > 2 .and. nyq > 2) then
       do i = 1, kx
         if (xp(i) < x1 .or. xp(i) > x2) cycle
@@ -772,6 +773,11 @@ program cliffs_gpu_driver
   integer :: nargs, k1, i, j, irec, istep, err, ns, mx, nxsub, nysub, nshot
   logical :: hin, uin, vin, bndr, ended
   logical :: warned = .false.   ! non-finite state reported
+  ! CLIFFS_PRECISION=fp64 in the environment: the FP64 line mode
+  ! (include/cliffs_fp64.h), the reference's FP64 results bit for bit
+  logical :: fp64
+  character(len=16) :: prec
+  integer :: ii1, ii2, jj1, jj2
   real(dp) :: t, cc, celmin, t0, t1
   real(dp), allocatable, target :: cel(:, :), xvel(:, :), yvel(:, :), hmax(:, :), velmax(:, :)
   real(dp), allocatable, target :: xe1(:, :), xe2(:, :), ye1(:, :), ye2(:, :)
@@ -816,7 +822,13 @@ program cliffs_gpu_driver
   write(LOGU, *) trim(adjustl(notes))
   write(LOGU, *) 'Case Title: ', trim(casetitle)
   write(LOGU, *) 'Input Data: ', trim(bprfx), ', ', trim(aprfx)
-  write(LOGU, *) 'Time steps on the GPU (libcliffs_gpu)'
+  call get_environment_variable('CLIFFS_PRECISION', prec)
+  fp64 = trim(prec) == 'fp64'
+  if (fp64) then
+    write(LOGU, *) 'Time steps on the GPU (libcliffs_gpu), FP64 line mode'
+  else
+    write(LOGU, *) 'Time steps on the GPU (libcliffs_gpu)'
+  end if
 
   call read_params(indir, pfile, err)
   if (err /= 0) call finish(1)
@@ -832,6 +844,18 @@ program cliffs_gpu_driver
   mx = 1
   if (xrun) mx = max(mx, ny)
   if (yrun) mx = max(mx, nx)
+  ii1 = 1
+  ii2 = 1
+  jj1 = 1
+  jj2 = 1
+  if (xrun) then
+    ii1 = 2
+    ii2 = nx - 1
+  end if
+  if (yrun) then
+    jj1 = 2
+    jj2 = ny - 1
+  end if
 
   ! inputs (cliffs_main.f:167-187)
   if (trim(bprfx) == '0' .and. trim(aprfx) == '0') then
@@ -878,15 +902,24 @@ program cliffs_gpu_driver
   celmin = sqrt(GRAV * ground)
 
   ! the grid on the device
+  if (fp64 .and. nests > 0) then
+    write(LOGU, *) 'the FP64 line mode has no nest feed: run the nested case without CLIFFS_PRECISION=fp64'
+    write(*, *) 'the FP64 line mode has no nest feed: run the nested case without CLIFFS_PRECISION=fp64'
+    call finish(1)
+  end if
   if (xrun .and. yrun) then
-    rc = cliffs_gpu_open(h, nx, ny, dep, dt, ground, crough, 0, s1, s2, zeta)
+    rc = cliffs_gpu_open(h, nx, ny, dep, dt, ground, crough, 0, s1, s2, zeta, fp64=fp64)
   else if (xrun) then
-    rc = cliffs_gpu_open(h, nx, ny, dep, dt, ground, crough, 0, s1=s1)
+    rc = cliffs_gpu_open(h, nx, ny, dep, dt, ground, crough, 0, s1=s1, fp64=fp64)
   else
-    rc = cliffs_gpu_open(h, nx, ny, dep, dt, ground, crough, 0, s2=s2, zeta=zeta)
+    rc = cliffs_gpu_open(h, nx, ny, dep, dt, ground, crough, 0, s2=s2, zeta=zeta, fp64=fp64)
   end if
   call gpu(rc, 'cliffs_create')
-  call gpu(cliffs_upload_state(h, cel, xvel, yvel), 'cliffs_upload_state')
+  if (fp64) then
+    call gpu(cliffs64_upload_state(h, cel, xvel, yvel), 'cliffs64_upload_state')
+  else
+    call gpu(cliffs_upload_state(h, cel, xvel, yvel), 'cliffs_upload_state')
+  end if
 
   ! nested grids: parent maps on the device and feed files
   do ns = 1, nests
@@ -957,7 +990,15 @@ program cliffs_gpu_driver
     else
       pe = c_null_ptr
     end if
-    call gpu(cliffs_step(h, pe(1), pe(2), pe(3), pe(4), int(mx, c_int)), 'cliffs_step')
+    if (fp64) then   ! x-sweep, y-sweep, maxima (cliffs_main.f:325-355)
+      if (xrun) call gpu(cliffs64_sweep(h, 1, int(jj1, c_int), int(jj2, c_int), pe(1), pe(2), int(mx, c_int)), &
+                         'cliffs64_sweep')
+      if (yrun) call gpu(cliffs64_sweep(h, 2, int(ii1, c_int), int(ii2, c_int), pe(3), pe(4), int(mx, c_int)), &
+                         'cliffs64_sweep')
+      call gpu(cliffs64_maxwave(h), 'cliffs64_maxwave')
+    else
+      call gpu(cliffs_step(h, pe(1), pe(2), pe(3), pe(4), int(mx, c_int)), 'cliffs_step')
+    end if
     if (nests > 0 .and. mod(istep, bndout) == 0) call feed_nests()
     if (mod(istep, seaout) == 0) call snapshot()
     if (ngauges >= 1 .and. mod(istep, gout) == 0) call gauges()
@@ -976,7 +1017,11 @@ program cliffs_gpu_driver
     call close_nc(fsea(i))
   end do
   call close_nc(fmax)
-  call gpu(cliffs_destroy(h), 'cliffs_destroy')
+  if (fp64) then
+    call gpu(cliffs64_destroy(h), 'cliffs64_destroy')
+  else
+    call gpu(cliffs_destroy(h), 'cliffs_destroy')
+  end if
   call cpu_time(t1)
   write(LOGU, *) 'Run finished: ', istep - 1, ' steps, ', t1 - t0, ' s of host CPU time'
   call finish(0)
@@ -1005,7 +1050,11 @@ contains
   subroutine snapshot()
     integer :: ii, jj
     real(dp) :: c
-    call gpu(cliffs_download_state(h, cel, xvel, yvel), 'cliffs_download_state')
+    if (fp64) then
+      call gpu(cliffs64_download_state(h, cel, xvel, yvel), 'cliffs64_download_state')
+    else
+      call gpu(cliffs_download_state(h, cel, xvel, yvel), 'cliffs_download_state')
+    end if
     do j = 1, nysub
       jj = latsub * (j - 1) + 1
       do i = 1, nxsub
@@ -1033,7 +1082,11 @@ contains
   subroutine health()
     real(c_double) :: courant
     integer(c_long_long) :: nbad
-    call gpu(cliffs_health(h, courant, nbad), 'cliffs_health')
+    if (fp64) then
+      call gpu(cliffs64_health(h, courant, nbad), 'cliffs64_health')
+    else
+      call gpu(cliffs_health(h, courant, nbad), 'cliffs_health')
+    end if
     write(LOGU, '(1x,a,f8.4)') 'max Courant number ', courant
     if (nbad > 0 .and. .not. warned) then
       write(LOGU, '(1x,a,i0,a,i0)') 'WARNING: state not finite in ', nbad, ' cells at step ', istep
@@ -1042,12 +1095,20 @@ contains
   end subroutine health
 
   subroutine gauges()
-    call gpu(cliffs_gauges(h, int(ngauges, c_int), ig, jg, geta, gu, gv), 'cliffs_gauges')
+    if (fp64) then
+      call gpu(cliffs64_gauges(h, int(ngauges, c_int), ig, jg, geta, gu, gv), 'cliffs64_gauges')
+    else
+      call gpu(cliffs_gauges(h, int(ngauges, c_int), ig, jg, geta, gu, gv), 'cliffs_gauges')
+    end if
     call write_gauges(fgau, t, geta, gu, gv)
   end subroutine gauges
 
   subroutine maxwave()
-    call gpu(cliffs_download_max(h, hmax, velmax), 'cliffs_download_max')
+    if (fp64) then
+      call gpu(cliffs64_download_max(h, hmax, velmax), 'cliffs64_download_max')
+    else
+      call gpu(cliffs_download_max(h, hmax, velmax), 'cliffs_download_max')
+    end if
     call write_max(fmax, hmax, velmax, celmin)
   end subroutine maxwave
 

@@ -42,6 +42,7 @@ module cliffs_gpu_mod
   ! FP64 line mode (include/cliffs_fp64.h)
   public :: cliffs64_create, cliffs64_destroy, cliffs64_upload_state, cliffs64_download_state
   public :: cliffs64_download_max, cliffs64_sweep, cliffs64_maxwave, cliffs64_run
+  public :: cliffs64_gauges, cliffs64_health
 
   interface
     integer(c_int) function cliffs_create(desc, h) bind(C, name="cliffs_create")
@@ -314,19 +315,34 @@ module cliffs_gpu_mod
       type(c_ptr), value :: h
       integer(c_int), value :: nsteps
     end function
+    integer(c_int) function cliffs64_gauges(h, n, I, J, eta, u, v) bind(C, name="cliffs64_gauges")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: n
+      integer(c_int), intent(in) :: I(*), J(*)
+      real(c_double), intent(out) :: eta(*), u(*), v(*)
+    end function
+    integer(c_int) function cliffs64_health(h, courant, nonfinite) bind(C, name="cliffs64_health")
+      import :: c_int, c_ptr, c_double, c_long_long
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: courant
+      integer(c_long_long), intent(out) :: nonfinite
+    end function
   end interface
 
 contains
 
   ! Create a device grid from the arrays MasterGrid fills (MasterGrid.f:85-121)
   ! and the PARAMETERS dt, ground, crough.  s1/s2/zeta may be absent for 1-D
-  ! grids (nx <= 2 or ny <= 2).
-  integer(c_int) function cliffs_gpu_open(h, nx, ny, dep, dt, ground, crough, device, s1, s2, zeta)
+  ! grids (nx <= 2 or ny <= 2).  fp64 = .true.: a grid in the FP64 line mode
+  ! (cliffs64_create) instead of the FP32 one.
+  integer(c_int) function cliffs_gpu_open(h, nx, ny, dep, dt, ground, crough, device, s1, s2, zeta, fp64)
     type(c_ptr), intent(out) :: h
     integer, intent(in) :: nx, ny, device
     real(c_double), intent(in), target :: dep(nx, ny)
     real(c_double), intent(in) :: dt, ground, crough
     real(c_double), intent(in), target, contiguous, optional :: s1(:, :), s2(:), zeta(:)
+    logical, intent(in), optional :: fp64
     type(cliffs_grid_desc) :: d
     d%nx = nx
     d%ny = ny
@@ -341,6 +357,12 @@ contains
     d%ground = ground
     d%crough = crough
     d%device = device
+    if (present(fp64)) then
+      if (fp64) then
+        cliffs_gpu_open = cliffs64_create(d, h)
+        return
+      end if
+    end if
     cliffs_gpu_open = cliffs_create(d, h)
   end function cliffs_gpu_open
 

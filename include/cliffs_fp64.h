@@ -48,6 +48,13 @@ int cliffs64_sweep(cliffs64_handle *h, int iu, int first, int last, const double
                    const double *edge2, int mx);
 /* The running maxima update of one step (cliffs_main.f:347-355). */
 int cliffs64_maxwave(cliffs64_handle *h);
+/* Gauge sampling (output_gages.f:18-29), as cliffs_gauges: eta = c^2/g - d,
+ * or -1e32 where c <= celmin; u = xvel, v = yvel; I, J 1-based. */
+int cliffs64_gauges(cliffs64_handle *h, int n, const int *I, const int *J, double *eta, double *u,
+                    double *v);
+/* Health of the current state, as cliffs_health: the largest Courant number
+ * (|u| + c) dt / dx over wet cells and the number of non-finite cells. */
+int cliffs64_health(cliffs64_handle *h, double *courant, long long *nonfinite);
 /* nsteps full steps with open edges at rest: x-sweep of rows jj1..jj2,
  * y-sweep of columns ii1..ii2 (cliffs_main.f:140-164, 317-344), maxima. */
 int cliffs64_run(cliffs64_handle *h, int nsteps);

@@ -35,11 +35,12 @@ NAN4 = np.float32(-1.0e32)
 G = synth.GRAV
 
 
-def _drive(args, log):
+def _drive(args, log, env=None):
     if not os.path.exists(DRIVER):
         pytest.fail("cliffs_gpu_driver not built (cliffs-src_amd/fortran/build.sh)")
     print("driver:", " ".join(args), flush=True)
-    out = subprocess.run([DRIVER, *args], capture_output=True, text=True, timeout=300)
+    out = subprocess.run([DRIVER, *args], capture_output=True, text=True, timeout=300,
+                         env=None if env is None else dict(os.environ, **env))
     text = open(log).read() if os.path.exists(log) else ""
     assert out.returncode == 0, out.stdout + out.stderr + text
     return text
@@ -102,6 +103,44 @@ def test_slpbeach_against_oracle(tmp_path, gpu_lib):
           f"run-up cells {runup_cells}")
     assert worst <= 1e-5 * amp and me <= 1e-5 * amp
     assert flips <= 1 and runup_cells <= 1
+
+
+def test_slpbeach_fp64_line_mode_bit_identical(tmp_path, gpu_lib):
+    """CLIFFS_PRECISION=fp64: the driver steps the FP64 line mode; every
+    screenshot and the max-wave file hold the FP64 oracle's values rounded
+    to the files' real*4, bit for bit (wet cells; the NAN value on dry
+    ones)."""
+    out = str(tmp_path) + "/"
+    log = _drive([out + "slp", SLP + "/", "0", "solitary.0185", SLP + "/slpbeach_params.txt"],
+                 out + "slp_log.txt", env={"CLIFFS_PRECISION": "fp64"})
+    assert "FP64 line mode" in log and "not finite" not in log
+    cour = [float(l.split()[-1]) for l in log.splitlines() if "max Courant number" in l]
+    assert len(cour) == 2000 // 3 + 1 and 0.0 < max(cour) < 1.0
+    case = slpbeach_case("0185")
+    # the initial fields as the reference reads them: real*4
+    # (get_initial_state.f:128-150), then added to depth and velocity
+    # (apply_initial_conditions.f) and turned into celerity
+    z = np.load(os.path.join(ROOT, "tests", "golden", "slpbeach.npz"))
+    hq = z["h0185"].astype(np.float32).astype(np.float64).reshape(1, -1)
+    uq = z["u0185"].astype(np.float32).astype(np.float64).reshape(1, -1)
+    zero = np.zeros(1)
+    xvel = O.apply_initial_conditions(z["xq"], zero, uq, z["x"], z["y"], np.zeros_like(case["dep"]))
+    H = O.apply_initial_conditions(z["xq"], zero, hq, z["x"], z["y"], case["dep"])
+    o = O.Oracle(case["dep"], case["s1"], case["s2"], case["zeta"], dt=case["dt"], ground=case["ground"])
+    o.set_state(synth.depth_to_celerity(H, case["ground"]), xvel, np.zeros_like(case["dep"]))
+    ha = _var(out + "slp_sea_h.nc", "ha")[:, 0, :]
+    dep = case["dep"][0]
+    celmin = np.sqrt(G * case["ground"])
+
+    def field(c):
+        return np.where(c > celmin, (c * c / G - dep).astype(np.float32), NAN4)
+
+    for k in range(ha.shape[0]):
+        if k:
+            o.steps(3)
+        _same(ha[k], field(o.get_state()[0][0]), f"screenshot {k}")
+    o.steps(2000 - 3 * (ha.shape[0] - 1))
+    _same(_var(out + "slp_maxwave.nc", "MaxE")[0], field(o.get_state()[3][0]), "MaxE")
 
 
 # --------------------------------------------------------------------------

@@ -38,7 +38,7 @@ def test_fp64_header_plain_c_and_exported():
     subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-x", "c", "-", "-o", os.devnull],
                    input=src, text=True, check=True)
     names = _symbols()
-    assert len(names) == 8, names
+    assert len(names) == 10, names
     lib = ctypes.CDLL(cliffs_gpu.LIB_PATH)
     for n in names:
         assert hasattr(lib, n), n
@@ -90,6 +90,29 @@ def test_fp64_coastal_fast_thin_films(gpu_lib):
     g.run(400)
     _same(orc, g)
     assert _flooded(g, orc, case) > 100
+    # gauges (output_gages.f:18-29) and health of the same state
+    c, u, v = orc.get_state()[:3]
+    rng = np.random.default_rng(2)
+    ny, nx = c.shape
+    I = rng.integers(1, nx + 1, 200)
+    J = rng.integers(1, ny + 1, 200)
+    eta, gu, gv = g.gauges(I, J)
+    cc = c[J - 1, I - 1]
+    want = np.where(cc > orc.celmin, cc * cc / orc.grav - case.dep[J - 1, I - 1], -1e32)
+    assert (cc <= orc.celmin).any() and (cc > orc.celmin).any()
+    assert np.array_equal(eta, want) and np.array_equal(gu, u[J - 1, I - 1]) and np.array_equal(gv, v[J - 1, I - 1])
+    s1, s2, _ = case.metrics()
+    rdx = np.zeros((ny, nx))
+    rdx[:, 1:] = 1.0 / s1
+    rdx[:, :-1] = np.maximum(rdx[:, :-1], 1.0 / s1)
+    rdy = np.zeros(ny)
+    rdy[1:] = 1.0 / s2
+    rdy[:-1] = np.maximum(rdy[:-1], 1.0 / s2)
+    wet = c > orc.celmin
+    cour = max(float(((np.abs(u) + c) * case.dt * rdx)[wet].max()),
+               float(((np.abs(v) + c) * case.dt * rdy[:, None])[wet].max()))
+    got, nbad = g.health()
+    assert nbad == 0 and np.isclose(got, cour, rtol=1e-14, atol=0), (got, cour)
     g.close()
 
 
