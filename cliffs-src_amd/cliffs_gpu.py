@@ -115,6 +115,11 @@ def load_library():
         "cliffs64_run": ([H, C.c_int], C.c_int),
         "cliffs64_gauges": ([H, C.c_int, _ip, _ip, _dp, _dp, _dp], C.c_int),
         "cliffs64_health": ([H, _dp, C.POINTER(C.c_longlong)], C.c_int),
+        "cliffs64_nest_create": ([H, C.c_int, C.c_int, _ip, _dp, _ip, _dp, C.c_int, C.c_double,
+                                  C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, C.c_double,
+                                  C.POINTER(H)], C.c_int),
+        "cliffs64_nest_feed": ([H, C.c_double, C.c_double, _ip, _ip, _dp, _dp, _dp, _dp], C.c_int),
+        "cliffs64_nest_destroy": ([H], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -324,9 +329,12 @@ class Grid64:
         h = C.c_void_p()
         _chk(lib.cliffs64_create(C.byref(d), C.byref(h)))
         self._h = h
+        self._deps = set()     # nests of this grid, closed before it
         self.mx = max(ny if nx > 2 else 1, nx if ny > 2 else 1)
 
     def close(self):
+        for d in list(getattr(self, "_deps", ())):
+            d.close()
         if getattr(self, "_h", None):
             load_library().cliffs64_destroy(self._h)
             self._h = None
@@ -383,6 +391,48 @@ class Grid64:
         c, n = C.c_double(), C.c_longlong()
         _chk(load_library().cliffs64_health(self._h, C.byref(c), C.byref(n)))
         return c.value, n.value
+
+
+class Nest64:
+    """One child grid's boundary feed from a Grid64 parent (feed_children_grid
+    in the reference's FP64 arithmetic)."""
+
+    def __init__(self, parent: Grid64, maps: dict):
+        lib = load_library()
+        self.parent = parent
+        self.nyc, self.nxc = len(maps["we_ny"]), len(maps["sn_nx"])
+        we_ny = np.ascontiguousarray(maps["we_ny"], dtype=np.int32)
+        sn_nx = np.ascontiguousarray(maps["sn_nx"], dtype=np.int32)
+        we_cy, sn_cx = _f64(maps["we_cy"]), _f64(maps["sn_cx"])
+        h = C.c_void_p()
+        _chk(lib.cliffs64_nest_create(parent._h, self.nxc, self.nyc, we_ny.ctypes.data_as(_ip),
+                                      _p(we_cy), sn_nx.ctypes.data_as(_ip), _p(sn_cx),
+                                      int(maps["w_nx"]), float(maps["w_cx"]), int(maps["e_nx"]),
+                                      float(maps["e_cx"]), int(maps["s_ny"]), float(maps["s_cy"]),
+                                      int(maps["n_ny"]), float(maps["n_cy"]), C.byref(h)))
+        self._h = h
+        parent._deps.add(self)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().cliffs64_nest_destroy(self._h)
+            self._h = None
+            self.parent._deps.discard(self)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def feed(self, cuke, hmin, iarv=False):
+        """-> (arrived, fed, uvq_w (nyc,3), uvq_e, uvq_s (nxc,3), uvq_n)."""
+        a, f = C.c_int(1 if iarv else 0), C.c_int(0)
+        w, e = np.zeros((3, self.nyc)), np.zeros((3, self.nyc))
+        s, n = np.zeros((3, self.nxc)), np.zeros((3, self.nxc))
+        _chk(load_library().cliffs64_nest_feed(self._h, float(cuke), float(hmin), C.byref(a),
+                                               C.byref(f), _p(w), _p(e), _p(s), _p(n)))
+        return bool(a.value), bool(f.value), w.T.copy(), e.T.copy(), s.T.copy(), n.T.copy()
 
 
 class Ensemble:

@@ -474,6 +474,84 @@ __global__ void health64_kernel(const double* __restrict__ cel, const double* __
     }
 }
 
+// Nest feed from the FP64 state (feed_children_grid.f:10-179).
+struct Nest64Args {
+    const double* cel;
+    const double* xv;
+    const double* yv;
+    const double* dep;
+    int nx;
+    int nxc, nyc;
+    const int* we_ny;  // 1-based parent rows per child row, weights
+    const double* we_cy;
+    const int* sn_nx;  // 1-based parent columns per child column, weights
+    const double* sn_cx;
+    int w_nx, e_nx, s_ny, n_ny;
+    double w_cx, e_cx, s_cy, n_cy;
+    int ii1, ii2, kk1, kk2;  // footprint, 1-based inclusive
+    double cuke, hmin, thr;
+    unsigned* flag;
+    double* uvq;  // west (3, nyc), east, south (3, nxc), north
+};
+
+// arrival (feed_children_grid.f:26-46): any footprint cell with dep > hmin
+// and |c^2/g - dep| > 0.1 cuke or (|u| + |v|) c > cuke; only 1s are stored
+__global__ void nest64_arrival_kernel(const Nest64Args a) {
+    const int i = a.ii1 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = a.kk1 + blockIdx.y;
+    if (i > a.ii2 || k > a.kk2) return;
+    const size_t o = (size_t)(i - 1) + (size_t)a.nx * (k - 1);
+    if (!(a.dep[o] > a.hmin)) return;
+    const double c = a.cel[o];
+    if (fabs(c * c / kG - a.dep[o]) > a.thr || (fabs(a.xv[o]) + fabs(a.yv[o])) * c > a.cuke) *a.flag = 1u;
+}
+
+// (u, v, eta) of parent cell (i, k), 1-based, times w; 0 on dry cells
+__device__ __forceinline__ void feed64_val(const Nest64Args& a, int i, int k, double w, double out[3]) {
+    const size_t o = (size_t)(i - 1) + (size_t)a.nx * (k - 1);
+    if (a.dep[o] > a.hmin) {
+        const double c = a.cel[o];
+        out[0] = w * a.xv[o];
+        out[1] = w * a.yv[o];
+        out[2] = w * (c * c / kG - a.dep[o]);
+    } else {
+        out[0] = out[1] = out[2] = 0.0;
+    }
+}
+
+// one thread per child edge node: the blend across the two parent lines on
+// either side of the edge, then along the edge (feed_children_grid.f:53-179)
+__global__ void nest64_feed_kernel(const Nest64Args a) {
+    const int side = blockIdx.y;  // 0 west, 1 east, 2 south, 3 north
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int len = side < 2 ? a.nyc : a.nxc;
+    if (q >= len) return;
+    double v0[3], v1[3], w0[3], w1[3], cc;
+    double* out;
+    if (side < 2) {
+        const int ii = side == 0 ? a.w_nx : a.e_nx;
+        const double c0 = side == 0 ? a.w_cx : a.e_cx;
+        const int kk = a.we_ny[q];
+        cc = a.we_cy[q];
+        feed64_val(a, ii, kk, c0, v0);
+        feed64_val(a, ii + 1, kk, 1.0 - c0, v1);
+        feed64_val(a, ii, kk + 1, c0, w0);
+        feed64_val(a, ii + 1, kk + 1, 1.0 - c0, w1);
+        out = a.uvq + (size_t)side * 3 * a.nyc;
+    } else {
+        const int kk = side == 2 ? a.s_ny : a.n_ny;
+        const double c0 = side == 2 ? a.s_cy : a.n_cy;
+        const int ii = a.sn_nx[q];
+        cc = a.sn_cx[q];
+        feed64_val(a, ii, kk, c0, v0);
+        feed64_val(a, ii, kk + 1, 1.0 - c0, v1);
+        feed64_val(a, ii + 1, kk, c0, w0);
+        feed64_val(a, ii + 1, kk + 1, 1.0 - c0, w1);
+        out = a.uvq + (size_t)6 * a.nyc + (size_t)(side - 2) * 3 * a.nxc;
+    }
+    for (int f = 0; f < 3; f++) out[q + (size_t)len * f] = (v0[f] + v1[f]) * cc + (w0[f] + w1[f]) * (1.0 - cc);
+}
+
 int fail64(int code, const std::string& msg) { return cliffs::set_error(code, msg.c_str()); }
 
 #define HIPCHK64(call)                                                                      \
@@ -498,9 +576,16 @@ struct cliffs64_handle {
     unsigned long long* health = nullptr;  // cliffs64_health result: Courant bits, non-finite count
     void* gbuf = nullptr;                   // cliffs64_gauges: samples, then indices
     size_t gcap = 0;
+    int users = 0;                          // nests of this parent (destroyed first)
     double* wd = nullptr;
     int* wi = nullptr;
     hipStream_t st = nullptr;
+};
+
+struct cliffs64_nest {
+    cliffs64_handle* parent = nullptr;
+    Nest64Args a{};
+    void* dmem = nullptr;  // feed output, maps, arrival flag
 };
 
 namespace {
@@ -629,6 +714,7 @@ int cliffs64_create(const cliffs_grid_desc* d, cliffs64_handle** out) {
 }
 
 int cliffs64_destroy(cliffs64_handle* h) {
+    if (h && h->users > 0) return fail64(CLIFFS_ERR_STATE, "grid still used by a nest: destroy the nest first");
     free64(h);
     return CLIFFS_OK;
 }
@@ -764,6 +850,105 @@ int cliffs64_run(cliffs64_handle* h, int nsteps) {
         if ((rc = maxwave64(h))) return rc;
     }
     HIPCHK64(hipStreamSynchronize(h->st));
+    return CLIFFS_OK;
+}
+
+int cliffs64_nest_create(cliffs64_handle* parent, int nxc, int nyc, const int* we_ny, const double* we_cy,
+                         const int* sn_nx, const double* sn_cx, int w_nx, double w_cx, int e_nx, double e_cx,
+                         int s_ny, double s_cy, int n_ny, double n_cy, cliffs64_nest** out) {
+    if (!parent || !out || !we_ny || !we_cy || !sn_nx || !sn_cx || nxc < 2 || nyc < 2)
+        return fail64(CLIFFS_ERR_ARG, "bad nest arguments");
+    *out = nullptr;
+    const int nx = parent->nx, ny = parent->ny;
+    auto in = [](int v, int lo, int hi) { return v >= lo && v <= hi; };
+    for (int q = 0; q < nyc; q++)
+        if (!in(we_ny[q], 1, ny - 1)) return fail64(CLIFFS_ERR_ARG, "child row map outside the parent");
+    for (int q = 0; q < nxc; q++)
+        if (!in(sn_nx[q], 1, nx - 1)) return fail64(CLIFFS_ERR_ARG, "child column map outside the parent");
+    if (!in(w_nx, 1, nx - 1) || !in(e_nx, 1, nx - 1) || !in(s_ny, 1, ny - 1) || !in(n_ny, 1, ny - 1))
+        return fail64(CLIFFS_ERR_ARG, "child edge lines outside the parent");
+    cliffs64_nest* n = new (std::nothrow) cliffs64_nest;
+    if (!n) return fail64(CLIFFS_ERR_NOMEM, "host allocation");
+    n->parent = parent;
+    Nest64Args& a = n->a;
+    a.nx = nx;
+    a.nxc = nxc;
+    a.nyc = nyc;
+    a.w_nx = w_nx, a.e_nx = e_nx, a.s_ny = s_ny, a.n_ny = n_ny;
+    a.w_cx = w_cx, a.e_cx = e_cx, a.s_cy = s_cy, a.n_cy = n_cy;
+    a.kk1 = we_ny[0];
+    a.kk2 = we_ny[nyc - 1] + 1;
+    a.ii1 = sn_nx[0];
+    a.ii2 = sn_nx[nxc - 1] + 1;
+    if (hipSetDevice(parent->dev) != hipSuccess ||
+        hipMalloc(&n->dmem, sizeof(int) * (nyc + nxc) + sizeof(double) * (nyc + nxc) * 7 + 256) != hipSuccess) {
+        delete n;
+        return fail64(CLIFFS_ERR_NOMEM, "nest device allocation");
+    }
+    char* p = static_cast<char*>(n->dmem);
+    double* uvq = reinterpret_cast<double*>(p);
+    double* wcy = uvq + (size_t)6 * (nyc + nxc);
+    double* scx = wcy + nyc;
+    int* wny = reinterpret_cast<int*>(scx + nxc);
+    int* snx = wny + nyc;
+    unsigned* flag = reinterpret_cast<unsigned*>(snx + nxc);
+    a.uvq = uvq, a.we_cy = wcy, a.sn_cx = scx, a.we_ny = wny, a.sn_nx = snx, a.flag = flag;
+    hipStream_t st = parent->st;
+    if (hipMemcpyAsync(wcy, we_cy, sizeof(double) * nyc, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(scx, sn_cx, sizeof(double) * nxc, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(wny, we_ny, sizeof(int) * nyc, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(snx, sn_nx, sizeof(int) * nxc, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        (void)hipFree(n->dmem);
+        delete n;
+        return fail64(CLIFFS_ERR_HIP, "nest map upload");
+    }
+    parent->users++;
+    *out = n;
+    return CLIFFS_OK;
+}
+
+int cliffs64_nest_destroy(cliffs64_nest* n) {
+    if (!n) return CLIFFS_OK;
+    (void)hipSetDevice(n->parent->dev);
+    (void)hipStreamSynchronize(n->parent->st);
+    (void)hipFree(n->dmem);
+    n->parent->users--;
+    delete n;
+    return CLIFFS_OK;
+}
+
+int cliffs64_nest_feed(cliffs64_nest* n, double cuke, double hmin, int* iarv, int* fed, double* uvq_w,
+                       double* uvq_e, double* uvq_s, double* uvq_n) {
+    if (!n || !iarv || !fed || !uvq_w || !uvq_e || !uvq_s || !uvq_n) return fail64(CLIFFS_ERR_ARG, "NULL argument");
+    cliffs64_handle* h = n->parent;
+    if (!h->state) return fail64(CLIFFS_ERR_STATE, "no state uploaded");
+    HIPCHK64(hipSetDevice(h->dev));
+    Nest64Args a = n->a;
+    a.cel = h->cel, a.xv = h->xvel, a.yv = h->yvel, a.dep = h->dep;
+    a.cuke = cuke, a.hmin = hmin, a.thr = (double)0.1f * cuke;  // feed_children_grid.f:30
+    *fed = 0;
+    if (!*iarv) {
+        HIPCHK64(hipMemsetAsync(a.flag, 0, sizeof(unsigned), h->st));
+        const int w = a.ii2 - a.ii1 + 1, ht = a.kk2 - a.kk1 + 1;
+        hipLaunchKernelGGL(nest64_arrival_kernel, dim3((w + 255) / 256, ht), dim3(256), 0, h->st, a);
+        HIPCHK64(hipGetLastError());
+        unsigned f = 0;
+        HIPCHK64(hipMemcpyAsync(&f, a.flag, sizeof(f), hipMemcpyDeviceToHost, h->st));
+        HIPCHK64(hipStreamSynchronize(h->st));
+        if (!f) return CLIFFS_OK;
+        *iarv = 1;
+    }
+    const int len = std::max(a.nxc, a.nyc);
+    hipLaunchKernelGGL(nest64_feed_kernel, dim3((len + 127) / 128, 4), dim3(128), 0, h->st, a);
+    HIPCHK64(hipGetLastError());
+    const size_t bw = sizeof(double) * 3 * a.nyc, bs = sizeof(double) * 3 * a.nxc;
+    HIPCHK64(hipMemcpyAsync(uvq_w, a.uvq, bw, hipMemcpyDeviceToHost, h->st));
+    HIPCHK64(hipMemcpyAsync(uvq_e, a.uvq + (size_t)3 * a.nyc, bw, hipMemcpyDeviceToHost, h->st));
+    HIPCHK64(hipMemcpyAsync(uvq_s, a.uvq + (size_t)6 * a.nyc, bs, hipMemcpyDeviceToHost, h->st));
+    HIPCHK64(hipMemcpyAsync(uvq_n, a.uvq + (size_t)6 * a.nyc + (size_t)3 * a.nxc, bs, hipMemcpyDeviceToHost, h->st));
+    HIPCHK64(hipStreamSynchronize(h->st));
+    *fed = 1;
     return CLIFFS_OK;
 }
 

@@ -902,11 +902,6 @@ program cliffs_gpu_driver
   celmin = sqrt(GRAV * ground)
 
   ! the grid on the device
-  if (fp64 .and. nests > 0) then
-    write(LOGU, *) 'the FP64 line mode has no nest feed: run the nested case without CLIFFS_PRECISION=fp64'
-    write(*, *) 'the FP64 line mode has no nest feed: run the nested case without CLIFFS_PRECISION=fp64'
-    call finish(1)
-  end if
   if (xrun .and. yrun) then
     rc = cliffs_gpu_open(h, nx, ny, dep, dt, ground, crough, 0, s1, s2, zeta, fp64=fp64)
   else if (xrun) then
@@ -926,8 +921,13 @@ program cliffs_gpu_driver
     allocate(we_ny(nys(ns)), we_cy(nys(ns)), sn_nx(nxs(ns)), sn_cx(nxs(ns)))
     call gpu(cliffs_nest_maps(nx, xcrd, ny, ycrd, nxs(ns), xn(:, ns), nys(ns), yn(:, ns), we_ny, we_cy, &
              sn_nx, sn_cx, w_nx, w_cx, e_nx, e_cx, s_ny, s_cy, n_ny, n_cy), 'cliffs_nest_maps')
-    call gpu(cliffs_nest_create(h, nxs(ns), nys(ns), we_ny, we_cy, sn_nx, sn_cx, w_nx, w_cx, e_nx, e_cx, &
-             s_ny, s_cy, n_ny, n_cy, nest(ns)), 'cliffs_nest_create')
+    if (fp64) then
+      call gpu(cliffs64_nest_create(h, nxs(ns), nys(ns), we_ny, we_cy, sn_nx, sn_cx, w_nx, w_cx, e_nx, e_cx, &
+               s_ny, s_cy, n_ny, n_cy, nest(ns)), 'cliffs64_nest_create')
+    else
+      call gpu(cliffs_nest_create(h, nxs(ns), nys(ns), we_ny, we_cy, sn_nx, sn_cx, w_nx, w_cx, e_nx, e_cx, &
+               s_ny, s_cy, n_ny, n_cy, nest(ns)), 'cliffs_nest_create')
+    end if
     deallocate(we_ny, we_cy, sn_nx, sn_cx)
     call open_feed(trim(casename)//'_'//trim(nestname(ns)), nxs(ns), nys(ns), ffeed(:, ns))
   end do
@@ -1010,7 +1010,11 @@ program cliffs_gpu_driver
     do i = 1, 4
       call close_nc(ffeed(i, ns))
     end do
-    call gpu(cliffs_nest_destroy(nest(ns)), 'cliffs_nest_destroy')
+    if (fp64) then
+      call gpu(cliffs64_nest_destroy(nest(ns)), 'cliffs64_nest_destroy')
+    else
+      call gpu(cliffs_nest_destroy(nest(ns)), 'cliffs_nest_destroy')
+    end if
   end do
   call close_nc(fgau)
   do i = 1, 3
@@ -1118,7 +1122,11 @@ contains
     do ns = 1, nests
       allocate(fw(nys(ns), 3), fe(nys(ns), 3), fs(nxs(ns), 3), fn(nxs(ns), 3))
       ia = iarv(ns)
-      call gpu(cliffs_nest_feed(nest(ns), cuke, ground, ia, fed, fw, fe, fs, fn), 'cliffs_nest_feed')
+      if (fp64) then
+        call gpu(cliffs64_nest_feed(nest(ns), cuke, ground, ia, fed, fw, fe, fs, fn), 'cliffs64_nest_feed')
+      else
+        call gpu(cliffs_nest_feed(nest(ns), cuke, ground, ia, fed, fw, fe, fs, fn), 'cliffs_nest_feed')
+      end if
       if (ia /= 0 .and. iarv(ns) == 0) write(LOGU, *) 'reached child grid ', ns, ' at ', t, 'sec'
       iarv(ns) = ia
       if (fed /= 0) then

@@ -42,7 +42,7 @@ module cliffs_gpu_mod
   ! FP64 line mode (include/cliffs_fp64.h)
   public :: cliffs64_create, cliffs64_destroy, cliffs64_upload_state, cliffs64_download_state
   public :: cliffs64_download_max, cliffs64_sweep, cliffs64_maxwave, cliffs64_run
-  public :: cliffs64_gauges, cliffs64_health
+  public :: cliffs64_gauges, cliffs64_health, cliffs64_nest_create, cliffs64_nest_feed, cliffs64_nest_destroy
 
   interface
     integer(c_int) function cliffs_create(desc, h) bind(C, name="cliffs_create")
@@ -327,6 +327,29 @@ module cliffs_gpu_mod
       type(c_ptr), value :: h
       real(c_double), intent(out) :: courant
       integer(c_long_long), intent(out) :: nonfinite
+    end function
+    integer(c_int) function cliffs64_nest_create(parent, nxc, nyc, we_ny, we_cy, sn_nx, sn_cx, &
+        w_nx, w_cx, e_nx, e_cx, s_ny, s_cy, n_ny, n_cy, nest) bind(C, name="cliffs64_nest_create")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: parent
+      integer(c_int), value :: nxc, nyc, w_nx, e_nx, s_ny, n_ny
+      integer(c_int), intent(in) :: we_ny(*), sn_nx(*)
+      real(c_double), intent(in) :: we_cy(*), sn_cx(*)
+      real(c_double), value :: w_cx, e_cx, s_cy, n_cy
+      type(c_ptr), intent(out) :: nest
+    end function
+    integer(c_int) function cliffs64_nest_feed(nest, cuke, hmin, iarv, fed, uvq_w, uvq_e, uvq_s, uvq_n) &
+        bind(C, name="cliffs64_nest_feed")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: nest
+      real(c_double), value :: cuke, hmin
+      integer(c_int), intent(inout) :: iarv
+      integer(c_int), intent(out) :: fed
+      real(c_double), intent(out) :: uvq_w(*), uvq_e(*), uvq_s(*), uvq_n(*)
+    end function
+    integer(c_int) function cliffs64_nest_destroy(nest) bind(C, name="cliffs64_nest_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: nest
     end function
   end interface
 

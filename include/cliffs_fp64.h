@@ -55,6 +55,23 @@ int cliffs64_gauges(cliffs64_handle *h, int n, const int *I, const int *J, doubl
 /* Health of the current state, as cliffs_health: the largest Courant number
  * (|u| + c) dt / dx over wet cells and the number of non-finite cells. */
 int cliffs64_health(cliffs64_handle *h, double *courant, long long *nonfinite);
+/* One-way nesting from an FP64 parent, as cliffs_nest_create / _feed /
+ * _destroy (the maps from cliffs_nest_maps): feed_children_grid
+ * (feed_children_grid.f:10-179) on the parent's FP64 state, in the
+ * reference's arithmetic -- the arrival test over the child footprint (|eta|
+ * > 0.1*cuke or (|u|+|v|) c > cuke on cells with dep > hmin, latched in
+ * *iarv), then (u, v, eta) blended onto the four child edges, dry parent
+ * cells contributing 0; uvq_w/uvq_e (nyc,3), uvq_s/uvq_n (nxc,3) column-major;
+ * *fed = 1 when the edges were filled.  A nest must be destroyed before its
+ * parent. */
+typedef struct cliffs64_nest cliffs64_nest;
+int cliffs64_nest_create(cliffs64_handle *parent, int nxc, int nyc, const int *we_ny,
+                         const double *we_cy, const int *sn_nx, const double *sn_cx, int w_nx,
+                         double w_cx, int e_nx, double e_cx, int s_ny, double s_cy, int n_ny,
+                         double n_cy, cliffs64_nest **out);
+int cliffs64_nest_feed(cliffs64_nest *n, double cuke, double hmin, int *iarv, int *fed,
+                       double *uvq_w, double *uvq_e, double *uvq_s, double *uvq_n);
+int cliffs64_nest_destroy(cliffs64_nest *n);
 /* nsteps full steps with open edges at rest: x-sweep of rows jj1..jj2,
  * y-sweep of columns ii1..ii2 (cliffs_main.f:140-164, 317-344), maxima. */
 int cliffs64_run(cliffs64_handle *h, int nsteps);

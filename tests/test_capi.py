@@ -117,7 +117,7 @@ def test_fortran_module_binds_every_header_function():
     hdr64 = os.path.join(os.path.dirname(cliffs_gpu.HEADER), "cliffs_fp64.h")
     code = re.sub(r"/\*.*?\*/", "", open(hdr64).read(), flags=re.S)
     want64 = set(re.findall(r"^\s*int\s+(cliffs64_\w+)\s*\(", code, re.M))
-    assert len(want64) == 10 and want64 == set(re.findall(r'bind\(C, name="(cliffs64_\w+)"\)', txt))
+    assert len(want64) == 13 and want64 == set(re.findall(r'bind\(C, name="(cliffs64_\w+)"\)', txt))
 
 
 def test_surrogate_header_exported_and_plain_c():

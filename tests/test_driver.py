@@ -196,6 +196,92 @@ def _params(path, grid, nest, cuke, ground, dt, steps, freeze_flag, seaout, sub,
         f.write("\n".join(lines) + "\n")
 
 
+def test_nest_parent_fp64_line_mode_matches_oracle(tmp_path, gpu_lib):
+    """CLIFFS_PRECISION=fp64 on a parent grid with a nest: screenshots,
+    gauges, max-wave file and the four feed files equal, bit for bit, the
+    FP64 oracle stepped the same way with feed_children_grid every bndout
+    steps (the feed from the FP64 parent state, cliffs64_nest_feed)."""
+    d = str(tmp_path) + "/"
+    case = synth.coastal(nx=200, ny=160, seed=31)
+    x, y, dep = case.xcrd, case.ycrd, case.dep
+    xc = np.linspace(x[100], x[185], 3 * 85 + 1)
+    yc = np.linspace(y[30], y[120], 3 * 90 + 1)
+    interp = RegularGridInterpolator((y, x), dep)
+    Yc, Xc = np.meshgrid(yc, xc, indexing="ij")
+    depc = interp(np.stack([Yc, Xc], axis=-1))
+    eta = np.where(dep > 0, case.cel ** 2 / G - dep, 0.0).astype(np.float32).astype(np.float64)
+    _write_grid(d + "parent.nc", x, y, dep)
+    _write_grid(d + "child.nc", xc, yc, depc)
+    _write_field(d + "src_h.nc", "ha", x, y, eta)
+    steps, bndout, seaout, gout, maxout, sub = 80, 2, 20, 5, 25, 2
+    gauges = [(150, 80), (120, 40)]
+    cuke, ground, dt = 0.01, case.ground, case.dt
+    _params(d + "pp.txt", "parent.nc", "child.nc", cuke, ground, dt, steps, 1, seaout, sub, bndout,
+            maxout, gauges)
+    log = _drive([d + "par", d, "0", "src", d + "pp.txt"], d + "par_log.txt", env={"CLIFFS_PRECISION": "fp64"})
+    assert "FP64 line mode" in log
+
+    s1, s2, zeta = synth.metrics(x, y, True)
+    o = O.Oracle(dep, s1, s2, zeta, dt=dt, ground=ground)
+    z = np.zeros_like(dep)
+    o.set_state(synth.depth_to_celerity(dep + eta, ground), z, z)
+    maps = O.nest_maps(x, y, xc, yc)
+    celmin = np.sqrt(G * ground)
+    shots, gts, feeds = [], [], []
+    I = np.array([i for i, _ in gauges])
+    J = np.array([j for _, j in gauges])
+    arrived = False
+    t = 0.0
+
+    def shot():
+        c, u, v = o.get_state()[:3]
+        c, u, v = c[::sub, ::sub], u[::sub, ::sub], v[::sub, ::sub]
+        wet = c > celmin
+        sh = [np.where(wet, c * c / G - dep[::sub, ::sub], 0).astype(np.float32), u.astype(np.float32),
+              v.astype(np.float32)]
+        shots.append((t, [np.where(wet, a, NAN4) for a in sh]))
+
+    def gauge():
+        c, u, v = o.get_state()[:3]
+        cg = c[J - 1, I - 1]
+        gts.append((t, (np.where(cg > celmin, cg * cg / G - dep[J - 1, I - 1], -1e32), u[J - 1, I - 1],
+                        v[J - 1, I - 1])))
+
+    def feed():
+        nonlocal arrived
+        arrived, w, e, s_, n_ = O.feed_child(o, maps, cuke, ground, arrived)
+        if arrived:
+            feeds.append((t, w, e, s_, n_))
+
+    shot()
+    gauge()
+    feed()
+    for k in range(1, steps + 1):
+        t = t + dt
+        o.steps(1)
+        if k % bndout == 0:
+            feed()
+        if k % seaout == 0:
+            shot()
+        if k % gout == 0:
+            gauge()
+    hm, vm = o.get_state()[3:5]
+    assert len(feeds) >= 10
+    for q, name in enumerate(("h", "u", "v")):
+        path = d + f"par_sea_{name}.nc"
+        _same(_var(path, "time"), np.array([s[0] for s in shots]), f"sea_{name} time")
+        _same(_var(path, name + "a"), np.array([s[1][q] for s in shots]), f"sea_{name}")
+    for q, name in enumerate(("gage", "u", "v")):
+        _same(_var(d + "par_gages.nc", name), np.array([s[1][q] for s in gts]), f"gauge {name}")
+    wet = hm > celmin
+    _same(_var(d + "par_maxwave.nc", "MaxE"), np.where(wet, hm * hm / G - dep, NAN4).astype(np.float32), "MaxE")
+    _same(_var(d + "par_maxwave.nc", "MaxV"), np.where(wet, np.sqrt(vm), NAN4).astype(np.float32), "MaxV")
+    for q, side in enumerate(("west", "east", "south", "north")):
+        path = d + f"par_child_{side}.nc"
+        _same(_var(path, "time"), np.array([f[0] for f in feeds]), f"feed {side} time")
+        _same(_var(path, "vals"), np.array([f[1 + q].T for f in feeds]), f"feed {side}")
+
+
 def test_nest_cascade_bit_identical_to_c_abi(tmp_path, gpu_lib):
     d = str(tmp_path) + "/"
     case = synth.coastal(nx=200, ny=160, seed=31)

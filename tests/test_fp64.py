@@ -38,7 +38,7 @@ def test_fp64_header_plain_c_and_exported():
     subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-x", "c", "-", "-o", os.devnull],
                    input=src, text=True, check=True)
     names = _symbols()
-    assert len(names) == 10, names
+    assert len(names) == 13, names
     lib = ctypes.CDLL(cliffs_gpu.LIB_PATH)
     for n in names:
         assert hasattr(lib, n), n
@@ -189,6 +189,42 @@ def test_fp64_single_sweeps_random_wetdry(gpu_lib, seed):
         for a, b, name in zip(g.download(), orc.get_state()[:3], NAMES):
             bad = int((a.view(np.int64) != b.view(np.int64)).sum())
             assert bad == 0, f"sweep {iu}, {name}: {bad} cells differ"
+    g.close()
+
+
+@pytest.mark.gpu
+def test_fp64_nest_feed_matches_oracle(gpu_lib):
+    """feed_children_grid on the FP64 parent: the arrival latch and the four
+    edges bit-identical to the oracle's, before the wave reaches the child
+    footprint and every few steps after it; a grid with a live nest cannot
+    be destroyed."""
+    case = synth.coastal(nx=200, ny=160, seed=31)
+    orc, g = _pair(gpu_lib, case)
+    x, y = case.xcrd, case.ycrd
+    xc = np.linspace(x[100], x[185], 3 * 85 + 1)
+    yc = np.linspace(y[30], y[120], 3 * 90 + 1)
+    maps = gpu_lib.nest_maps(x, y, xc, yc)
+    nest = gpu_lib.Nest64(g, maps)
+    assert gpu_lib.load_library().cliffs64_destroy(g._h) == -6     # CLIFFS_ERR_STATE
+    # no arrival while no cell reaches the threshold: the latch stays open
+    ao, *_ = O.feed_child(orc, maps, 1e3, case.ground, False)
+    ag, fg, *_ = nest.feed(1e3, case.ground, False)
+    assert not ao and not ag and not fg
+    arrived_o = arrived_g = False
+    fed = 0
+    for k in range(40):
+        orc.steps(3)
+        g.run(3)
+        ao, *eo = O.feed_child(orc, maps, 0.01, case.ground, arrived_o)
+        ag, fg, *eg = nest.feed(0.01, case.ground, arrived_g)
+        assert ao == ag and fg == ag
+        arrived_o, arrived_g = ao, ag
+        if ag:
+            fed += 1
+            for a, b, side in zip(eg, eo, ("west", "east", "south", "north")):
+                assert np.array_equal(a.view(np.int64), b.view(np.int64)), f"step {3 * (k + 1)}, {side}"
+    assert fed >= 5
+    nest.close()
     g.close()
 
 
