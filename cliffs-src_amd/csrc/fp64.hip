@@ -53,6 +53,7 @@ enum : int {
 
 struct Line64 {
     int nx, iu, first, nlines, n;
+    int ld;  // node i of line l at ld*(l-1) + (i-1): nx for rows, ny for the transposed columns
     const double* dep;
     double* cel;
     double* xvel;
@@ -97,9 +98,7 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
     const double G = kG, celmin = a.celmin, ground = a.ground, dt = a.dt;
     const size_t np = (size_t)n + 2;
     const Work w{a.wd + (size_t)t * kDoubles * np, a.wi + (size_t)t * kInts * np, np};
-    auto at = [&](int i) {
-        return iu == 1 ? (size_t)nx * (irw - 1) + (i - 1) : (size_t)(irw - 1) + (size_t)nx * (i - 1);
-    };
+    auto at = [&](int i) { return (size_t)a.ld * (irw - 1) + (i - 1); };
 
     // gather (cliffs.f:36-54): the line's depth, c, normal and tangential
     // velocity, metric term and spacing; land flags (cliffs.f:58-63)
@@ -405,6 +404,25 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
     }
 }
 
+// out (cols x rows) = in (rows x cols) transposed, through 32 x 32 LDS tiles
+// (the y-sweep runs on the column-major copy, so its lanes read along
+// contiguous lines as the x-sweep's do)
+__global__ __launch_bounds__(256) void transpose64_kernel(const double* __restrict__ in, double* __restrict__ out,
+                                                          int rows, int cols) {
+    __shared__ double tile[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8 threads
+    for (int k = ty; k < 32; k += 8) {
+        const int r = r0 + k, c = c0 + tx;
+        if (r < rows && c < cols) tile[k][tx] = in[(size_t)r * cols + c];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int c = c0 + k, r = r0 + tx;
+        if (r < rows && c < cols) out[(size_t)c * rows + r] = tile[tx][k];
+    }
+}
+
 // running maxima (cliffs_main.f:347-355)
 __global__ void maxwave64_kernel(const double* __restrict__ cel, const double* __restrict__ xv,
                                  const double* __restrict__ yv, double* __restrict__ hm,
@@ -571,6 +589,7 @@ struct cliffs64_handle {
     bool state = false;
     double *dep = nullptr, *cel = nullptr, *xvel = nullptr, *yvel = nullptr;
     double *hmax = nullptr, *velmax = nullptr;
+    double *celT = nullptr, *xvelT = nullptr, *yvelT = nullptr, *depT = nullptr;  // column-major copies
     double *s1 = nullptr, *s2 = nullptr, *zeta = nullptr;
     double* edges = nullptr;  // edge1 (mx, 3) then edge2 (mx, 3)
     unsigned long long* health = nullptr;  // cliffs64_health result: Courant bits, non-finite count
@@ -595,7 +614,7 @@ void free64(cliffs64_handle* h) {
     (void)hipSetDevice(h->dev);
     if (h->st) (void)hipStreamSynchronize(h->st);
     for (double* p : {h->dep, h->cel, h->xvel, h->yvel, h->hmax, h->velmax, h->s1, h->s2, h->zeta, h->edges,
-                      h->wd})
+                      h->wd, h->celT, h->xvelT, h->yvelT, h->depT})
         if (p) (void)hipFree(p);
     if (h->wi) (void)hipFree(h->wi);
     if (h->health) (void)hipFree(h->health);
@@ -604,18 +623,30 @@ void free64(cliffs64_handle* h) {
     delete h;
 }
 
+hipError_t transpose64(const double* in, double* out, int rows, int cols, hipStream_t st) {
+    hipLaunchKernelGGL(transpose64_kernel, dim3((cols + 31) / 32, (rows + 31) / 32), dim3(256), 0, st, in, out, rows,
+                       cols);
+    return hipGetLastError();
+}
+
 int sweep64(cliffs64_handle* h, int iu, int first, int last, const double* e1, const double* e2, int mx) {
     const int n = iu == 1 ? h->nx : h->ny;
+    const int nx = h->nx, ny = h->ny;
+    if (iu == 2)  // columns as contiguous lines
+        for (int f = 0; f < 3; f++)
+            HIPCHK64(transpose64(f == 0 ? h->cel : f == 1 ? h->xvel : h->yvel,
+                                 f == 0 ? h->celT : f == 1 ? h->xvelT : h->yvelT, ny, nx, h->st));
     Line64 a;
-    a.nx = h->nx;
+    a.nx = nx;
     a.iu = iu;
     a.first = first;
     a.nlines = last - first + 1;
     a.n = n;
-    a.dep = h->dep;
-    a.cel = h->cel;
-    a.xvel = h->xvel;
-    a.yvel = h->yvel;
+    a.ld = iu == 1 ? nx : ny;
+    a.dep = iu == 1 ? h->dep : h->depT;
+    a.cel = iu == 1 ? h->cel : h->celT;
+    a.xvel = iu == 1 ? h->xvel : h->xvelT;
+    a.yvel = iu == 1 ? h->yvel : h->yvelT;
     a.s1 = h->s1;
     a.s2 = h->s2;
     a.zeta = h->zeta;
@@ -631,6 +662,10 @@ int sweep64(cliffs64_handle* h, int iu, int first, int last, const double* e1, c
     hipLaunchKernelGGL(line64_kernel, dim3((a.nlines + kLineWaves - 1) / kLineWaves), dim3(64 * kLineWaves), 0,
                        h->st, a);
     HIPCHK64(hipGetLastError());
+    if (iu == 2)
+        for (int f = 0; f < 3; f++)
+            HIPCHK64(transpose64(f == 0 ? h->celT : f == 1 ? h->xvelT : h->yvelT,
+                                 f == 0 ? h->cel : f == 1 ? h->xvel : h->yvel, nx, ny, h->st));
     return CLIFFS_OK;
 }
 
@@ -695,7 +730,9 @@ int cliffs64_create(const cliffs_grid_desc* d, cliffs64_handle** out) {
               hipMalloc(&h->edges, (size_t)6 * h->mx * 8) == hipSuccess &&
               hipMalloc(&h->wd, (size_t)kDoubles * wn * 8) == hipSuccess &&
               hipMalloc(&h->wi, (size_t)kInts * wn * 4) == hipSuccess &&
-              hipMalloc(&h->health, 2 * sizeof(unsigned long long)) == hipSuccess;
+              hipMalloc(&h->health, 2 * sizeof(unsigned long long)) == hipSuccess &&
+              hipMalloc(&h->celT, m * 8) == hipSuccess && hipMalloc(&h->xvelT, m * 8) == hipSuccess &&
+              hipMalloc(&h->yvelT, m * 8) == hipSuccess && hipMalloc(&h->depT, m * 8) == hipSuccess;
     if (!ok) return bail(fail64(CLIFFS_ERR_NOMEM, "device allocation"));
     ok = hipMemcpyAsync(h->dep, d->dep, m * 8, hipMemcpyHostToDevice, h->st) == hipSuccess &&
          (nx <= 2 || hipMemcpyAsync(h->s1, d->s1, n_s1 * 8, hipMemcpyHostToDevice, h->st) == hipSuccess) &&
@@ -707,6 +744,7 @@ int cliffs64_create(const cliffs_grid_desc* d, cliffs64_handle** out) {
          hipMemsetAsync(h->yvel, 0, m * 8, h->st) == hipSuccess &&
          hipMemsetAsync(h->hmax, 0, m * 8, h->st) == hipSuccess &&
          hipMemsetAsync(h->velmax, 0, m * 8, h->st) == hipSuccess &&
+         transpose64(h->dep, h->depT, ny, nx, h->st) == hipSuccess &&
          hipStreamSynchronize(h->st) == hipSuccess;
     if (!ok) return bail(fail64(CLIFFS_ERR_HIP, "grid upload"));
     *out = h;
