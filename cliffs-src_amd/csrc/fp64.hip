@@ -91,7 +91,9 @@ constexpr int kLineWaves = 4;  // lines per block
 // One wave per line, one node (or segment) per lane in every pass.
 __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a) {
     const int lane = threadIdx.x & 63;
-    const int t = blockIdx.x * kLineWaves + (threadIdx.x >> 6);  // line of this wave
+    // line of this wave (made scalar: the work arrays' bases then sit in
+    // scalar registers and every access is a 32-bit offset from them)
+    const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * kLineWaves + (threadIdx.x >> 6));
     if (t >= a.nlines) return;                                   // wave-uniform
     const int irw = a.first + t;                                 // 1-based line
     const int n = a.n, nx = a.nx, iu = a.iu;
