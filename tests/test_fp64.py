@@ -140,6 +140,33 @@ def test_fp64_beach_1d(gpu_lib):
 
 
 @pytest.mark.gpu
+def test_fp64_beach_1d_along_y(gpu_lib):
+    """The 1-D beach turned along y (one column: the y-sweep only, through
+    the column-major copy of a one-column grid)."""
+    c = synth.beach_1d(seed=4, n=900)
+    case = synth.Case("beach_1d_T", c.dep.T.copy(), c.cel.T.copy(), c.yvel.T.copy(), c.xvel.T.copy(),
+                      c.ycrd, c.xcrd, c.cartesian, c.dt, c.ground, c.crough, c.steps, dict(c.meta))
+    orc, g = _pair(gpu_lib, case)
+    orc.steps(600)
+    g.run(600)
+    _same(orc, g)
+    assert _flooded(g, orc, case) > 0
+    g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape", [(3, 60), (60, 3), (130, 4)])
+def test_fp64_thin_grids(gpu_lib, shape):
+    nx, ny = shape
+    case = synth.coastal(nx=nx, ny=ny, seed=11)
+    orc, g = _pair(gpu_lib, case)
+    orc.steps(60)
+    g.run(60)
+    _same(orc, g)
+    g.close()
+
+
+@pytest.mark.gpu
 def test_fp64_forced_edges_and_line_ranges(gpu_lib):
     """Sweeps of chosen line ranges with forced edge values (elevation,
     normal and tangential velocity at both ends of each line)."""
