@@ -270,13 +270,13 @@ def test_fp64_rejects_bad_arguments(gpu_lib):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("key", ["131f"])
+@pytest.mark.parametrize("key", ["131f", "0f", "1f", "3f", "5f"])
 def test_fp64_full_size_scenario(gpu_lib, key):
-    """Ensemble scenario 131 at full size (1331 x 916, 4000 steps),
+    """Ensemble scenarios at full size (767-1610 cells a side, 4000 steps),
     frictionless with a 1 cm minimal flow depth: there the FP32 state's
-    trajectory leaves FP64's after ~1,250 steps and ends with another
-    run-up; the FP64 line mode gives the FP64 oracle's run-up and flooded
-    area exactly (tests/golden/ensemble_runup.json)."""
+    trajectory leaves FP64's (on scenario 131 after ~1,250 steps) and ends
+    with another run-up; the FP64 line mode gives the FP64 oracle's run-up
+    and flooded area exactly (tests/golden/ensemble_runup.json)."""
     fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ensemble_runup.json")))[key]
     case = synth.ensemble_scenario(int(key.rstrip("f")), ground=fx["ground"], crough=fx["crough"])
     assert list(case.dep.shape) == fx["shape"]
