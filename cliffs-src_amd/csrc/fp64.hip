@@ -40,7 +40,7 @@ constexpr double kSqr2 = (double)1.41421356f;  // cliffs.f:34
 
 // work arrays of a line (1-based, n + 2 slots each)
 enum : int {
-    kH, kU, kV, kVV, kP, kQ, kPP, kQQ, kDX, kSCL, kDEP,
+    kH, kU, kV, kVV, kP, kQ, kDX, kSCL, kDEP,
     kPL1, kPL2, kPL3, kPL4, kPR1, kPR2, kPR3, kPR4,  // ghost (P, Q, v, d) per segment, left and right
     kDoubles
 };
@@ -265,13 +265,11 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
         }
     }
     pass_done();
-    for (int i = 1 + lane; i <= n; i += 64) {
-        w(kQQ, i) = w(kQ, i);
-        w(kPP, i) = w(kP, i);
-    }
-    pass_done();
 
     // VTCS-2 integration of the segments' interior nodes (cliffs.f:168-253)
+    // and c, u from the new invariants (cliffs.f:255-262), node by node: a
+    // node reads its neighbours' P, Q, v and depth only, which no node of
+    // this pass changes, and its own c and u before it replaces them
     for (int k = 1; k <= kseg; k++) {
         const int i1 = w.ix(kLG, k) + 1, i2 = w.ix(kRG, k) - 1;
         for (int i = i1 + lane; i <= i2; i += 64) {
@@ -308,7 +306,7 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
             else
                 Qsum = (Qpj + Qjm) / 2;
             Qsum = Qsum + 0.25 * ej * dtx * (Qjm - Qpj);
-            w(kPP, i) = pj - dt * (Qsum + cc - sphr);
+            const double pp = pj - dt * (Qsum + cc - sphr);
 
             // Q family
             ej = 3 * qj + pj, e2 = 3 * q2 + p2, e1 = 3 * q1 + p1;
@@ -319,19 +317,11 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
             else
                 Qsum = (Qpj + Qjm) / 2;
             Qsum = Qsum + 0.25 * ej * dtx * (Qjm - Qpj);
-            w(kQQ, i) = qj - dt * (Qsum + cc + sphr);
+            const double qq = qj - dt * (Qsum + cc + sphr);
 
             // tangential velocity (cliffs.f:249-250)
             Qsum = (v2 - vj) / fp - (vj - v1) / fm;
             w(kVV, i) = vj - uj * dtx * (v2 - v1 - Qsum * (uj * dt + fp - fm));
-        }
-    }
-    pass_done();
-    // c and u from the new invariants (cliffs.f:255-262)
-    for (int k = 1; k <= kseg; k++) {
-        const int i1 = w.ix(kLG, k) + 1, i2 = w.ix(kRG, k) - 1;
-        for (int i = i1 + lane; i <= i2; i += 64) {
-            const double pp = w(kPP, i), qq = w(kQQ, i);
             w(kH, i) = (pp - qq) / 4;
             w(kU, i) = (pp + qq) / 2;
         }
