@@ -49,6 +49,7 @@ enum : int {
     kLand = 1, kNewland = 2,                         // as loaded; still dry after the expansions
     kR = 4, kW = 8, kS = 16, kL = 32, kM = 64, kE = 128,  // decisions of the segment scan
     kEnd = 256,                                      // a segment ends here (not merged away)
+    kIn = 512,                                       // inside a segment: lghost < i < rghost
 };
 
 struct Line64 {
@@ -182,17 +183,27 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
         w.ix(kFL, i) = fl;
     }
     pass_done();
-    // segment numbers: starts not merged, counted along the line 64 nodes at a time
-    int kseg = 0;
+    // segment numbers: starts not merged, counted along the line 64 nodes at
+    // a time; a node is inside a segment (lghost < i < rghost) when more
+    // segments have started than ended up to it, or when the start after it
+    // floods it (left expansion), never at the line's ends
+    int kseg = 0, kend = 0;
     for (int c = 0; c < n; c += 64) {
         const int i = c + 1 + lane;
         const int fl = i <= n ? w.ix(kFL, i) : 0;
+        const int fp1 = i + 1 <= n ? w.ix(kFL, i + 1) : 0;
         const bool st = (fl & kS) && !(fl & kM);
-        const unsigned long long ms = __ballot(st);
+        const bool en = (fl & kEnd) != 0;
+        const unsigned long long ms = __ballot(st), me = __ballot(en);
+        const unsigned long long upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);  // lanes <= this one
         const int below = __popcll(ms & ((1ull << lane) - 1ull));
         if (st) w.ix(kLG, kseg + below + 1) = i == 1 ? 1 : ((fl & kL) ? (i - 2 > 1 ? i - 2 : 1) : i - 1);
-        if (fl & kEnd) w.ix(kRG, kseg + below + (st ? 1 : 0)) = i;
+        if (en) w.ix(kRG, kseg + below + (st ? 1 : 0)) = i;
+        const bool flooded_by_next = (fp1 & kS) && !(fp1 & kM) && (fp1 & kL);
+        const bool inside = kseg + __popcll(ms & upto) > kend + __popcll(me & upto) || flooded_by_next;
+        if (i >= 2 && i <= n - 1 && inside) w.ix(kFL, i) = fl | kIn;
         kseg += __popcll(ms);
+        kend += __popcll(me);
     }
     if (kseg == 0) return;  // cliffs.f:113: the line is left untouched
     pass_done();
@@ -205,26 +216,21 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
     const int k2 = (w.ix(kFL, rlast) & kNewland) ? kseg : kseg - 1;
     pass_done();
 
-    // nodes still dry are emptied (cliffs.f:114-124); invariants cleared
+    // nodes still dry are emptied (cliffs.f:114-124); Riemann invariants
+    // inside the segments, 0 elsewhere (cliffs.f:125-136)
     for (int i = 1 + lane; i <= n; i += 64) {
-        if (newland_of(w, i)) {
+        const int fl = w.ix(kFL, i);
+        double h = w(kH, i), u = w(kU, i), v = w(kV, i);
+        if (fl & kNewland) {
+            h = u = v = 0;
             w(kH, i) = 0;
             w(kU, i) = 0;
             w(kV, i) = 0;
         }
-        w(kVV, i) = w(kV, i);
-        w(kP, i) = 0;
-        w(kQ, i) = 0;
-    }
-    pass_done();
-    // Riemann invariants inside the segments (cliffs.f:125-136)
-    for (int k = 1; k <= kseg; k++) {
-        const int i1 = w.ix(kLG, k) + 1, i2 = w.ix(kRG, k) - 1;
-        for (int i = i1 + lane; i <= i2; i += 64) {
-            const double u = w(kU, i), h = w(kH, i);
-            w(kP, i) = u + 2 * h;
-            w(kQ, i) = u - 2 * h;
-        }
+        w(kVV, i) = v;
+        const bool in = (fl & kIn) != 0;
+        w(kP, i) = in ? u + 2 * h : 0.0;
+        w(kQ, i) = in ? u - 2 * h : 0.0;
     }
     pass_done();
     // reflective ghosts at shorelines (cliffs.f:137-152), one segment per lane
