@@ -2,13 +2,16 @@
  * c, xvel, yvel) carried in double precision on the device, each line swept
  * by the reference's own line procedure (cliffs.f:17-346: segment scan with
  * run-up expansion, Riemann invariants, ghosts, VTCS-2 integration,
- * limiter, edges) in the same operation order, one GPU thread per line.
- * Its results equal the reference's FP64 arithmetic bit for bit, for flow
- * that amplifies rounding (frictionless thin films on steep coasts, grids at
- * the scheme's stability limit) where the FP32 perturbation state of
- * cliffs_gpu.h and FP64 part.  It is a fidelity mode: it keeps the
- * reference's sequential scan per line, so a step costs far more than the
- * FP32 fused step.
+ * limiter, edges) in the same per-node operation order, one wave of 64
+ * lanes per line.  The segment scan, which the reference runs node after
+ * node, is restated as per-node rules over a node's neighbours and a
+ * prefix count across the wave, with the same outcome.  Its results equal
+ * the reference's FP64 arithmetic bit for bit, for flow that amplifies
+ * rounding (frictionless thin films on steep coasts, grids at the scheme's
+ * stability limit) where the FP32 perturbation state of cliffs_gpu.h and
+ * FP64 part.  It is a fidelity mode: a line is walked in 64-node chunks
+ * through per-line work arrays, so a step costs far more than the FP32
+ * fused step.
  *
  * Plain C ABI: pointers and sizes, column-major (Fortran) arrays as in
  * cliffs_gpu.h.  Return codes are those of cliffs_gpu.h (CLIFFS_OK, ...);
