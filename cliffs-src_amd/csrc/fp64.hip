@@ -2,12 +2,11 @@
 // (cliffs.f:17-346) in double precision on the device, one wave per line.
 //
 // HBM layout.  The state (c, xvel, yvel), the maxima and the depth are
-// (nx, ny) arrays of doubles with x fastest, the reference's own order.
-// Each wave of a sweep owns one line; the line's work arrays (the
-// reference's per-line scratch: h, u, v, the invariants, their updates, the
-// ghost values per segment, the segment bounds and the land flags) are
-// contiguous in one HBM buffer, so the wave's lanes, one node each, touch
-// consecutive addresses.
+// (nx, ny) arrays of doubles with x fastest, the reference's own order; a
+// y-sweep works on column-major copies, so every line is contiguous.  Each
+// wave of a sweep owns one line.  The line's per-node flags and segment
+// bounds are contiguous in one int buffer, so the wave's lanes, one node
+// each, touch consecutive addresses; the sweep updates the state in place.
 //
 // Arithmetic.  The reference's expressions in the reference's order, in
 // double precision, compiled with FP contraction off (no fused
@@ -17,11 +16,10 @@
 // Parallelism.  Every pass of a line runs on its wave's 64 lanes: the
 // segment scan (cliffs.f:65-111), sequential in the reference, as local
 // rules per node and a prefix count of the segment starts (see the kernel),
-// the other passes (gather, invariants, ghosts, integration,
-// reconstruction, limiter, scatter) per node or per segment.  Lines are
+// the integration per node, the limiter per segment.  Lines are
 // independent, so a sweep runs them all at once.  It is a fidelity mode for
-// flow that FP32 state cannot follow: the FP64 traffic and the passes over
-// the work arrays make it much slower than the FP32 fused step.
+// flow that FP32 state cannot follow: its FP64 traffic makes it several
+// times slower than the FP32 fused step.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,19 +36,19 @@ namespace {
 constexpr double kG = (double)9.8f;            // g (global_modules.f:17, an FP32 literal)
 constexpr double kSqr2 = (double)1.41421356f;  // cliffs.f:34
 
-// work arrays of a line (1-based, n + 2 slots each)
-enum : int {
-    kH, kU, kV, kVV, kP, kQ, kDX, kSCL, kDEP,
-    kPL1, kPL2, kPL3, kPL4, kPR1, kPR2, kPR3, kPR4,  // ghost (P, Q, v, d) per segment, left and right
-    kDoubles
-};
+// Per-line work arrays: the per-node flags and the segments' bounds (ints,
+// n + 2 slots each), and twelve doubles per line that keep the state of the
+// line's first two and last two nodes after the run-up expansions, which
+// the edge integration reads once the sweep has overwritten them.
 enum : int { kLG, kRG, kFL, kInts };  // segment bounds (by segment); per-node flag bits below
 enum : int {
     kLand = 1, kNewland = 2,                         // as loaded; still dry after the expansions
     kR = 4, kW = 8, kS = 16, kL = 32, kM = 64, kE = 128,  // decisions of the segment scan
     kEnd = 256,                                      // a segment ends here (not merged away)
     kIn = 512,                                       // inside a segment: lghost < i < rghost
+    kFromR = 1024,                                   // flooded by the start at i+1 (its left expansion)
 };
+constexpr int kStash = 16;  // doubles per line: (h, u, v) of nodes 1, 2, n-1, n
 
 struct Line64 {
     int nx, iu, first, nlines, n;
@@ -66,62 +64,63 @@ struct Line64 {
     const double* e2;
     int mx;
     double dt, ground, crough, celmin;
-    double* wd;  // nlines * kDoubles * (n + 2)
+    double* wd;  // nlines * kStash
     int* wi;     // nlines * kInts * (n + 2)
 };
 
-// Work arrays of one line: array k, element i at (line * kDoubles + k) * np + i
+// Int work arrays of one line: array k, element i at (line * kInts + k) * np + i
 // (np = n + 2), so the lanes of the line's wave touch consecutive addresses.
 struct Work {
-    double* d;
     int* k;
     size_t np;
-    __device__ double& operator()(int a, int i) const { return d[(size_t)a * np + i]; }
     __device__ int& ix(int a, int i) const { return k[(size_t)a * np + i]; }
 };
 
-__device__ __forceinline__ bool land_of(const Work& w, int i) { return (w.ix(kFL, i) & kLand) != 0; }
-__device__ __forceinline__ bool newland_of(const Work& w, int i) { return (w.ix(kFL, i) & kNewland) != 0; }
-
-// The lanes of a wave read what other lanes of it wrote to the work arrays
-// in the pass before (all of one work-group: one L1).
+// The lanes of a wave read what other lanes of it wrote in the pass before
+// (all of one work-group: one L1).
 __device__ __forceinline__ void pass_done() { __threadfence_block(); }
+
+// x of lane l, in every lane
+__device__ __forceinline__ double lane_value(double x, int l) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, l);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+struct Node { double h, u, v; };  // celerity, normal and tangential velocity
 
 constexpr int kLineWaves = 4;  // lines per block
 
-// One wave per line, one node (or segment) per lane in every pass.
+// One wave per line, one node (or segment) per lane in every pass.  Two
+// passes over the line's flags decide the segment scan; one pass over the
+// state then integrates the nodes in place, 64 at a time, each from its own
+// and its neighbours' state after the run-up expansions; the limiter and the
+// edges close the sweep.
 __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a) {
     const int lane = threadIdx.x & 63;
-    // line of this wave (made scalar: the work arrays' bases then sit in
-    // scalar registers and every access is a 32-bit offset from them)
+    // line of this wave (made scalar: the line's bases then sit in scalar
+    // registers and every access is a 32-bit offset from them)
     const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * kLineWaves + (threadIdx.x >> 6));
     if (t >= a.nlines) return;                                   // wave-uniform
     const int irw = a.first + t;                                 // 1-based line
     const int n = a.n, nx = a.nx, iu = a.iu;
     const double G = kG, celmin = a.celmin, ground = a.ground, dt = a.dt;
     const size_t np = (size_t)n + 2;
-    const Work w{a.wd + (size_t)t * kDoubles * np, a.wi + (size_t)t * kInts * np, np};
+    const Work w{a.wi + (size_t)t * kInts * np, np};
+    double* const sd = a.wd + (size_t)t * kStash;
+    // the line (cliffs.f:36-54): c, normal velocity (xvel in x, yvel in y),
+    // tangential velocity, depth, spacing dx(i) between nodes i and i+1, and
+    // the metric term (zeta in y, 0 in x)
+    double* const cel = a.cel;
+    double* const nv = iu == 1 ? a.xvel : a.yvel;
+    double* const tv = iu == 1 ? a.yvel : a.xvel;
+    const double* const dep = a.dep;
     auto at = [&](int i) { return (size_t)a.ld * (irw - 1) + (i - 1); };
+    auto dx = [&](int i) { return iu == 1 ? a.s1[(size_t)(nx - 1) * (irw - 1) + (i - 1)] : a.s2[i - 1]; };
+    auto scl = [&](int i) { return iu == 1 ? 0.0 : a.zeta[i - 1]; };
 
-    // gather (cliffs.f:36-54): the line's depth, c, normal and tangential
-    // velocity, metric term and spacing; land flags (cliffs.f:58-63)
-    for (int i = 1 + lane; i <= n; i += 64) {
-        const size_t o = at(i);
-        const double h = a.cel[o];
-        w(kDEP, i) = a.dep[o];
-        w(kH, i) = h;
-        w(kU, i) = iu == 1 ? a.xvel[o] : a.yvel[o];
-        w(kV, i) = iu == 1 ? a.yvel[o] : a.xvel[o];
-        w(kSCL, i) = iu == 1 ? 0.0 : a.zeta[i - 1];
-        w.ix(kLG, i) = 0;
-        w.ix(kRG, i) = 0;
-        const int land = h <= celmin ? 1 : 0;
-        w.ix(kFL, i) = land | (land << 1);
-        if (i <= n - 1) w(kDX, i) = iu == 1 ? a.s1[(size_t)(nx - 1) * (irw - 1) + (i - 1)] : a.s2[i - 1];
-    }
-    pass_done();
-
-    // Segment scan with run-up expansion (cliffs.f:65-111).  The reference
+    // Segment scan with run-up expansion (cliffs.f:58-111).  The reference
     // walks the line once; every decision of that walk depends only on a
     // node and its neighbours' original values, because a node that floods
     // (run-up expansion) is land and its wet neighbour is never changed:
@@ -133,204 +132,189 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
     //         S, L and W(i-2), i.e. a single land node between the two
     //   E(i)  the segment ends at land node i: W(i-1) and not R(i)
     // so the lanes decide them node by node, and a prefix count of the
-    // starts numbers the segments.
-    for (int i = 1 + lane; i <= n; i += 64) {  // R, W
-        int fl = w.ix(kFL, i);
+    // starts numbers the segments.  This pass: land, R, W.
+    for (int i = 1 + lane; i <= n; i += 64) {
+        const bool land = cel[at(i)] <= celmin;
         bool r = false;
-        if (i >= 2 && (fl & kLand) && !(w.ix(kFL, i - 1) & kLand)) {
-            const double hp = w(kH, i - 1);
-            const double flood = (hp * hp) / G - w(kDEP, i - 1) + w(kDEP, i);
-            r = (flood > ground) && (w(kU, i - 1) >= 0);
+        if (i >= 2 && land) {
+            const double hp = cel[at(i - 1)];
+            if (!(hp <= celmin)) {
+                const double flood = (hp * hp) / G - dep[at(i - 1)] + dep[at(i)];
+                r = (flood > ground) && (nv[at(i - 1)] >= 0);
+            }
         }
-        fl |= (r ? kR : 0) | ((!(fl & kLand) || r) ? kW : 0);
-        w.ix(kFL, i) = fl;
+        w.ix(kFL, i) = (land ? kLand : 0) | (r ? kR : 0) | ((!land || r) ? kW : 0);
     }
     pass_done();
-    for (int i = 1 + lane; i <= n; i += 64) {  // S, L, M, E
-        int fl = w.ix(kFL, i);
-        const int fm1 = i >= 2 ? w.ix(kFL, i - 1) : 0;
-        const int fm2 = i >= 3 ? w.ix(kFL, i - 2) : 0;
-        const bool land = fl & kLand;
-        const bool st = i == 1 ? !land : (!land && (fm1 & kLand) && !(fm1 & kR));
-        bool le = false;
-        if (st && i >= 2) {
-            const double hi = w(kH, i);
-            const double flood = (hi * hi) / G - w(kDEP, i) + w(kDEP, i - 1);
-            le = (flood > ground) && (w(kU, i) <= 0);
-        }
-        const bool mg = st && le && i >= 3 && (fm2 & kW);
-        const bool en = i >= 2 && land && (fm1 & kW) && !(fl & kR);
-        fl |= (st ? kS : 0) | (le ? kL : 0) | (mg ? kM : 0) | (en ? kE : 0);
-        w.ix(kFL, i) = fl;
-    }
-    pass_done();
-    for (int i = 1 + lane; i <= n; i += 64) {  // expansions, final newland, ends not merged away
-        int fl = w.ix(kFL, i);
-        const int fp1 = i + 1 <= n ? w.ix(kFL, i + 1) : 0;
-        if (fl & kR) {  // the wet node i-1 floods node i (it is never changed itself)
-            w(kU, i) = w(kU, i - 1);
-            w(kV, i) = w(kV, i - 1);
-            w(kH, i) = celmin;
-            fl &= ~kNewland;
-        }
-        if (fp1 & kL) {  // the segment starting at i+1 floods node i
-            w(kU, i) = w(kU, i + 1);
-            w(kV, i) = w(kV, i + 1);
-            w(kH, i) = celmin;
-            fl &= ~kNewland;
-        }
-        if ((fl & kE) && !(fp1 & kM)) fl |= kEnd;
-        w.ix(kFL, i) = fl;
-    }
-    pass_done();
-    // segment numbers: starts not merged, counted along the line 64 nodes at
-    // a time; a node is inside a segment (lghost < i < rghost) when more
-    // segments have started than ended up to it, or when the start after it
-    // floods it (left expansion), never at the line's ends
+    // S, L, M, E of node i and of node i+1 (whose left expansion floods i
+    // and whose merge removes i's end), nodes still dry, and the segment
+    // numbers: starts not merged, counted along the line 64 nodes at a time;
+    // a node is inside a segment (lghost < i < rghost) when more segments
+    // have started than ended up to it, or when the start after it floods it,
+    // never at the line's ends.  A lane rewrites its node's flags keeping
+    // land, R and W, the only bits the neighbours read here.
     int kseg = 0, kend = 0;
     for (int c = 0; c < n; c += 64) {
         const int i = c + 1 + lane;
-        const int fl = i <= n ? w.ix(kFL, i) : 0;
-        const int fp1 = i + 1 <= n ? w.ix(kFL, i + 1) : 0;
-        const bool st = (fl & kS) && !(fl & kM);
-        const bool en = (fl & kEnd) != 0;
+        int fl = 0;
+        bool st = false, en = false, fbn = false;
+        if (i <= n) {
+            fl = w.ix(kFL, i) & (kLand | kR | kW);
+            const int fm1 = i >= 2 ? w.ix(kFL, i - 1) : 0;
+            const int fm2 = i >= 3 ? w.ix(kFL, i - 2) : 0;
+            const int fp1 = i + 1 <= n ? w.ix(kFL, i + 1) : 0;
+            const bool land = fl & kLand;
+            const bool s0 = i == 1 ? !land : (!land && (fm1 & kLand) && !(fm1 & kR));
+            bool l0 = false;
+            if (s0 && i >= 2) {
+                const double hi = cel[at(i)];
+                const double flood = (hi * hi) / G - dep[at(i)] + dep[at(i - 1)];
+                l0 = (flood > ground) && (nv[at(i)] <= 0);
+            }
+            const bool m0 = s0 && l0 && i >= 3 && (fm2 & kW);
+            const bool e0 = i >= 2 && land && (fm1 & kW) && !(fl & kR);
+            const bool sn = i + 1 <= n && !(fp1 & kLand) && land && !(fl & kR);
+            bool ln = false;
+            if (sn) {
+                const double hn = cel[at(i + 1)];
+                const double flood = (hn * hn) / G - dep[at(i + 1)] + dep[at(i)];
+                ln = (flood > ground) && (nv[at(i + 1)] <= 0);
+            }
+            const bool mn = sn && ln && i >= 2 && (fm1 & kW);
+            const bool end = e0 && !mn;
+            const bool newland = land && !(fl & kR) && !ln;
+            fl |= (newland ? kNewland : 0) | (s0 ? kS : 0) | (l0 ? kL : 0) | (m0 ? kM : 0) | (e0 ? kE : 0) |
+                  (end ? kEnd : 0) | (ln ? kFromR : 0);
+            st = s0 && !m0;
+            en = end;
+            fbn = sn && !mn && ln;
+        }
         const unsigned long long ms = __ballot(st), me = __ballot(en);
         const unsigned long long upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);  // lanes <= this one
         const int below = __popcll(ms & ((1ull << lane) - 1ull));
         if (st) w.ix(kLG, kseg + below + 1) = i == 1 ? 1 : ((fl & kL) ? (i - 2 > 1 ? i - 2 : 1) : i - 1);
         if (en) w.ix(kRG, kseg + below + (st ? 1 : 0)) = i;
-        const bool flooded_by_next = (fp1 & kS) && !(fp1 & kM) && (fp1 & kL);
-        const bool inside = kseg + __popcll(ms & upto) > kend + __popcll(me & upto) || flooded_by_next;
-        if (i >= 2 && i <= n - 1 && inside) w.ix(kFL, i) = fl | kIn;
+        const bool inside = kseg + __popcll(ms & upto) > kend + __popcll(me & upto) || fbn;
+        if (i <= n) w.ix(kFL, i) = fl | ((i >= 2 && i <= n - 1 && inside) ? kIn : 0);
         kseg += __popcll(ms);
         kend += __popcll(me);
     }
     if (kseg == 0) return;  // cliffs.f:113: the line is left untouched
+    if (kend < kseg && lane == 0) w.ix(kRG, kseg) = n;  // the last segment runs to the line's end
     pass_done();
     const int k1 = (w.ix(kFL, 1) & kLand) ? 1 : 2;
-    int rlast = w.ix(kRG, kseg);
-    if (rlast == 0) {  // the last segment runs to the line's end
-        rlast = n;
-        if (lane == 0) w.ix(kRG, kseg) = n;
-    }
+    const int rlast = w.ix(kRG, kseg);
     const int k2 = (w.ix(kFL, rlast) & kNewland) ? kseg : kseg - 1;
-    pass_done();
 
-    // nodes still dry are emptied (cliffs.f:114-124); Riemann invariants
-    // inside the segments, 0 elsewhere (cliffs.f:125-136)
-    for (int i = 1 + lane; i <= n; i += 64) {
+    // Node i after the run-up expansions (cliffs.f:81-107: a flooded node
+    // takes c_min and the velocities of the wet node that floods it) and the
+    // emptying of nodes still dry (cliffs.f:114-124).  The sources are
+    // original values: a node that floods is wet and never changed itself.
+    auto post = [&](int i) -> Node {
+        if (i > n) return {0.0, 0.0, 0.0};
         const int fl = w.ix(kFL, i);
-        double h = w(kH, i), u = w(kU, i), v = w(kV, i);
-        if (fl & kNewland) {
-            h = u = v = 0;
-            w(kH, i) = 0;
-            w(kU, i) = 0;
-            w(kV, i) = 0;
-        }
-        w(kVV, i) = v;
-        const bool in = (fl & kIn) != 0;
-        w(kP, i) = in ? u + 2 * h : 0.0;
-        w(kQ, i) = in ? u - 2 * h : 0.0;
-    }
-    pass_done();
-    // reflective ghosts at shorelines (cliffs.f:137-152), one segment per lane
-    for (int k = k1 + lane; k <= kseg; k += 64) {
-        const int j = w.ix(kLG, k) + 1;
-        w(kPL1, k) = -w(kQ, j);
-        w(kPL2, k) = -w(kP, j);
-        w(kPL3, k) = w(kV, j);
-        w(kPL4, k) = w(kDEP, j);
-    }
-    for (int k = 1 + lane; k <= k2; k += 64) {
-        const int j = w.ix(kRG, k) - 1;
-        w(kPR1, k) = -w(kQ, j);
-        w(kPR2, k) = -w(kP, j);
-        w(kPR3, k) = w(kV, j);
-        w(kPR4, k) = w(kDEP, j);
-    }
-    pass_done();
-    // the line's own end nodes (cliffs.f:153-164), after the ghosts they may replace
-    if (lane == 0) {
-        if (!newland_of(w, 1)) {
-            const double u = w(kU, 1), h = w(kH, 1);
-            w(kP, 1) = u + 2 * h;
-            w(kQ, 1) = u - 2 * h;
-            w(kPL1, 1) = w(kP, 1);
-            w(kPL2, 1) = w(kQ, 1);
-            w(kPL3, 1) = w(kV, 1);
-            w(kPL4, 1) = w(kDEP, 1);
-        }
-        if (!newland_of(w, n)) {
-            const double u = w(kU, n), h = w(kH, n);
-            w(kP, n) = u + 2 * h;
-            w(kQ, n) = u - 2 * h;
-            w(kPR1, kseg) = w(kP, n);
-            w(kPR2, kseg) = w(kQ, n);
-            w(kPR3, kseg) = w(kV, n);
-            w(kPR4, kseg) = w(kDEP, n);
-        }
-    }
-    pass_done();
+        if (fl & kNewland) return {0.0, 0.0, 0.0};
+        if (fl & kFromR) return {celmin, nv[at(i + 1)], tv[at(i + 1)]};
+        if (fl & kR) return {celmin, nv[at(i - 1)], tv[at(i - 1)]};
+        return {cel[at(i)], nv[at(i)], tv[at(i)]};
+    };
+    // Riemann invariants P = u + 2c, Q = u - 2c inside the segments and at
+    // the line's wet end nodes (cliffs.f:125-136, 153-164).  At a segment's
+    // first (last) interior node the node before (after) it is a ghost: the
+    // line's end node 1 (n) itself when wet (cliffs.f:153-164), else the
+    // reflection (-Q, -P, v, d) of the interior node (cliffs.f:137-152).
+    // VTCS-2 integration of the interior nodes (cliffs.f:168-253) and c, u
+    // from the new invariants (cliffs.f:255-262), 64 nodes at a time, in
+    // place: a chunk's own nodes and the next chunk's first node are read
+    // before the chunk is written, and the previous chunk's last node is
+    // kept in registers.
+    Node cur = post(1 + lane);
+    Node prev{0.0, 0.0, 0.0};
+    for (int c = 0; c < n; c += 64) {
+        const int i = c + 1 + lane;
+        const Node nxt = c + 64 < n ? post(i + 64) : Node{0.0, 0.0, 0.0};
+        Node lft, rgt;
+        lft.h = __shfl_up(cur.h, 1);
+        lft.u = __shfl_up(cur.u, 1);
+        lft.v = __shfl_up(cur.v, 1);
+        rgt.h = __shfl_down(cur.h, 1);
+        rgt.u = __shfl_down(cur.u, 1);
+        rgt.v = __shfl_down(cur.v, 1);
+        const double nh = lane_value(nxt.h, 0), nu = lane_value(nxt.u, 0), nvv = lane_value(nxt.v, 0);
+        if (lane == 0) lft = prev;
+        if (lane == 63) rgt = Node{nh, nu, nvv};
+        if (i <= n) {
+            const int fl = w.ix(kFL, i);
+            double H = cur.h, U = cur.u, VV = cur.v;
+            if (fl & kIn) {  // 2 <= i <= n-1
+                const double hc = cur.h, uj = cur.u, vj = cur.v;
+                const double pj = uj + 2 * hc, qj = uj - 2 * hc;
+                const double dj = dep[at(i)];
+                const int flm = w.ix(kFL, i - 1), flp = w.ix(kFL, i + 1);
+                double p1, q1, v1, d1, p2, q2, v2, d2;
+                if ((flm & kIn) || (i - 1 == 1 && !(flm & kNewland))) {
+                    p1 = lft.u + 2 * lft.h, q1 = lft.u - 2 * lft.h, v1 = lft.v, d1 = dep[at(i - 1)];
+                } else {
+                    p1 = -qj, q1 = -pj, v1 = vj, d1 = dj;
+                }
+                if ((flp & kIn) || (i + 1 == n && !(flp & kNewland))) {
+                    p2 = rgt.u + 2 * rgt.h, q2 = rgt.u - 2 * rgt.h, v2 = rgt.v, d2 = dep[at(i + 1)];
+                } else {
+                    p2 = -qj, q2 = -pj, v2 = vj, d2 = dj;
+                }
+                const double hj = hc * hc / G;
+                // almost-Manning friction (cliffs.f:204)
+                const double cc = G * a.crough * uj * sqrt((uj * uj + vj * vj) / hj) / hj;
+                const double sphr = (pj - qj) * (pj + qj) * scl(i);
+                const double fm = dx(i - 1), fp = dx(i);
+                const double dtx = dt / (fm + fp);
+                const double dpj = G * (d2 - dj), dpm = G * (dj - d1);
+                const bool wet3 = (dj > ground) && (d1 > ground) && (d2 > ground);
 
-    // VTCS-2 integration of the segments' interior nodes (cliffs.f:168-253)
-    // and c, u from the new invariants (cliffs.f:255-262), node by node: a
-    // node reads its neighbours' P, Q, v and depth only, which no node of
-    // this pass changes, and its own c and u before it replaces them
-    for (int k = 1; k <= kseg; k++) {
-        const int i1 = w.ix(kLG, k) + 1, i2 = w.ix(kRG, k) - 1;
-        for (int i = i1 + lane; i <= i2; i += 64) {
-            double p1, q1, v1, d1, p2, q2, v2, d2;
-            if (i == i1) {
-                p1 = w(kPL1, k), q1 = w(kPL2, k), v1 = w(kPL3, k), d1 = w(kPL4, k);
-            } else {
-                p1 = w(kP, i - 1), q1 = w(kQ, i - 1), v1 = w(kV, i - 1), d1 = w(kDEP, i - 1);
+                // P family
+                double ej = 3 * pj + qj, e2 = 3 * p2 + q2, e1 = 3 * p1 + q1;
+                double Qpj = (0.125 * (e2 + ej) * (p2 - pj) - dpj) / fp;
+                double Qjm = (0.125 * (ej + e1) * (pj - p1) - dpm) / fm;
+                double Qsum;
+                if ((e1 < 0) && (e2 > 0) && wet3)
+                    Qsum = (0.25 * ej * (p2 - p1) - 2 * G * (sqrt(dj * d2) - sqrt(dj * d1))) / (fm + fp);
+                else
+                    Qsum = (Qpj + Qjm) / 2;
+                Qsum = Qsum + 0.25 * ej * dtx * (Qjm - Qpj);
+                const double pp = pj - dt * (Qsum + cc - sphr);
+
+                // Q family
+                ej = 3 * qj + pj, e2 = 3 * q2 + p2, e1 = 3 * q1 + p1;
+                Qpj = (0.125 * (e2 + ej) * (q2 - qj) - dpj) / fp;
+                Qjm = (0.125 * (ej + e1) * (qj - q1) - dpm) / fm;
+                if ((e1 < 0) && (e2 > 0) && wet3)
+                    Qsum = (0.25 * ej * (q2 - q1) - 2 * G * (sqrt(dj * d2) - sqrt(dj * d1))) / (fm + fp);
+                else
+                    Qsum = (Qpj + Qjm) / 2;
+                Qsum = Qsum + 0.25 * ej * dtx * (Qjm - Qpj);
+                const double qq = qj - dt * (Qsum + cc + sphr);
+
+                // tangential velocity (cliffs.f:249-250)
+                Qsum = (v2 - vj) / fp - (vj - v1) / fm;
+                VV = vj - uj * dtx * (v2 - v1 - Qsum * (uj * dt + fp - fm));
+                H = (pp - qq) / 4;
+                U = (pp + qq) / 2;
             }
-            if (i == i2) {
-                p2 = w(kPR1, k), q2 = w(kPR2, k), v2 = w(kPR3, k), d2 = w(kPR4, k);
-            } else {
-                p2 = w(kP, i + 1), q2 = w(kQ, i + 1), v2 = w(kV, i + 1), d2 = w(kDEP, i + 1);
+            // the edge integration reads nodes 1, 2, n-1, n as they were
+            if (i <= 2) {
+                sd[3 * (i - 1)] = cur.h, sd[3 * (i - 1) + 1] = cur.u, sd[3 * (i - 1) + 2] = cur.v;
             }
-            const double uj = w(kU, i), vj = w(kV, i), hc = w(kH, i);
-            const double hj = hc * hc / G;
-            // almost-Manning friction (cliffs.f:204)
-            const double cc = G * a.crough * uj * sqrt((uj * uj + vj * vj) / hj) / hj;
-            const double pj = w(kP, i), qj = w(kQ, i);
-            const double sphr = (pj - qj) * (pj + qj) * w(kSCL, i);
-            const double fm = w(kDX, i - 1), fp = w(kDX, i);
-            const double dtx = dt / (fm + fp);
-            const double dj = w(kDEP, i);
-            const double dpj = G * (d2 - dj), dpm = G * (dj - d1);
-            const bool wet3 = (dj > ground) && (d1 > ground) && (d2 > ground);
-
-            // P family
-            double ej = 3 * pj + qj, e2 = 3 * p2 + q2, e1 = 3 * p1 + q1;
-            double Qpj = (0.125 * (e2 + ej) * (p2 - pj) - dpj) / fp;
-            double Qjm = (0.125 * (ej + e1) * (pj - p1) - dpm) / fm;
-            double Qsum;
-            if ((e1 < 0) && (e2 > 0) && wet3)
-                Qsum = (0.25 * ej * (p2 - p1) - 2 * G * (sqrt(dj * d2) - sqrt(dj * d1))) / (fm + fp);
-            else
-                Qsum = (Qpj + Qjm) / 2;
-            Qsum = Qsum + 0.25 * ej * dtx * (Qjm - Qpj);
-            const double pp = pj - dt * (Qsum + cc - sphr);
-
-            // Q family
-            ej = 3 * qj + pj, e2 = 3 * q2 + p2, e1 = 3 * q1 + p1;
-            Qpj = (0.125 * (e2 + ej) * (q2 - qj) - dpj) / fp;
-            Qjm = (0.125 * (ej + e1) * (qj - q1) - dpm) / fm;
-            if ((e1 < 0) && (e2 > 0) && wet3)
-                Qsum = (0.25 * ej * (q2 - q1) - 2 * G * (sqrt(dj * d2) - sqrt(dj * d1))) / (fm + fp);
-            else
-                Qsum = (Qpj + Qjm) / 2;
-            Qsum = Qsum + 0.25 * ej * dtx * (Qjm - Qpj);
-            const double qq = qj - dt * (Qsum + cc + sphr);
-
-            // tangential velocity (cliffs.f:249-250)
-            Qsum = (v2 - vj) / fp - (vj - v1) / fm;
-            w(kVV, i) = vj - uj * dtx * (v2 - v1 - Qsum * (uj * dt + fp - fm));
-            w(kH, i) = (pp - qq) / 4;
-            w(kU, i) = (pp + qq) / 2;
+            if (i >= n - 1) {
+                const int s = i == n ? 9 : 6;
+                sd[s] = cur.h, sd[s + 1] = cur.u, sd[s + 2] = cur.v;
+            }
+            const size_t o = at(i);
+            cel[o] = H;
+            nv[o] = U;
+            tv[o] = VV;
         }
+        prev = Node{lane_value(cur.h, 63), lane_value(cur.u, 63), lane_value(cur.v, 63)};
+        cur = nxt;
     }
     pass_done();
     // shoreline surface-angle limiter on newly flooded end nodes
@@ -338,22 +322,25 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
     // per lane (segments are disjoint, so the ends of one pass do not meet)
     for (int k = k1 + lane; k <= kseg; k += 64) {
         const int i = w.ix(kLG, k) + 1, j = i + 1;
-        if (land_of(w, i)) {
-            if (kSqr2 * w(kH, i) > w(kH, j)) w(kH, i) = w(kH, j) / kSqr2;
-            if (w(kU, i) < w(kU, j)) w(kU, i) = w(kU, j);
+        if ((w.ix(kFL, i) & kLand) && j <= n) {
+            const size_t oi = at(i), oj = at(j);
+            if (kSqr2 * cel[oi] > cel[oj]) cel[oi] = cel[oj] / kSqr2;
+            if (nv[oi] < nv[oj]) nv[oi] = nv[oj];
         }
     }
     pass_done();
     for (int k = 1 + lane; k <= k2; k += 64) {
         const int i = w.ix(kRG, k) - 1, j = i - 1;
-        if (land_of(w, i)) {
-            if (kSqr2 * w(kH, i) > w(kH, j)) w(kH, i) = w(kH, j) / kSqr2;
-            if (w(kU, i) > w(kU, j)) w(kU, i) = w(kU, j);
+        if (j >= 1 && (w.ix(kFL, i) & kLand)) {
+            const size_t oi = at(i), oj = at(j);
+            if (kSqr2 * cel[oi] > cel[oj]) cel[oi] = cel[oj] / kSqr2;
+            if (nv[oi] > nv[oj]) nv[oi] = nv[oj];
         }
     }
     pass_done();
 
-    // open or forced edges at the line's ends (cliffs.f:287-329), lane 0
+    // open or forced edges at the line's ends (cliffs.f:287-329), lane 0,
+    // from the invariants of nodes 1, 2 (n-1, n) before the integration
     if (lane == 0) {
         const int mx = a.mx;
         auto edge = [&](const double* e, int col) {
@@ -361,43 +348,35 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
         };
         const double b1n = edge(a.e1, iu - 1), b1t = edge(a.e1, 3 - iu - 1), b1h = edge(a.e1, 2);
         const double b2n = edge(a.e2, iu - 1), b2t = edge(a.e2, 3 - iu - 1), b2h = edge(a.e2, 2);
-        double flood = b1h + w(kDEP, 1);
-        if (!newland_of(w, 1) && !newland_of(w, 2) && (flood > ground)) {
-            const double pj = w(kP, 1), qj = w(kQ, 1), p2 = w(kP, 2), q2 = w(kQ, 2);
-            const double sphr = (pj - qj) * (pj + qj) * w(kSCL, 1);
-            const double Qpj = (3 * (q2 + qj) + p2 + pj) * (q2 - qj) / 8 - G * (w(kDEP, 2) - w(kDEP, 1));
-            const double qq = qj - dt * (Qpj / w(kDX, 1) + sphr);
+        double flood = b1h + dep[at(1)];
+        if (!(w.ix(kFL, 1) & kNewland) && !(w.ix(kFL, 2) & kNewland) && (flood > ground)) {
+            const double h1 = sd[0], u1 = sd[1], v1 = sd[2], h2 = sd[3], u2 = sd[4], v2 = sd[5];
+            const bool in2 = (w.ix(kFL, 2) & kIn) != 0;
+            const double pj = u1 + 2 * h1, qj = u1 - 2 * h1;
+            const double p2 = in2 ? u2 + 2 * h2 : 0.0, q2 = in2 ? u2 - 2 * h2 : 0.0;
+            const double sphr = (pj - qj) * (pj + qj) * scl(1);
+            const double Qpj = (3 * (q2 + qj) + p2 + pj) * (q2 - qj) / 8 - G * (dep[at(2)] - dep[at(1)]);
+            const double qq = qj - dt * (Qpj / dx(1) + sphr);
             const double pp = b1n + 2 * sqrt(G * flood);
-            const double u1 = w(kU, 1);
-            w(kVV, 1) = u1 < 0 ? w(kV, 1) - dt * (w(kV, 2) - w(kV, 1)) * u1 / w(kDX, 1) : b1t;
-            w(kH, 1) = (pp - qq) / 4;
-            w(kU, 1) = (pp + qq) / 2;
+            const size_t o = at(1);
+            tv[o] = u1 < 0 ? v1 - dt * (v2 - v1) * u1 / dx(1) : b1t;
+            cel[o] = (pp - qq) / 4;
+            nv[o] = (pp + qq) / 2;
         }
-        flood = w(kDEP, n) + b2h;
-        if (!newland_of(w, n - 1) && !newland_of(w, n) && (flood > ground)) {
-            const double pj = w(kP, n), qj = w(kQ, n), p1 = w(kP, n - 1), q1 = w(kQ, n - 1);
-            const double sphr = (pj - qj) * (pj + qj) * w(kSCL, n);
-            const double Qjm = (3 * (p1 + pj) + q1 + qj) * (pj - p1) / 8 - G * (w(kDEP, n) - w(kDEP, n - 1));
-            const double pp = pj - dt * (Qjm / w(kDX, n - 1) - sphr);
+        flood = dep[at(n)] + b2h;
+        if (!(w.ix(kFL, n - 1) & kNewland) && !(w.ix(kFL, n) & kNewland) && (flood > ground)) {
+            const double h1 = sd[6], u1 = sd[7], v1 = sd[8], hn = sd[9], un = sd[10], vn = sd[11];
+            const bool in1 = (w.ix(kFL, n - 1) & kIn) != 0;
+            const double pj = un + 2 * hn, qj = un - 2 * hn;
+            const double p1 = in1 ? u1 + 2 * h1 : 0.0, q1 = in1 ? u1 - 2 * h1 : 0.0;
+            const double sphr = (pj - qj) * (pj + qj) * scl(n);
+            const double Qjm = (3 * (p1 + pj) + q1 + qj) * (pj - p1) / 8 - G * (dep[at(n)] - dep[at(n - 1)]);
+            const double pp = pj - dt * (Qjm / dx(n - 1) - sphr);
             const double qq = b2n - 2 * sqrt(G * flood);
-            const double un = w(kU, n);
-            w(kVV, n) = un > 0 ? w(kV, n) - dt * (w(kV, n) - w(kV, n - 1)) * un / w(kDX, n - 1) : b2t;
-            w(kH, n) = (pp - qq) / 4;
-            w(kU, n) = (pp + qq) / 2;
-        }
-    }
-    pass_done();
-
-    // scatter (cliffs.f:331-343)
-    for (int i = 1 + lane; i <= n; i += 64) {
-        const size_t o = at(i);
-        a.cel[o] = w(kH, i);
-        if (iu == 1) {
-            a.xvel[o] = w(kU, i);
-            a.yvel[o] = w(kVV, i);
-        } else {
-            a.xvel[o] = w(kVV, i);
-            a.yvel[o] = w(kU, i);
+            const size_t o = at(n);
+            tv[o] = un > 0 ? vn - dt * (vn - v1) * un / dx(n - 1) : b2t;
+            cel[o] = (pp - qq) / 4;
+            nv[o] = (pp + qq) / 2;
         }
     }
 }
@@ -726,7 +705,7 @@ int cliffs64_create(const cliffs_grid_desc* d, cliffs64_handle** out) {
               hipMalloc(&h->s1, n_s1 * 8) == hipSuccess && hipMalloc(&h->s2, n_s2 * 8) == hipSuccess &&
               hipMalloc(&h->zeta, (size_t)ny * 8) == hipSuccess &&
               hipMalloc(&h->edges, (size_t)6 * h->mx * 8) == hipSuccess &&
-              hipMalloc(&h->wd, (size_t)kDoubles * wn * 8) == hipSuccess &&
+              hipMalloc(&h->wd, (size_t)kStash * std::max(nx, ny) * 8) == hipSuccess &&
               hipMalloc(&h->wi, (size_t)kInts * wn * 4) == hipSuccess &&
               hipMalloc(&h->health, 2 * sizeof(unsigned long long)) == hipSuccess &&
               hipMalloc(&h->celT, m * 8) == hipSuccess && hipMalloc(&h->xvelT, m * 8) == hipSuccess &&
