@@ -91,6 +91,7 @@ __device__ __forceinline__ double lane_value(double x, int l) {
 struct Node { double h, u, v; };  // celerity, normal and tangential velocity
 
 constexpr int kLineWaves = 4;  // lines per block
+constexpr int kChunks = 4;     // 64-node chunks whose loads a flag pass keeps in flight
 
 // One wave per line, one node (or segment) per lane in every pass.  Two
 // passes over the line's flags decide the segment scan; one pass over the
@@ -132,18 +133,28 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
     //         S, L and W(i-2), i.e. a single land node between the two
     //   E(i)  the segment ends at land node i: W(i-1) and not R(i)
     // so the lanes decide them node by node, and a prefix count of the
-    // starts numbers the segments.  This pass: land, R, W.
-    for (int i = 1 + lane; i <= n; i += 64) {
-        const bool land = cel[at(i)] <= celmin;
-        bool r = false;
-        if (i >= 2 && land) {
-            const double hp = cel[at(i - 1)];
-            if (!(hp <= celmin)) {
-                const double flood = (hp * hp) / G - dep[at(i - 1)] + dep[at(i)];
+    // starts numbers the segments.  This pass: land, R, W, kChunks chunks of
+    // 64 nodes at a time (their loads in flight together).
+    for (int c = 0; c < n; c += 64 * kChunks) {
+        double h[kChunks], hp[kChunks];
+#pragma unroll
+        for (int u = 0; u < kChunks; u++) {
+            const int i = c + 1 + lane + 64 * u;
+            h[u] = i <= n ? cel[at(i)] : 0.0;
+            hp[u] = (i >= 2 && i <= n) ? cel[at(i - 1)] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kChunks; u++) {
+            const int i = c + 1 + lane + 64 * u;
+            if (i > n) break;
+            const bool land = h[u] <= celmin;
+            bool r = false;
+            if (i >= 2 && land && !(hp[u] <= celmin)) {
+                const double flood = (hp[u] * hp[u]) / G - dep[at(i - 1)] + dep[at(i)];
                 r = (flood > ground) && (nv[at(i - 1)] >= 0);
             }
+            w.ix(kFL, i) = (land ? kLand : 0) | (r ? kR : 0) | ((!land || r) ? kW : 0);
         }
-        w.ix(kFL, i) = (land ? kLand : 0) | (r ? kR : 0) | ((!land || r) ? kW : 0);
     }
     pass_done();
     // S, L, M, E of node i and of node i+1 (whose left expansion floods i
@@ -152,52 +163,64 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
     // a node is inside a segment (lghost < i < rghost) when more segments
     // have started than ended up to it, or when the start after it floods it,
     // never at the line's ends.  A lane rewrites its node's flags keeping
-    // land, R and W, the only bits the neighbours read here.
+    // land, R and W, the only bits the neighbours read here; the flags of
+    // kChunks chunks are loaded together.
     int kseg = 0, kend = 0;
-    for (int c = 0; c < n; c += 64) {
-        const int i = c + 1 + lane;
-        int fl = 0;
-        bool st = false, en = false, fbn = false;
-        if (i <= n) {
-            fl = w.ix(kFL, i) & (kLand | kR | kW);
-            const int fm1 = i >= 2 ? w.ix(kFL, i - 1) : 0;
-            const int fm2 = i >= 3 ? w.ix(kFL, i - 2) : 0;
-            const int fp1 = i + 1 <= n ? w.ix(kFL, i + 1) : 0;
-            const bool land = fl & kLand;
-            const bool s0 = i == 1 ? !land : (!land && (fm1 & kLand) && !(fm1 & kR));
-            bool l0 = false;
-            if (s0 && i >= 2) {
-                const double hi = cel[at(i)];
-                const double flood = (hi * hi) / G - dep[at(i)] + dep[at(i - 1)];
-                l0 = (flood > ground) && (nv[at(i)] <= 0);
-            }
-            const bool m0 = s0 && l0 && i >= 3 && (fm2 & kW);
-            const bool e0 = i >= 2 && land && (fm1 & kW) && !(fl & kR);
-            const bool sn = i + 1 <= n && !(fp1 & kLand) && land && !(fl & kR);
-            bool ln = false;
-            if (sn) {
-                const double hn = cel[at(i + 1)];
-                const double flood = (hn * hn) / G - dep[at(i + 1)] + dep[at(i)];
-                ln = (flood > ground) && (nv[at(i + 1)] <= 0);
-            }
-            const bool mn = sn && ln && i >= 2 && (fm1 & kW);
-            const bool end = e0 && !mn;
-            const bool newland = land && !(fl & kR) && !ln;
-            fl |= (newland ? kNewland : 0) | (s0 ? kS : 0) | (l0 ? kL : 0) | (m0 ? kM : 0) | (e0 ? kE : 0) |
-                  (end ? kEnd : 0) | (ln ? kFromR : 0);
-            st = s0 && !m0;
-            en = end;
-            fbn = sn && !mn && ln;
+    for (int c0 = 0; c0 < n; c0 += 64 * kChunks) {
+        int f0[kChunks], fm1[kChunks], fm2[kChunks], fp1[kChunks];
+#pragma unroll
+        for (int u = 0; u < kChunks; u++) {
+            const int i = c0 + 1 + lane + 64 * u;
+            f0[u] = i <= n ? w.ix(kFL, i) : 0;
+            fm1[u] = (i >= 2 && i <= n) ? w.ix(kFL, i - 1) : 0;
+            fm2[u] = (i >= 3 && i <= n) ? w.ix(kFL, i - 2) : 0;
+            fp1[u] = i + 1 <= n ? w.ix(kFL, i + 1) : 0;
         }
-        const unsigned long long ms = __ballot(st), me = __ballot(en);
-        const unsigned long long upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);  // lanes <= this one
-        const int below = __popcll(ms & ((1ull << lane) - 1ull));
-        if (st) w.ix(kLG, kseg + below + 1) = i == 1 ? 1 : ((fl & kL) ? (i - 2 > 1 ? i - 2 : 1) : i - 1);
-        if (en) w.ix(kRG, kseg + below + (st ? 1 : 0)) = i;
-        const bool inside = kseg + __popcll(ms & upto) > kend + __popcll(me & upto) || fbn;
-        if (i <= n) w.ix(kFL, i) = fl | ((i >= 2 && i <= n - 1 && inside) ? kIn : 0);
-        kseg += __popcll(ms);
-        kend += __popcll(me);
+#pragma unroll
+        for (int u = 0; u < kChunks; u++) {
+            const int c = c0 + 64 * u;
+            if (c >= n) break;  // wave-uniform
+            const int i = c + 1 + lane;
+            int fl = 0;
+            bool st = false, en = false, fbn = false;
+            if (i <= n) {
+                fl = f0[u] & (kLand | kR | kW);
+                const bool land = fl & kLand;
+                const bool s0 = i == 1 ? !land : (!land && (fm1[u] & kLand) && !(fm1[u] & kR));
+                bool l0 = false;
+                if (s0 && i >= 2) {
+                    const double hi = cel[at(i)];
+                    const double flood = (hi * hi) / G - dep[at(i)] + dep[at(i - 1)];
+                    l0 = (flood > ground) && (nv[at(i)] <= 0);
+                }
+                const bool m0 = s0 && l0 && i >= 3 && (fm2[u] & kW);
+                const bool e0 = i >= 2 && land && (fm1[u] & kW) && !(fl & kR);
+                const bool sn = i + 1 <= n && !(fp1[u] & kLand) && land && !(fl & kR);
+                bool ln = false;
+                if (sn) {
+                    const double hn = cel[at(i + 1)];
+                    const double flood = (hn * hn) / G - dep[at(i + 1)] + dep[at(i)];
+                    ln = (flood > ground) && (nv[at(i + 1)] <= 0);
+                }
+                const bool mn = sn && ln && i >= 2 && (fm1[u] & kW);
+                const bool end = e0 && !mn;
+                const bool newland = land && !(fl & kR) && !ln;
+                fl |= (newland ? kNewland : 0) | (s0 ? kS : 0) | (l0 ? kL : 0) | (m0 ? kM : 0) | (e0 ? kE : 0) |
+                      (end ? kEnd : 0) | (ln ? kFromR : 0);
+                st = s0 && !m0;
+                en = end;
+                fbn = sn && !mn && ln;
+            }
+            const unsigned long long ms = __ballot(st), me = __ballot(en);
+            const unsigned long long upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);  // lanes <= this one
+            const int below = __popcll(ms & ((1ull << lane) - 1ull));
+            if (st) w.ix(kLG, kseg + below + 1) = i == 1 ? 1 : ((fl & kL) ? (i - 2 > 1 ? i - 2 : 1) : i - 1);
+            if (en) w.ix(kRG, kseg + below + (st ? 1 : 0)) = i;
+            const bool inside = kseg + __popcll(ms & upto) > kend + __popcll(me & upto) || fbn;
+            if (i <= n) w.ix(kFL, i) = fl | ((i >= 2 && i <= n - 1 && inside) ? kIn : 0);
+            kseg += __popcll(ms);
+            kend += __popcll(me);
+        }
     }
     if (kseg == 0) return;  // cliffs.f:113: the line is left untouched
     if (kend < kseg && lane == 0) w.ix(kRG, kseg) = n;  // the last segment runs to the line's end
@@ -225,14 +248,34 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
     // reflection (-Q, -P, v, d) of the interior node (cliffs.f:137-152).
     // VTCS-2 integration of the interior nodes (cliffs.f:168-253) and c, u
     // from the new invariants (cliffs.f:255-262), 64 nodes at a time, in
-    // place: a chunk's own nodes and the next chunk's first node are read
+    // place: a chunk's own nodes and the next two chunks' nodes are read
     // before the chunk is written, and the previous chunk's last node is
     // kept in registers.
+    // Flags, depths and spacings of node i and its neighbours.
+    struct Aux {
+        int fl, flm, flp;
+        double dj, d1, d2, fm, fp;
+    };
+    auto aux = [&](int i) -> Aux {
+        Aux x{0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (i <= n) {
+            x.fl = w.ix(kFL, i);
+            x.dj = dep[at(i)];
+            if (i >= 2) x.flm = w.ix(kFL, i - 1), x.d1 = dep[at(i - 1)], x.fm = dx(i - 1);
+            if (i + 1 <= n) x.flp = w.ix(kFL, i + 1), x.d2 = dep[at(i + 1)], x.fp = dx(i);
+        }
+        return x;
+    };
+    // The loads run ahead of the integration: the state of the chunk after
+    // next, the flags, depths and spacings of the next chunk.
     Node cur = post(1 + lane);
+    Node nxt = 64 < n ? post(65 + lane) : Node{0.0, 0.0, 0.0};
+    Aux ac = aux(1 + lane);
     Node prev{0.0, 0.0, 0.0};
     for (int c = 0; c < n; c += 64) {
         const int i = c + 1 + lane;
-        const Node nxt = c + 64 < n ? post(i + 64) : Node{0.0, 0.0, 0.0};
+        const Node nx2 = c + 128 < n ? post(i + 128) : Node{0.0, 0.0, 0.0};
+        const Aux an = c + 64 < n ? aux(i + 64) : Aux{0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0.0};
         Node lft, rgt;
         lft.h = __shfl_up(cur.h, 1);
         lft.u = __shfl_up(cur.u, 1);
@@ -244,21 +287,21 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
         if (lane == 0) lft = prev;
         if (lane == 63) rgt = Node{nh, nu, nvv};
         if (i <= n) {
-            const int fl = w.ix(kFL, i);
+            const int fl = ac.fl;
             double H = cur.h, U = cur.u, VV = cur.v;
             if (fl & kIn) {  // 2 <= i <= n-1
                 const double hc = cur.h, uj = cur.u, vj = cur.v;
                 const double pj = uj + 2 * hc, qj = uj - 2 * hc;
-                const double dj = dep[at(i)];
-                const int flm = w.ix(kFL, i - 1), flp = w.ix(kFL, i + 1);
+                const double dj = ac.dj;
+                const int flm = ac.flm, flp = ac.flp;
                 double p1, q1, v1, d1, p2, q2, v2, d2;
                 if ((flm & kIn) || (i - 1 == 1 && !(flm & kNewland))) {
-                    p1 = lft.u + 2 * lft.h, q1 = lft.u - 2 * lft.h, v1 = lft.v, d1 = dep[at(i - 1)];
+                    p1 = lft.u + 2 * lft.h, q1 = lft.u - 2 * lft.h, v1 = lft.v, d1 = ac.d1;
                 } else {
                     p1 = -qj, q1 = -pj, v1 = vj, d1 = dj;
                 }
                 if ((flp & kIn) || (i + 1 == n && !(flp & kNewland))) {
-                    p2 = rgt.u + 2 * rgt.h, q2 = rgt.u - 2 * rgt.h, v2 = rgt.v, d2 = dep[at(i + 1)];
+                    p2 = rgt.u + 2 * rgt.h, q2 = rgt.u - 2 * rgt.h, v2 = rgt.v, d2 = ac.d2;
                 } else {
                     p2 = -qj, q2 = -pj, v2 = vj, d2 = dj;
                 }
@@ -266,7 +309,7 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
                 // almost-Manning friction (cliffs.f:204)
                 const double cc = G * a.crough * uj * sqrt((uj * uj + vj * vj) / hj) / hj;
                 const double sphr = (pj - qj) * (pj + qj) * scl(i);
-                const double fm = dx(i - 1), fp = dx(i);
+                const double fm = ac.fm, fp = ac.fp;
                 const double dtx = dt / (fm + fp);
                 const double dpj = G * (d2 - dj), dpm = G * (dj - d1);
                 const bool wet3 = (dj > ground) && (d1 > ground) && (d2 > ground);
@@ -315,6 +358,8 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
         }
         prev = Node{lane_value(cur.h, 63), lane_value(cur.u, 63), lane_value(cur.v, 63)};
         cur = nxt;
+        nxt = nx2;
+        ac = an;
     }
     pass_done();
     // shoreline surface-angle limiter on newly flooded end nodes
