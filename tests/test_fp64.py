@@ -188,16 +188,11 @@ def test_fp64_forced_edges_and_line_ranges(gpu_lib):
     g.close()
 
 
-@pytest.mark.gpu
-@pytest.mark.parametrize("seed", range(8))
-def test_fp64_single_sweeps_random_wetdry(gpu_lib, seed):
-    """One x-sweep and one y-sweep over dense random wet/dry lines (every
-    run-up expansion, segment merge, ghost, limiter and edge branch of the
-    segment scan, on lines longer and shorter than a wave) with forced
-    edges, the metric term and friction on or off: bit-identical."""
+def _random_sweeps(gpu_lib, rng, nx, ny):
+    """One x-sweep and one y-sweep of the dense random wet/dry grid nx x ny
+    from the same state, forced edges, random metric term, friction on or
+    off: each bit-identical to the oracle's."""
     from test_gpu_parity import _random_wetdry
-    rng = np.random.default_rng(300 + seed)
-    nx, ny = int(rng.integers(40, 300)), int(rng.integers(40, 200))
     dep, cel, u, v, x, y = _random_wetdry(rng, nx, ny)
     s1, s2, zeta = synth.metrics(x, y, True)
     zeta = rng.normal(0, 1e-3, ny)
@@ -215,8 +210,29 @@ def test_fp64_single_sweeps_random_wetdry(gpu_lib, seed):
         g.sweep(iu, 1, nl, e1, e2)
         for a, b, name in zip(g.download(), orc.get_state()[:3], NAMES):
             bad = int((a.view(np.int64) != b.view(np.int64)).sum())
-            assert bad == 0, f"sweep {iu}, {name}: {bad} cells differ"
+            assert bad == 0, f"{nx}x{ny} sweep {iu}, {name}: {bad} cells differ"
     g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(8))
+def test_fp64_single_sweeps_random_wetdry(gpu_lib, seed):
+    """Dense random wet/dry lines (every run-up expansion, segment merge,
+    ghost, limiter and edge branch of the segment scan, on lines longer and
+    shorter than a wave): bit-identical."""
+    rng = np.random.default_rng(300 + seed)
+    _random_sweeps(gpu_lib, rng, int(rng.integers(40, 300)), int(rng.integers(40, 200)))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape", [(1025, 48), (48, 513), (257, 130), (640, 65), (1100, 1100)])
+def test_fp64_single_sweeps_random_wetdry_long_lines(gpu_lib, shape):
+    """Dense random wet/dry lines whose lengths fall on and just past the
+    kernel's 64-node chunks and its 256-node load batches, over several
+    batches (segments, expansions and merges that straddle a chunk edge
+    meet the carried neighbour and the look-ahead): bit-identical."""
+    rng = np.random.default_rng(sum(shape))
+    _random_sweeps(gpu_lib, rng, *shape)
 
 
 @pytest.mark.gpu
