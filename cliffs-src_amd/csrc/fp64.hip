@@ -135,14 +135,21 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
     // so the lanes decide them node by node, and a prefix count of the
     // starts numbers the segments.  This pass: land, R, W, kChunks chunks of
     // 64 nodes at a time (their loads in flight together).
+    double hlast = 0.0;
     for (int c = 0; c < n; c += 64 * kChunks) {
         double h[kChunks], hp[kChunks];
 #pragma unroll
         for (int u = 0; u < kChunks; u++) {
             const int i = c + 1 + lane + 64 * u;
             h[u] = i <= n ? cel[at(i)] : 0.0;
-            hp[u] = (i >= 2 && i <= n) ? cel[at(i - 1)] : 0.0;
         }
+#pragma unroll
+        for (int u = 0; u < kChunks; u++) {  // c(i-1): the lane before, or the chunk before's last
+            const double left = u == 0 ? hlast : lane_value(h[u - 1], 63);
+            const double up = __shfl_up(h[u], 1);
+            hp[u] = lane == 0 ? left : up;
+        }
+        hlast = lane_value(h[kChunks - 1], 63);
 #pragma unroll
         for (int u = 0; u < kChunks; u++) {
             const int i = c + 1 + lane + 64 * u;
@@ -261,8 +268,12 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
         if (i <= n) {
             x.fl = w.ix(kFL, i);
             x.dj = dep[at(i)];
-            if (i >= 2) x.flm = w.ix(kFL, i - 1), x.d1 = dep[at(i - 1)], x.fm = dx(i - 1);
-            if (i + 1 <= n) x.flp = w.ix(kFL, i + 1), x.d2 = dep[at(i + 1)], x.fp = dx(i);
+            if (i >= 2) x.flm = w.ix(kFL, i - 1), x.fm = dx(i - 1);
+            if (i + 1 <= n) x.flp = w.ix(kFL, i + 1), x.fp = dx(i);
+            // the neighbours' depths come from the neighbour lanes, but at
+            // the chunk's edges
+            if (lane == 0 && i >= 2) x.d1 = dep[at(i - 1)];
+            if (lane == 63 && i + 1 <= n) x.d2 = dep[at(i + 1)];
         }
         return x;
     };
@@ -286,6 +297,9 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
         const double nh = lane_value(nxt.h, 0), nu = lane_value(nxt.u, 0), nvv = lane_value(nxt.v, 0);
         if (lane == 0) lft = prev;
         if (lane == 63) rgt = Node{nh, nu, nvv};
+        const double dl = __shfl_up(ac.dj, 1), dr = __shfl_down(ac.dj, 1);
+        if (lane != 0) ac.d1 = dl;
+        if (lane != 63) ac.d2 = dr;
         if (i <= n) {
             const int fl = ac.fl;
             double H = cur.h, U = cur.u, VV = cur.v;
