@@ -248,16 +248,6 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
         if (fl & kR) return {celmin, nv[at(i - 1)], tv[at(i - 1)]};
         return {cel[at(i)], nv[at(i)], tv[at(i)]};
     };
-    // Riemann invariants P = u + 2c, Q = u - 2c inside the segments and at
-    // the line's wet end nodes (cliffs.f:125-136, 153-164).  At a segment's
-    // first (last) interior node the node before (after) it is a ghost: the
-    // line's end node 1 (n) itself when wet (cliffs.f:153-164), else the
-    // reflection (-Q, -P, v, d) of the interior node (cliffs.f:137-152).
-    // VTCS-2 integration of the interior nodes (cliffs.f:168-253) and c, u
-    // from the new invariants (cliffs.f:255-262), 64 nodes at a time, in
-    // place: a chunk's own nodes and the next two chunks' nodes are read
-    // before the chunk is written, and the previous chunk's last node is
-    // kept in registers.
     // Flags, depths and spacings of node i and its neighbours.
     struct Aux {
         int fl, flm, flp;
@@ -277,6 +267,16 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
         }
         return x;
     };
+    // Riemann invariants P = u + 2c, Q = u - 2c inside the segments and at
+    // the line's wet end nodes (cliffs.f:125-136, 153-164).  At a segment's
+    // first (last) interior node the node before (after) it is a ghost: the
+    // line's end node 1 (n) itself when wet (cliffs.f:153-164), else the
+    // reflection (-Q, -P, v, d) of the interior node (cliffs.f:137-152).
+    // VTCS-2 integration of the interior nodes (cliffs.f:168-253) and c, u
+    // from the new invariants (cliffs.f:255-262), 64 nodes at a time, in
+    // place: a chunk's own nodes and the next two chunks' nodes are read
+    // before the chunk is written, and the previous chunk's last node is
+    // kept in registers.
     // The loads run ahead of the integration: the state of the chunk after
     // next, the flags, depths and spacings of the next chunk.
     Node cur = post(1 + lane);
