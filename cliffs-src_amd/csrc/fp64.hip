@@ -133,56 +133,58 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
     //         S, L and W(i-2), i.e. a single land node between the two
     //   E(i)  the segment ends at land node i: W(i-1) and not R(i)
     // so the lanes decide them node by node, and a prefix count of the
-    // starts numbers the segments.  This pass: land, R, W, kChunks chunks of
-    // 64 nodes at a time (their loads in flight together).
-    double hlast = 0.0;
-    for (int c = 0; c < n; c += 64 * kChunks) {
-        double h[kChunks], hp[kChunks];
-#pragma unroll
-        for (int u = 0; u < kChunks; u++) {
-            const int i = c + 1 + lane + 64 * u;
-            h[u] = i <= n ? cel[at(i)] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < kChunks; u++) {  // c(i-1): the lane before, or the chunk before's last
-            const double left = u == 0 ? hlast : lane_value(h[u - 1], 63);
-            const double up = __shfl_up(h[u], 1);
-            hp[u] = lane == 0 ? left : up;
-        }
-        hlast = lane_value(h[kChunks - 1], 63);
-#pragma unroll
-        for (int u = 0; u < kChunks; u++) {
-            const int i = c + 1 + lane + 64 * u;
-            if (i > n) break;
-            const bool land = h[u] <= celmin;
-            bool r = false;
-            if (i >= 2 && land && !(hp[u] <= celmin)) {
-                const double flood = (hp[u] * hp[u]) / G - dep[at(i - 1)] + dep[at(i)];
-                r = (flood > ground) && (nv[at(i - 1)] >= 0);
-            }
-            w.ix(kFL, i) = (land ? kLand : 0) | (r ? kR : 0) | ((!land || r) ? kW : 0);
-        }
-    }
-    pass_done();
-    // S, L, M, E of node i and of node i+1 (whose left expansion floods i
-    // and whose merge removes i's end), nodes still dry, and the segment
-    // numbers: starts not merged, counted along the line 64 nodes at a time;
-    // a node is inside a segment (lghost < i < rghost) when more segments
-    // have started than ended up to it, or when the start after it floods it,
-    // never at the line's ends.  A lane rewrites its node's flags keeping
-    // land, R and W, the only bits the neighbours read here; the flags of
-    // kChunks chunks are loaded together.
+    // starts numbers the segments.  One pass, kChunks chunks of 64 nodes
+    // at a time (their loads in flight together): each lane decides land, R
+    // and W of its node, takes those of its neighbours from the neighbour
+    // lanes (the chunk before's last two nodes, the chunk after's first,
+    // carried across chunks), then S, L, M, E of its node and of node i+1
+    // (whose left expansion floods i and whose merge removes i's end), the
+    // nodes still dry, and the segment numbers: starts not merged, counted
+    // along the line; a node is inside a segment (lghost < i < rghost) when
+    // more segments have started than ended up to it, or when the start
+    // after it floods it, never at the line's ends.
     int kseg = 0, kend = 0;
+    double hlast = 0.0;  // c of the node before the batch
+    int fl62 = 0, fl63 = 0;  // land, R, W of the two nodes before the batch
     for (int c0 = 0; c0 < n; c0 += 64 * kChunks) {
-        int f0[kChunks], fm1[kChunks], fm2[kChunks], fp1[kChunks];
+        double h[kChunks];
+        int fa[kChunks], fm1[kChunks], fm2[kChunks], fp1[kChunks];
 #pragma unroll
         for (int u = 0; u < kChunks; u++) {
             const int i = c0 + 1 + lane + 64 * u;
-            f0[u] = i <= n ? w.ix(kFL, i) : 0;
-            fm1[u] = (i >= 2 && i <= n) ? w.ix(kFL, i - 1) : 0;
-            fm2[u] = (i >= 3 && i <= n) ? w.ix(kFL, i - 2) : 0;
-            fp1[u] = i + 1 <= n ? w.ix(kFL, i + 1) : 0;
+            h[u] = i <= n ? cel[at(i)] : 0.0;
         }
+        const int inext = c0 + 64 * kChunks + 1;  // the node after the batch
+        const bool land_next = inext <= n && cel[at(inext)] <= celmin;
+#pragma unroll
+        for (int u = 0; u < kChunks; u++) {  // land, R, W
+            const int i = c0 + 1 + lane + 64 * u;
+            const double left = u == 0 ? hlast : lane_value(h[u - 1], 63);
+            const double up = __shfl_up(h[u], 1);
+            const double hp = lane == 0 ? left : up;  // c(i-1)
+            const bool land = h[u] <= celmin;
+            bool r = false;
+            if (i >= 2 && i <= n && land && !(hp <= celmin)) {
+                const double flood = (hp * hp) / G - dep[at(i - 1)] + dep[at(i)];
+                r = (flood > ground) && (nv[at(i - 1)] >= 0);
+            }
+            fa[u] = (land ? kLand : 0) | (r ? kR : 0) | ((!land || r) ? kW : 0);
+        }
+#pragma unroll
+        for (int u = 0; u < kChunks; u++) {  // the neighbours' land, R, W
+            const int i = c0 + 1 + lane + 64 * u;
+            const int p62 = u == 0 ? fl62 : __builtin_amdgcn_readlane(fa[u - 1], 62);
+            const int p63 = u == 0 ? fl63 : __builtin_amdgcn_readlane(fa[u - 1], 63);
+            const int nx0 = u + 1 < kChunks ? __builtin_amdgcn_readlane(fa[u + 1 < kChunks ? u + 1 : u], 0)
+                                            : (land_next ? kLand : 0);
+            const int up1 = __shfl_up(fa[u], 1), up2 = __shfl_up(fa[u], 2), dn1 = __shfl_down(fa[u], 1);
+            fm1[u] = i >= 2 ? (lane == 0 ? p63 : up1) : 0;
+            fm2[u] = i >= 3 ? (lane == 0 ? p62 : lane == 1 ? p63 : up2) : 0;
+            fp1[u] = i + 1 <= n ? (lane == 63 ? nx0 : dn1) : 0;
+        }
+        hlast = lane_value(h[kChunks - 1], 63);
+        fl62 = __builtin_amdgcn_readlane(fa[kChunks - 1], 62);
+        fl63 = __builtin_amdgcn_readlane(fa[kChunks - 1], 63);
 #pragma unroll
         for (int u = 0; u < kChunks; u++) {
             const int c = c0 + 64 * u;
@@ -191,12 +193,12 @@ __global__ __launch_bounds__(64 * kLineWaves) void line64_kernel(const Line64 a)
             int fl = 0;
             bool st = false, en = false, fbn = false;
             if (i <= n) {
-                fl = f0[u] & (kLand | kR | kW);
+                fl = fa[u];
                 const bool land = fl & kLand;
                 const bool s0 = i == 1 ? !land : (!land && (fm1[u] & kLand) && !(fm1[u] & kR));
                 bool l0 = false;
                 if (s0 && i >= 2) {
-                    const double hi = cel[at(i)];
+                    const double hi = h[u];
                     const double flood = (hi * hi) / G - dep[at(i)] + dep[at(i - 1)];
                     l0 = (flood > ground) && (nv[at(i)] <= 0);
                 }
