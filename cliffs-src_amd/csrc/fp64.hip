@@ -461,6 +461,40 @@ __global__ __launch_bounds__(256) void transpose64_kernel(const double* __restri
     }
 }
 
+// The y-sweep's column-major c, xvel, yvel back into the (nx, ny) state,
+// through 32 x 32 LDS tiles, and the step's running maxima
+// (cliffs_main.f:347-355) from the same values: the sweep and the maxima
+// of one step in one pass over the state.
+__global__ __launch_bounds__(256) void untranspose_max64_kernel(const double* __restrict__ cT,
+                                                                const double* __restrict__ xT,
+                                                                const double* __restrict__ yT,
+                                                                double* __restrict__ cel, double* __restrict__ xv,
+                                                                double* __restrict__ yv, double* __restrict__ hm,
+                                                                double* __restrict__ vm, int rows, int cols) {
+    __shared__ double tc[32][33], tu[32][33], tw[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8 threads
+    for (int k = ty; k < 32; k += 8) {
+        const int r = r0 + k, c = c0 + tx;
+        if (r < rows && c < cols) {
+            const size_t g = (size_t)r * cols + c;
+            tc[k][tx] = cT[g], tu[k][tx] = xT[g], tw[k][tx] = yT[g];
+        }
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int c = c0 + k, r = r0 + tx;
+        if (r < rows && c < cols) {
+            const size_t o = (size_t)c * rows + r;
+            const double h = tc[tx][k], u = tu[tx][k], v = tw[tx][k];
+            cel[o] = h, xv[o] = u, yv[o] = v;
+            if (h > hm[o]) hm[o] = h;
+            const double aa = u * u + v * v;
+            if (aa > vm[o]) vm[o] = aa;
+        }
+    }
+}
+
 // running maxima (cliffs_main.f:347-355)
 __global__ void maxwave64_kernel(const double* __restrict__ cel, const double* __restrict__ xv,
                                  const double* __restrict__ yv, double* __restrict__ hm,
@@ -667,7 +701,9 @@ hipError_t transpose64(const double* in, double* out, int rows, int cols, hipStr
     return hipGetLastError();
 }
 
-int sweep64(cliffs64_handle* h, int iu, int first, int last, const double* e1, const double* e2, int mx) {
+// with_max: a y-sweep also updates the running maxima (the step's last sweep)
+int sweep64(cliffs64_handle* h, int iu, int first, int last, const double* e1, const double* e2, int mx,
+            bool with_max = false) {
     const int n = iu == 1 ? h->nx : h->ny;
     const int nx = h->nx, ny = h->ny;
     if (iu == 2)  // columns as contiguous lines
@@ -700,10 +736,15 @@ int sweep64(cliffs64_handle* h, int iu, int first, int last, const double* e1, c
     hipLaunchKernelGGL(line64_kernel, dim3((a.nlines + kLineWaves - 1) / kLineWaves), dim3(64 * kLineWaves), 0,
                        h->st, a);
     HIPCHK64(hipGetLastError());
-    if (iu == 2)
+    if (iu == 2 && with_max) {
+        hipLaunchKernelGGL(untranspose_max64_kernel, dim3((ny + 31) / 32, (nx + 31) / 32), dim3(256), 0, h->st,
+                           h->celT, h->xvelT, h->yvelT, h->cel, h->xvel, h->yvel, h->hmax, h->velmax, nx, ny);
+        HIPCHK64(hipGetLastError());
+    } else if (iu == 2) {
         for (int f = 0; f < 3; f++)
             HIPCHK64(transpose64(f == 0 ? h->celT : f == 1 ? h->xvelT : h->yvelT,
                                  f == 0 ? h->cel : f == 1 ? h->xvel : h->yvel, nx, ny, h->st));
+    }
     return CLIFFS_OK;
 }
 
@@ -922,8 +963,11 @@ int cliffs64_run(cliffs64_handle* h, int nsteps) {
     for (int s = 0; s < nsteps; s++) {
         int rc;
         if (h->xrun && (rc = sweep64(h, 1, h->jj1, h->jj2, nullptr, nullptr, h->mx))) return rc;
-        if (h->yrun && (rc = sweep64(h, 2, h->ii1, h->ii2, nullptr, nullptr, h->mx))) return rc;
-        if ((rc = maxwave64(h))) return rc;
+        if (h->yrun) {
+            if ((rc = sweep64(h, 2, h->ii1, h->ii2, nullptr, nullptr, h->mx, true))) return rc;
+        } else if ((rc = maxwave64(h))) {
+            return rc;
+        }
     }
     HIPCHK64(hipStreamSynchronize(h->st));
     return CLIFFS_OK;
